@@ -1,0 +1,162 @@
+"""Benchmark: restarted Halpern PDHG iterations per second on one MI355X.
+
+Workload: the plain-FP64 single solve at relaxation scale (400k rows × 900k
+columns, ~8M nonzeros, Zipf(1.3) column counts, 20 dense rows of 20k-60k
+nonzeros; seeded synthetic data). A step is one PDHG iteration of the full
+driver, including the KKT/restart/infeasibility check work every 64
+iterations; the timed region holds K steps with the problem already resident
+in HBM.
+
+Also reported, on one JSON line:
+  roofline     the dominant kernel's algorithmic bytes per launch over its
+               average HIP-event duration, against the 8 TB/s HBM peak;
+  cpu_baseline the oracle (a C restatement of the reference's CPU backend and
+               driver) on the host cores, over a bounded sample of the same
+               workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "LPs solved/s + p50/p99 latency at 1e-6 KKT vs plain FP64; iters/s, HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(m: int, n: int, nnz: int) -> dict:
+    """Algorithmic HBM bytes per launch of the fused iteration kernels (FP64
+    values and vectors, int32 indices; each gathered vector counted once)."""
+    return {
+        # Aᵀ side: values+indices, row pointers, gather y, read x c lv uv x_anchor, write x_bar x
+        "sp_spmv_t_primal_halpern_f64": 12 * nnz + 4 * (n + 1) + 8 * m + 40 * n + 16 * n,
+        # A side: values+indices, row pointers, gather x_bar, read y lc uc y_anchor, write y
+        "sp_spmv_dual_halpern_f64": 12 * nnz + 4 * (m + 1) + 8 * n + 32 * m + 8 * m,
+        "sp_spmv_f64": 12 * nnz + 4 * (m + 1) + 8 * n + 8 * m,
+    }
+
+
+def cpu_baseline(problem, seconds_budget: float) -> dict:
+    from oracle import oracle as O
+
+    cores = min(os.cpu_count() or 1, 16)
+    O.set_threads(cores)
+    params = O.default_params(tolerance=0.0)
+    # One 64-iteration window first, to size a sample of ~seconds_budget.
+    t64, scale_s = O.time_iterations(problem, 64, params)
+    k = max(64, int(seconds_budget / max(t64, 1e-9) / 64) * 64)
+    k = min(k, 64 * 8)
+    if k > 64:
+        t, _ = O.time_iterations(problem, k, params)
+    else:
+        t = t64
+    return {"value": k / t, "unit": "iters/s", "cores": cores, "kind": "port",
+            "sample": f"{k} iterations of the same instance (check work included, power iteration and "
+                      f"{scale_s:.1f}s of host scaling excluded), oracle C restatement with {cores} OpenMP threads"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1280)
+    ap.add_argument("--warmup", type=int, default=128)
+    ap.add_argument("--scale", type=float, default=1.0, help="shrink the workload (tests only)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported (this backend has no "
+              "multi-GPU path; LPs shard across GPUs as independent instances)", file=sys.stderr)
+        return 2
+
+    from saddlepoint_amd import generate
+    from saddlepoint_amd.backend import HipBackend, Solver, default_params
+
+    t0 = time.time()
+    p = generate.config1(scale=args.scale)
+    m, n, nnz = p.nrows, p.ncols, p.a.nnz
+    log(f"generated config1 m={m} n={n} nnz={nnz} in {time.time() - t0:.1f}s")
+
+    b = HipBackend(0)
+    log(f"backend: {b.name()}")
+    t0 = time.time()
+    s = Solver(b, p, default_params(tolerance=1e-6, fused=1))
+    log(f"solver ready (scale + load + power iteration) in {time.time() - t0:.1f}s")
+    s.advance(args.warmup)
+    b.sync()
+    log(f"warmup {args.warmup} iterations done")
+
+    b.profile(True)
+    it0 = s.result(with_solution=False).iterations
+    t0 = time.perf_counter()
+    done = s.advance(args.steps)
+    b.sync()
+    t1 = time.perf_counter()
+    prof = b.profile_read()
+    b.profile(False)
+    res = s.result(with_solution=False)
+    steps = res.iterations - it0
+    elapsed = t1 - t0
+    log(f"timed {steps} iterations in {elapsed:.3f}s (terminated={done}, status={res.status})")
+
+    kb = kernel_bytes(m, n, nnz)
+    per_kernel = {}
+    for name, (ms, launches) in prof.items():
+        if launches:
+            avg_us = 1e3 * ms / launches
+            entry = {"launches": launches, "avg_us": round(avg_us, 2), "total_ms": round(ms, 3)}
+            if name in kb:
+                entry["gbs"] = round(kb[name] / (avg_us * 1e-6) / 1e9, 1)
+            per_kernel[name] = entry
+    dom = max((k for k in per_kernel if k in kb), key=lambda k: per_kernel[k]["total_ms"])
+    ach = per_kernel[dom]["gbs"]
+    b_iter = 24 * nnz + 52 * m + 68 * n
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": kb[dom]}
+
+    out = {
+        "metric": METRIC,
+        "value": round(steps / elapsed, 2),
+        "unit": "iters/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded generator; random instance of the named shape)",
+        "config": {"workload": "config1 plain-FP64 single solve: 400k x 900k LP, ~8M nnz, Zipf(1.3) columns + "
+                               "20 dense rows", "m": m, "n": n, "nnz": nnz, "scale": args.scale,
+                   "parallelism": "single GPU"},
+        "iteration_gbs": round(b_iter * steps / elapsed / 1e9, 1),
+        "iteration_roofline_frac": round(b_iter * steps / elapsed / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": roofline,
+        "kernels": per_kernel,
+    }
+    if not args.no_cpu:
+        log("timing the CPU baseline ...")
+        out["cpu_baseline"] = cpu_baseline(p, args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+    s.close()
+    b.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

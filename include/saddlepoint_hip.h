@@ -1,0 +1,209 @@
+/*
+ * saddlepoint_hip: an MI355X (gfx950) backend for the saddlepoint PDHG LP solver.
+ *
+ * C ABI. Each Backend method of the reference is one function here, one to
+ * one, and keeps the method's argument meaning
+ * (crates/saddlepoint-core/src/backend.rs:43-137). Handles are opaque; all
+ * arrays are plain host pointers with explicit lengths. Every function returns
+ * SP_OK (0) or an SP_ERR_* code and leaves a message for sp_last_error().
+ *
+ * Threading: a backend handle, and every problem and vector made from it, is
+ * used by one host thread at a time (the reference's CudaBackend is not Sync
+ * either, crates/saddlepoint-cuda/src/backend.rs:99). Work is issued on one
+ * HIP stream in call order; functions that return scalars or host data block
+ * until that data is ready, as in the reference.
+ */
+#ifndef SADDLEPOINT_HIP_H
+#define SADDLEPOINT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SP_OK = 0,
+    SP_ERR_UNAVAILABLE = 1, /* no device / code object (CudaError::Unavailable, backend.rs:37-39) */
+    SP_ERR_HIP = 2,         /* a HIP runtime call failed (CudaError::Driver) */
+    SP_ERR_SHAPE = 3,       /* length mismatch (the reference panics, sparse.rs:156-157) */
+    SP_ERR_TOO_LARGE = 4,   /* beyond 32-bit sparse indices (CudaError::TooLarge, sparse.rs:60-62) */
+    SP_ERR_INVALID = 5,     /* null handle or bad argument */
+    SP_ERR_PROBLEM = 6,     /* ProblemError from validate() (problem.rs:147-194) */
+    SP_ERR_NUMERICAL = 7,   /* SolveError::Numerical (solver.rs:230-232, 284-286) */
+    SP_ERR_MEMORY = 8       /* out of device memory, or an admission budget exceeded */
+};
+
+typedef struct sp_backend sp_backend;
+typedef struct sp_problem sp_problem;
+typedef struct sp_vec sp_vec;
+
+/* PrimalStats / DualStats / RayStats (core/src/backend.rs:14-41). */
+typedef struct { double residual_sq, objective; } sp_primal_stats_t;
+typedef struct { double residual_sq, objective; } sp_dual_stats_t;
+typedef struct { double objective, violation_sq, norm_sq; } sp_ray_stats_t;
+
+/* A CSR matrix on the host: CsrMatrix (core/src/problem.rs:16-24), usize
+ * indices as int64. Columns sorted within each row. */
+typedef struct {
+    int64_t nrows, ncols, nnz;
+    const int64_t *row_ptr; /* nrows + 1 */
+    const int64_t *col_idx; /* nnz */
+    const double *values;   /* nnz */
+} sp_csr_view;
+
+/* ScaledProblem (core/src/scaling.rs:37-50), the input of Backend::load. */
+typedef struct {
+    sp_csr_view a, at;
+    const double *c, *lc, *uc, *lv, *uv; /* scaled: c, lv, uv length n; lc, uc length m */
+    const double *row_scale;             /* m */
+    const double *col_scale;             /* n */
+    double bound_scale, obj_scale;
+} sp_scaled_view;
+
+/* LpProblem (core/src/problem.rs:131-145): the input of solve(). */
+typedef struct {
+    sp_csr_view a;
+    const double *c, *lc, *uc, *lv, *uv;
+    double c0;
+    int maximize;
+} sp_lp_view;
+
+/* ---- lifecycle ----------------------------------------------------------- */
+
+/* CudaBackend::new(ordinal) (cuda/src/backend.rs:129-140). Loads the gfx950
+ * code object that sits next to this library. */
+int sp_backend_create(int ordinal, sp_backend **out);
+void sp_backend_destroy(sp_backend *b);
+/* Backend::name (backend.rs:48). The string lives as long as the backend. */
+const char *sp_backend_name(const sp_backend *b);
+/* Message of the last failed call on this backend ("" if none). b may be NULL
+ * for failures of sp_backend_create. */
+const char *sp_last_error(const sp_backend *b);
+
+/* ---- problem -------------------------------------------------------------- */
+
+/* Backend::load (backend.rs:51; cuda/src/backend.rs:229-248): uploads A', A'ᵀ,
+ * the scaled vectors and the residual factors (scaling.rs:124-131). */
+int sp_problem_load(sp_backend *b, const sp_scaled_view *p, sp_problem **out);
+void sp_problem_free(sp_problem *p);
+
+/* ---- vectors (backend.rs:53-55) --------------------------------------------- */
+
+int sp_vec_upload(sp_backend *b, const double *data, int64_t len, sp_vec **out);
+/* A zero vector, without a host copy (faster than uploading zeros). */
+int sp_vec_zeros(sp_backend *b, int64_t len, sp_vec **out);
+int sp_vec_download(sp_backend *b, const sp_vec *v, double *out, int64_t len);
+int sp_vec_copy(sp_backend *b, const sp_vec *src, sp_vec *dst);
+int64_t sp_vec_len(const sp_vec *v);
+void sp_vec_free(sp_vec *v);
+
+/* ---- compute: one function per Backend method (backend.rs:57-136) ----------- */
+
+int sp_spmv(sp_backend *b, const sp_problem *p, const sp_vec *x, sp_vec *out);   /* :58 */
+int sp_spmv_t(sp_backend *b, const sp_problem *p, const sp_vec *y, sp_vec *out); /* :60 */
+int sp_primal_step(sp_backend *b, const sp_problem *p, const sp_vec *x, const sp_vec *aty, double tau,
+                   sp_vec *x_t, sp_vec *x_bar);                                          /* :63-71 */
+int sp_dual_step(sp_backend *b, const sp_problem *p, const sp_vec *y, const sp_vec *ax_bar, double sigma,
+                 sp_vec *y_t);                                                           /* :74-81 */
+int sp_halpern_step(sp_backend *b, sp_vec *z, const sp_vec *z_t, const sp_vec *anchor, double w,
+                    double rho);                                                         /* :85-92 */
+int sp_distance_sq(sp_backend *b, const sp_vec *a, const sp_vec *bv, double *out);      /* :95 */
+int sp_sub(sp_backend *b, const sp_vec *a, const sp_vec *bv, sp_vec *out);              /* :97 */
+int sp_norm_sq(sp_backend *b, const sp_vec *v, double *out);                            /* :99 */
+int sp_dot(sp_backend *b, const sp_vec *a, const sp_vec *bv, double *out);              /* :101 */
+int sp_scale(sp_backend *b, sp_vec *v, double alpha);                                   /* :103 */
+int sp_primal_stats(sp_backend *b, const sp_problem *p, const sp_vec *x, const sp_vec *ax,
+                    sp_primal_stats_t *out);                                               /* :106 */
+int sp_dual_stats(sp_backend *b, const sp_problem *p, const sp_vec *y, const sp_vec *aty,
+                  sp_dual_stats_t *out);                                                   /* :108 */
+int sp_step_dual_residual_sq(sp_backend *b, const sp_problem *p, const sp_vec *x, const sp_vec *x_t,
+                             const sp_vec *aty, const sp_vec *aty_t, double tau, double *out); /* :113-121 */
+int sp_primal_ray_stats(sp_backend *b, const sp_problem *p, const sp_vec *dx, const sp_vec *adx,
+                        sp_ray_stats_t *out);                                              /* :124-129 */
+int sp_dual_ray_stats(sp_backend *b, const sp_problem *p, const sp_vec *dy, const sp_vec *atdy,
+                      sp_ray_stats_t *out);                                                /* :131-136 */
+
+/* ---- the PDHG driver over this backend ------------------------------------------ */
+
+/* SolverParams (core/src/solver.rs:25-93) plus this backend's execution
+ * options. Fill with sp_default_params. */
+typedef struct {
+    double tolerance;               /* 1e-6 */
+    double infeasibility_tolerance; /* 1e-8 */
+    uint64_t max_iterations;        /* 1e6 */
+    double time_limit_s;            /* < 0: none */
+    uint64_t check_every;           /* 64 */
+    double step_size_factor;        /* 0.998 */
+    double reflection;              /* 1.0 */
+    double restart_sufficient, restart_necessary, restart_artificial; /* 0.2, 0.5, 0.36 */
+    double pid_kp, pid_ki, pid_kd, pid_discount;                      /* 0.99, 0.01, 0, 0.3 */
+    int64_t power_iterations;                                         /* 200 */
+    int64_t geometric_mean_passes;                                    /* 12 (ScalingParams, scaling.rs:25-34) */
+    int64_t ruiz_iterations;                                          /* 10 */
+    int has_pock_chambolle;                                           /* 1 */
+    double pock_chambolle_alpha;                                      /* 1.0 */
+    int bound_objective_rescaling;                                    /* 1 */
+    int termination;                                                  /* 0 projected, 1 step-implied */
+    /* Execution options (no counterpart in the reference's SolverParams). */
+    int fused; /* 1: between checks, one fused Aᵀ-SpMV+primal+Halpern and one fused
+                  A-SpMV+dual+Halpern kernel per iteration (bitwise equal to 0) */
+    int log_every_checks; /* 0 = silent (solver.rs:49-50) */
+} sp_params;
+
+/* Status (solver.rs:95-102). */
+enum { SP_OPTIMAL = 0, SP_PRIMAL_INFEASIBLE = 1, SP_DUAL_INFEASIBLE = 2, SP_ITERATION_LIMIT = 3, SP_TIME_LIMIT = 4 };
+
+/* Solution (solver.rs:104-121) without x and y, which go to caller buffers. */
+typedef struct {
+    int status;
+    double objective, dual_objective, primal_residual, dual_residual, gap;
+    uint64_t iterations, restarts;
+    double solve_time_s; /* from the start of run() (solver.rs:312, 436) */
+    double total_time_s; /* scaling and upload included */
+    double setup_time_s; /* scaling, transpose, upload */
+    double final_omega;
+} sp_solution;
+
+void sp_default_params(sp_params *p);
+
+/* solve() (solver.rs:154-157) on this backend: validate, scale on the host,
+ * load, run the restarted Halpern PDHG loop, unscale. x_out (n) and y_out (m)
+ * may be NULL. */
+int sp_solve(sp_backend *b, const sp_lp_view *lp, const sp_params *params, sp_solution *out, double *x_out,
+             double *y_out);
+
+/* The same driver as an object that runs in slices, for callers that time
+ * or interleave iterations. sp_solver_create does solve()'s work up to the
+ * loop (validate, scale, load, power iteration); sp_solver_advance runs at
+ * most `iterations` more iterations and sets *done once the loop has ended;
+ * sp_solver_result reports the Solution (status -1 while still running). */
+typedef struct sp_solver sp_solver;
+int sp_solver_create(sp_backend *b, const sp_lp_view *lp, const sp_params *params, sp_solver **out);
+int sp_solver_advance(sp_solver *s, uint64_t iterations, int *done);
+int sp_solver_result(sp_solver *s, sp_solution *out, double *x_out, double *y_out);
+void sp_solver_free(sp_solver *s);
+
+/* ---- measurement ------------------------------------------------------------- */
+
+/* While enabled, every kernel launch on the backend's stream is bracketed by
+ * two HIP events; enabling clears the totals. */
+int sp_backend_profile(sp_backend *b, int enable);
+/* Totals for kernel `index` (0-based, in load order): its name, summed
+ * event time in milliseconds and launch count. SP_ERR_INVALID past the end. */
+int sp_backend_profile_read(sp_backend *b, int index, char *name, int name_len, double *total_ms,
+                            int64_t *launches);
+/* Waits for all work queued on the backend's stream. */
+int sp_backend_sync(sp_backend *b);
+
+/* ScaledProblem::new (scaling.rs:53-103) on the host, for callers that load
+ * themselves. Arrays are caller-allocated: a_values[nnz] (the scaled A' in
+ * the input's sparsity), c/lv/uv/col_scale[n], lc/uc/row_scale[m]. */
+int sp_scale_problem(const sp_lp_view *lp, const sp_params *params, double *a_values, double *c, double *lc,
+                     double *uc, double *lv, double *uv, double *row_scale, double *col_scale,
+                     double *bound_scale, double *obj_scale);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,558 @@
+// HIP backend implementation: device memory, the gfx950 code object, launch
+// shapes and the Backend methods (crates/saddlepoint-core/src/backend.rs:43-137,
+// behaviour of crates/saddlepoint-cpu/src/lib.rs:101-260).
+#include "backend.hpp"
+
+#include <dlfcn.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstring>
+
+namespace sp {
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+            throw Error(SP_ERR_MEMORY, std::string(what) + ": " + hipGetErrorString(e));
+        throw Error(SP_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+
+namespace {
+
+constexpr int kWG = 256;
+constexpr int kChunk = 4096;  // SP_CHUNK in kernels.hip
+constexpr int kStreamNnzF64 = 2048;  // SP_NNZ_F64 in kernels.hip
+
+template <class T>
+T* dmalloc(size_t count) {
+    void* p = nullptr;
+    if (count == 0) count = 1;
+    hip_check(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
+    return static_cast<T*>(p);
+}
+
+template <class T>
+T* dupload(hipStream_t s, const T* src, size_t count) {
+    T* d = dmalloc<T>(count);
+    if (count) hip_check(hipMemcpyAsync(d, src, count * sizeof(T), hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
+    return d;
+}
+
+void dfree(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+// Launches a kernel of the code object; every argument is passed by address,
+// as hipModuleLaunchKernel expects, so argument types must match the kernel's.
+hipEvent_t take_event(sp_backend* b) {
+    if (!b->event_pool.empty()) {
+        hipEvent_t e = b->event_pool.back();
+        b->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    hip_check(hipEventCreate(&e), "hipEventCreate");
+    return e;
+}
+
+template <class... Args>
+void launch(sp_backend* b, hipFunction_t f, unsigned grid, Args... args) {
+    void* params[] = {static_cast<void*>(&args)...};
+    if (!b->profiling) {
+        hip_check(hipModuleLaunchKernel(f, grid, 1, 1, kWG, 1, 1, 0, b->stream, params, nullptr),
+                  "hipModuleLaunchKernel");
+        return;
+    }
+    int id = 0;
+    while (id < (int)b->kernel_fns.size() && b->kernel_fns[(size_t)id] != f) ++id;
+    sp_backend::Pending pe{id, take_event(b), take_event(b)};
+    hip_check(hipEventRecord(pe.start, b->stream), "hipEventRecord");
+    hip_check(hipModuleLaunchKernel(f, grid, 1, 1, kWG, 1, 1, 0, b->stream, params, nullptr),
+              "hipModuleLaunchKernel");
+    hip_check(hipEventRecord(pe.stop, b->stream), "hipEventRecord");
+    b->pending.push_back(pe);
+    if (b->pending.size() >= 4096) profile_collect(b);
+}
+
+unsigned ew_grid(int64_t n) {
+    int64_t g = (n + kWG - 1) / kWG;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, 8192));
+}
+
+void need_len(const sp_vec* v, int64_t len, const char* what) {
+    if (!v) throw Error(SP_ERR_INVALID, std::string(what) + ": null vector");
+    if (v->len != len)
+        throw Error(SP_ERR_SHAPE, std::string(what) + ": length " + std::to_string(v->len) + ", expected " +
+                                      std::to_string(len));
+}
+
+// Uploads one CSR matrix with int32 indices and builds its SpMV plan.
+void upload_csr(sp_backend* b, DevCsr& d, const int64_t* row_ptr, const int64_t* col_idx, const double* values,
+                int64_t nrows, int64_t ncols, int64_t nnz) {
+    if (nrows > INT_MAX - 1 || ncols > INT_MAX - 1 || nnz > INT_MAX)
+        throw Error(SP_ERR_TOO_LARGE, "problem too large for 32-bit sparse indices");
+    d.nrows = (int)nrows;
+    d.ncols = (int)ncols;
+    d.nnz = nnz;
+    std::vector<int> rp((size_t)nrows + 1), ci((size_t)nnz);
+    for (int64_t i = 0; i <= nrows; ++i) rp[(size_t)i] = (int)row_ptr[i];
+    for (int64_t k = 0; k < nnz; ++k) ci[(size_t)k] = (int)col_idx[k];
+    d.rp = dupload(b->stream, rp.data(), rp.size());
+    d.ci = dupload(b->stream, ci.data(), ci.size());
+    d.va = dupload(b->stream, values, (size_t)nnz);
+    std::vector<Desc> desc = plan_spmv(row_ptr, nrows, d.tuning, &d.npartials);
+    d.nblocks = (int)desc.size();
+    d.desc = dupload(b->stream, desc.data(), desc.size());
+    d.partials = dmalloc<double>((size_t)std::max(1, d.npartials));
+    d.tickets = dmalloc<unsigned>((size_t)std::max(1, d.npartials));
+    hip_check(hipMemsetAsync(d.tickets, 0, sizeof(unsigned) * (size_t)std::max(1, d.npartials), b->stream),
+              "hipMemsetAsync");
+    // The host staging vectors die at return: finish the copies first.
+    hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+}
+
+void free_csr(DevCsr& d) {
+    dfree(d.rp), dfree(d.ci), dfree(d.va), dfree(d.desc), dfree(d.partials), dfree(d.tickets);
+    d = DevCsr{};
+}
+
+}  // namespace
+
+std::vector<Desc> plan_spmv(const int64_t* rp, int64_t nrows, const SpmvTuning& t, int* npartials) {
+    std::vector<Desc> out;
+    int np = 0;
+    const int64_t block_nnz = std::min<int64_t>(t.block_nnz, kStreamNnzF64);
+    const int64_t long_row = std::min<int64_t>(t.long_row, block_nnz);
+    int64_t r = 0;
+    while (r < nrows) {
+        const int64_t len = rp[r + 1] - rp[r];
+        if (len > long_row) {
+            const int64_t nch = (len + kChunk - 1) / kChunk;
+            if (nch == 1) {
+                out.push_back(Desc{(int)r, -1, (int)rp[r], (int)rp[r + 1]});
+            } else {
+                for (int64_t c = 0; c < nch; ++c) {
+                    const int64_t k0 = rp[r] + c * kChunk, k1 = std::min(rp[r] + (c + 1) * kChunk, rp[r + 1]);
+                    out.push_back(Desc{(int)r, -1 - (np + (int)c), (int)k0, (int)k1});
+                }
+                np += (int)nch;
+            }
+            ++r;
+            continue;
+        }
+        const int64_t start = r, k0 = rp[r];
+        while (r < nrows && rp[r + 1] - rp[r] <= long_row && rp[r + 1] - k0 <= block_nnz && r - start < t.max_rows)
+            ++r;
+        out.push_back(Desc{(int)start, (int)r, (int)k0, (int)rp[r]});
+    }
+    *npartials = np;
+    return out;
+}
+
+void profile_collect(sp_backend* b) {
+    if (b->pending.empty()) return;
+    hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    for (const auto& pe : b->pending) {
+        float ms = 0.0f;
+        hip_check(hipEventElapsedTime(&ms, pe.start, pe.stop), "hipEventElapsedTime");
+        if (pe.kernel < (int)b->kernel_ms.size()) {
+            b->kernel_ms[(size_t)pe.kernel] += ms;
+            b->kernel_launches[(size_t)pe.kernel] += 1;
+        }
+        b->event_pool.push_back(pe.start);
+        b->event_pool.push_back(pe.stop);
+    }
+    b->pending.clear();
+}
+
+void free_problem(sp_problem* p) {
+    if (!p) return;
+    free_csr(p->a);
+    free_csr(p->at);
+    for (double* v : {p->c, p->lc, p->uc, p->lv, p->uv, p->pf, p->df}) dfree(v);
+    delete p;
+}
+
+void free_vec(sp_vec* v) {
+    if (!v) return;
+    dfree(v->d);
+    delete v;
+}
+
+sp_problem* Hip::load(const ScaledProblem& s) {
+    sp_scaled_view v{};
+    v.a = {s.a.nrows, s.a.ncols, s.a.nnz(), s.a.row_ptr.data(), s.a.col_idx.data(), s.a.values.data()};
+    v.at = {s.at.nrows, s.at.ncols, s.at.nnz(), s.at.row_ptr.data(), s.at.col_idx.data(), s.at.values.data()};
+    v.c = s.c.data();
+    v.lc = s.lc.data();
+    v.uc = s.uc.data();
+    v.lv = s.lv.data();
+    v.uv = s.uv.data();
+    v.row_scale = s.row_scale.data();
+    v.col_scale = s.col_scale.data();
+    v.bound_scale = s.bound_scale;
+    v.obj_scale = s.obj_scale;
+    return load_view(v);
+}
+
+sp_problem* Hip::load_view(const sp_scaled_view& v) {
+    const int64_t m = v.a.nrows, n = v.a.ncols;
+    if (v.at.nrows != n || v.at.ncols != m || v.at.nnz != v.a.nnz)
+        throw Error(SP_ERR_SHAPE, "load: A'ᵀ does not match A'");
+    auto* p = new sp_problem;
+    p->b = b;
+    p->m = m;
+    p->n = n;
+    try {
+        upload_csr(b, p->a, v.a.row_ptr, v.a.col_idx, v.a.values, m, n, v.a.nnz);
+        upload_csr(b, p->at, v.at.row_ptr, v.at.col_idx, v.at.values, n, m, v.at.nnz);
+        // Residual factors are formed on the host with the reference's
+        // arithmetic (scaling.rs:124-131) and uploaded, not recomputed.
+        std::vector<double> pf((size_t)m), df((size_t)n);
+        for (int64_t i = 0; i < m; ++i) pf[(size_t)i] = 1.0 / (v.row_scale[i] * v.bound_scale);
+        for (int64_t j = 0; j < n; ++j) df[(size_t)j] = 1.0 / (v.col_scale[j] * v.obj_scale);
+        p->c = dupload(b->stream, v.c, (size_t)n);
+        p->lv = dupload(b->stream, v.lv, (size_t)n);
+        p->uv = dupload(b->stream, v.uv, (size_t)n);
+        p->lc = dupload(b->stream, v.lc, (size_t)m);
+        p->uc = dupload(b->stream, v.uc, (size_t)m);
+        p->pf = dupload(b->stream, pf.data(), (size_t)m);
+        p->df = dupload(b->stream, df.data(), (size_t)n);
+        hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    } catch (...) {
+        free_problem(p);
+        throw;
+    }
+    return p;
+}
+
+sp_vec* Hip::upload(const double* data, int64_t len) {
+    if (len < 0) throw Error(SP_ERR_INVALID, "upload: negative length");
+    auto* v = new sp_vec;
+    v->b = b;
+    v->len = len;
+    try {
+        v->d = dmalloc<double>((size_t)len);
+        if (len) {
+            hip_check(hipMemcpyAsync(v->d, data, sizeof(double) * (size_t)len, hipMemcpyHostToDevice, b->stream),
+                      "hipMemcpyAsync H2D");
+            hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        }
+    } catch (...) {
+        free_vec(v);
+        throw;
+    }
+    return v;
+}
+
+sp_vec* Hip::zeros(int64_t len) {
+    if (len < 0) throw Error(SP_ERR_INVALID, "zeros: negative length");
+    auto* v = new sp_vec;
+    v->b = b;
+    v->len = len;
+    try {
+        v->d = dmalloc<double>((size_t)len);
+        if (len) launch(b, b->k_fill, ew_grid(len), v->d, 0.0, (long long)len);
+    } catch (...) {
+        free_vec(v);
+        throw;
+    }
+    return v;
+}
+
+void Hip::download(const sp_vec* v, double* out, int64_t len) {
+    need_len(v, len, "download");
+    if (len) {
+        hip_check(hipMemcpyAsync(out, v->d, sizeof(double) * (size_t)len, hipMemcpyDeviceToHost, b->stream),
+                  "hipMemcpyAsync D2H");
+    }
+    hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+}
+
+void Hip::copy(const sp_vec* src, sp_vec* dst) {
+    need_len(dst, src ? src->len : -1, "copy");
+    if (src->len)
+        hip_check(hipMemcpyAsync(dst->d, src->d, sizeof(double) * (size_t)src->len, hipMemcpyDeviceToDevice,
+                                 b->stream),
+                  "hipMemcpyAsync D2D");
+}
+
+void Hip::sync() { hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize"); }
+
+// ---- SpMV --------------------------------------------------------------------
+
+void Hip::spmv(const sp_problem* p, const sp_vec* x, sp_vec* out) {
+    need_len(x, p->n, "spmv x");
+    need_len(out, p->m, "spmv out");
+    const DevCsr& a = p->a;
+    if (a.nblocks)
+        launch(b, b->k_spmv, (unsigned)a.nblocks, a.desc, a.rp, a.ci, a.va, (const double*)x->d, a.partials,
+               a.tickets, out->d);
+}
+
+void Hip::spmv_t(const sp_problem* p, const sp_vec* y, sp_vec* out) {
+    need_len(y, p->m, "spmv_t y");
+    need_len(out, p->n, "spmv_t out");
+    const DevCsr& a = p->at;
+    if (a.nblocks)
+        launch(b, b->k_spmv, (unsigned)a.nblocks, a.desc, a.rp, a.ci, a.va, (const double*)y->d, a.partials,
+               a.tickets, out->d);
+}
+
+void Hip::spmv_t_primal(const sp_problem* p, const sp_vec* y, const sp_vec* x, double tau, sp_vec* aty,
+                        sp_vec* x_t, sp_vec* x_bar) {
+    need_len(y, p->m, "spmv_t_primal y");
+    for (const sp_vec* v : {x, (const sp_vec*)aty, (const sp_vec*)x_t, (const sp_vec*)x_bar})
+        need_len(v, p->n, "spmv_t_primal");
+    const DevCsr& a = p->at;
+    if (a.nblocks)
+        launch(b, b->k_spmv_t_primal, (unsigned)a.nblocks, a.desc, a.rp, a.ci, a.va, (const double*)y->d,
+               a.partials, a.tickets, (const double*)x->d, (const double*)p->c, (const double*)p->lv,
+               (const double*)p->uv, tau, aty->d, x_t->d, x_bar->d);
+}
+
+void Hip::spmv_dual(const sp_problem* p, const sp_vec* x_bar, const sp_vec* y, double sigma, sp_vec* ax,
+                    sp_vec* y_t) {
+    need_len(x_bar, p->n, "spmv_dual x_bar");
+    for (const sp_vec* v : {y, (const sp_vec*)ax, (const sp_vec*)y_t}) need_len(v, p->m, "spmv_dual");
+    const DevCsr& a = p->a;
+    if (a.nblocks)
+        launch(b, b->k_spmv_dual, (unsigned)a.nblocks, a.desc, a.rp, a.ci, a.va, (const double*)x_bar->d,
+               a.partials, a.tickets, (const double*)y->d, (const double*)p->lc, (const double*)p->uc, sigma,
+               ax->d, y_t->d);
+}
+
+void Hip::plain_iteration(const sp_problem* p, sp_vec* x, sp_vec* y, const sp_vec* x_anchor,
+                          const sp_vec* y_anchor, sp_vec* x_bar, double tau, double sigma, double w, double rho) {
+    const DevCsr& at = p->at;
+    const DevCsr& a = p->a;
+    if (at.nblocks)
+        launch(b, b->k_spmv_t_primal_halpern, (unsigned)at.nblocks, at.desc, at.rp, at.ci, at.va,
+               (const double*)y->d, at.partials, at.tickets, x->d, (const double*)p->c, (const double*)p->lv,
+               (const double*)p->uv, (const double*)x_anchor->d, tau, w, rho, x_bar->d);
+    if (a.nblocks)
+        launch(b, b->k_spmv_dual_halpern, (unsigned)a.nblocks, a.desc, a.rp, a.ci, a.va, (const double*)x_bar->d,
+               a.partials, a.tickets, y->d, (const double*)p->lc, (const double*)p->uc, (const double*)y_anchor->d,
+               sigma, w, rho);
+}
+
+// ---- element-wise ------------------------------------------------------------------
+
+void Hip::primal_step(const sp_problem* p, const sp_vec* x, const sp_vec* aty, double tau, sp_vec* x_t,
+                      sp_vec* x_bar) {
+    for (const sp_vec* v : {x, aty, (const sp_vec*)x_t, (const sp_vec*)x_bar}) need_len(v, p->n, "primal_step");
+    if (p->n)
+        launch(b, b->k_primal_step, ew_grid(p->n), (const double*)x->d, (const double*)aty->d, (const double*)p->c,
+               (const double*)p->lv, (const double*)p->uv, tau, x_t->d, x_bar->d, (long long)p->n);
+}
+
+void Hip::dual_step(const sp_problem* p, const sp_vec* y, const sp_vec* ax_bar, double sigma, sp_vec* y_t) {
+    for (const sp_vec* v : {y, ax_bar, (const sp_vec*)y_t}) need_len(v, p->m, "dual_step");
+    if (p->m)
+        launch(b, b->k_dual_step, ew_grid(p->m), (const double*)y->d, (const double*)ax_bar->d, (const double*)p->lc,
+               (const double*)p->uc, sigma, y_t->d, (long long)p->m);
+}
+
+void Hip::halpern_step(sp_vec* z, const sp_vec* z_t, const sp_vec* anchor, double w, double rho) {
+    if (!z) throw Error(SP_ERR_INVALID, "halpern_step: null vector");
+    need_len(z_t, z->len, "halpern_step z_t");
+    need_len(anchor, z->len, "halpern_step anchor");
+    if (z->len)
+        launch(b, b->k_halpern, ew_grid(z->len), z->d, (const double*)z_t->d, (const double*)anchor->d, w, rho,
+               (long long)z->len);
+}
+
+void Hip::sub(const sp_vec* a, const sp_vec* bv, sp_vec* out) {
+    if (!a) throw Error(SP_ERR_INVALID, "sub: null vector");
+    need_len(bv, a->len, "sub b");
+    need_len(out, a->len, "sub out");
+    if (a->len)
+        launch(b, b->k_sub, ew_grid(a->len), (const double*)a->d, (const double*)bv->d, out->d, (long long)a->len);
+}
+
+void Hip::scale(sp_vec* v, double alpha) {
+    if (!v) throw Error(SP_ERR_INVALID, "scale: null vector");
+    if (v->len) launch(b, b->k_scale, ew_grid(v->len), v->d, alpha, (long long)v->len);
+}
+
+// ---- reductions ------------------------------------------------------------------------
+
+int Hip::reduce_grid(int64_t n) const {
+    // A fixed function of n, so a given call always reduces in the same order.
+    int64_t g = (n + 4 * kWG - 1) / (4 * kWG);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(g, kMaxReduceBlocks));
+}
+
+const double* Hip::read_result(int k) {
+    hip_check(hipMemcpyAsync(b->host_result, b->red_result, sizeof(double) * (size_t)k, hipMemcpyDeviceToHost,
+                             b->stream),
+              "hipMemcpyAsync D2H");
+    hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    return b->host_result;
+}
+
+double Hip::dot(const sp_vec* a, const sp_vec* bv) {
+    if (!a) throw Error(SP_ERR_INVALID, "dot: null vector");
+    need_len(bv, a->len, "dot");
+    launch(b, b->k_dot, (unsigned)reduce_grid(a->len), (const double*)a->d, (const double*)bv->d, (long long)a->len,
+           b->red_partials, b->red_ticket, b->red_result);
+    return read_result(1)[0];
+}
+
+double Hip::norm_sq(const sp_vec* v) { return dot(v, v); }
+
+double Hip::distance_sq(const sp_vec* a, const sp_vec* bv) {
+    if (!a) throw Error(SP_ERR_INVALID, "distance_sq: null vector");
+    need_len(bv, a->len, "distance_sq");
+    launch(b, b->k_dist, (unsigned)reduce_grid(a->len), (const double*)a->d, (const double*)bv->d,
+           (long long)a->len, b->red_partials, b->red_ticket, b->red_result);
+    return read_result(1)[0];
+}
+
+sp_primal_stats_t Hip::primal_stats(const sp_problem* p, const sp_vec* x, const sp_vec* ax) {
+    need_len(x, p->n, "primal_stats x");
+    need_len(ax, p->m, "primal_stats ax");
+    launch(b, b->k_primal_stats, (unsigned)reduce_grid(p->m + p->n), (const double*)ax->d, (const double*)p->lc,
+           (const double*)p->uc, (const double*)p->pf, (long long)p->m, (const double*)p->c, (const double*)x->d,
+           (long long)p->n, b->red_partials, b->red_ticket, b->red_result);
+    const double* r = read_result(2);
+    return sp_primal_stats_t{r[0], r[1]};
+}
+
+sp_dual_stats_t Hip::dual_stats(const sp_problem* p, const sp_vec* y, const sp_vec* aty) {
+    need_len(y, p->m, "dual_stats y");
+    need_len(aty, p->n, "dual_stats aty");
+    launch(b, b->k_dual_stats, (unsigned)reduce_grid(p->m + p->n), (const double*)y->d, (const double*)p->lc,
+           (const double*)p->uc, (long long)p->m, (const double*)p->c, (const double*)aty->d, (const double*)p->lv,
+           (const double*)p->uv, (const double*)p->df, (long long)p->n, b->red_partials, b->red_ticket,
+           b->red_result);
+    const double* r = read_result(3);
+    // cpu/src/lib.rs:223: objective = rows + cols.
+    return sp_dual_stats_t{r[1], r[0] + r[2]};
+}
+
+double Hip::step_dual_residual_sq(const sp_problem* p, const sp_vec* x, const sp_vec* x_t, const sp_vec* aty,
+                                  const sp_vec* aty_t, double tau) {
+    for (const sp_vec* v : {x, x_t, aty, aty_t}) need_len(v, p->n, "step_dual_residual_sq");
+    const double inv_tau = 1.0 / tau;  // cpu/src/lib.rs:235: multiply, don't divide
+    launch(b, b->k_step_dual, (unsigned)reduce_grid(p->n), (const double*)x->d, (const double*)x_t->d,
+           (const double*)aty->d, (const double*)aty_t->d, (const double*)p->df, inv_tau, (long long)p->n,
+           b->red_partials, b->red_ticket, b->red_result);
+    return read_result(1)[0];
+}
+
+sp_ray_stats_t Hip::primal_ray_stats(const sp_problem* p, const sp_vec* dx, const sp_vec* adx) {
+    need_len(dx, p->n, "primal_ray_stats dx");
+    need_len(adx, p->m, "primal_ray_stats adx");
+    launch(b, b->k_primal_ray, (unsigned)reduce_grid(p->m + p->n), (const double*)adx->d, (const double*)p->lc,
+           (const double*)p->uc, (long long)p->m, (const double*)dx->d, (const double*)p->c, (const double*)p->lv,
+           (const double*)p->uv, (long long)p->n, b->red_partials, b->red_ticket, b->red_result);
+    const double* r = read_result(4);
+    // cpu/src/lib.rs:245: violation = rows + cols.
+    return sp_ray_stats_t{r[3], r[0] + r[1], r[2]};
+}
+
+sp_ray_stats_t Hip::dual_ray_stats(const sp_problem* p, const sp_vec* dy, const sp_vec* atdy) {
+    need_len(dy, p->m, "dual_ray_stats dy");
+    need_len(atdy, p->n, "dual_ray_stats atdy");
+    launch(b, b->k_dual_ray, (unsigned)reduce_grid(p->m + p->n), (const double*)dy->d, (const double*)p->lc,
+           (const double*)p->uc, (long long)p->m, (const double*)atdy->d, (const double*)p->lv, (const double*)p->uv,
+           (long long)p->n, b->red_partials, b->red_ticket, b->red_result);
+    const double* r = read_result(5);
+    // cpu/src/lib.rs:258: objective = row_obj + col_obj, violation = row_viol + col_viol.
+    return sp_ray_stats_t{r[1] + r[4], r[0] + r[3], r[2]};
+}
+
+}  // namespace sp
+
+// ---- lifecycle (outside the namespace: used by the C ABI) ----------------------------
+
+namespace {
+
+std::string code_object_path() {
+    if (const char* env = std::getenv("SADDLEPOINT_HIP_CODE_OBJECT")) return env;
+    Dl_info info{};
+    if (dladdr(reinterpret_cast<void*>(&sp::hip_check), &info) && info.dli_fname) {
+        std::string p = info.dli_fname;
+        const size_t slash = p.rfind('/');
+        return (slash == std::string::npos ? std::string(".") : p.substr(0, slash)) + "/saddlepoint_kernels.hsaco";
+    }
+    return "saddlepoint_kernels.hsaco";
+}
+
+}  // namespace
+
+sp_backend* sp_backend_open(int ordinal) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        throw sp::Error(SP_ERR_UNAVAILABLE, "HIP unavailable: no GPU device");
+    if (ordinal < 0 || ordinal >= count)
+        throw sp::Error(SP_ERR_UNAVAILABLE, "HIP unavailable: no device " + std::to_string(ordinal));
+    auto* b = new sp_backend;
+    b->device = ordinal;
+    try {
+        sp::hip_check(hipSetDevice(ordinal), "hipSetDevice");
+        hipDeviceProp_t prop{};
+        sp::hip_check(hipGetDeviceProperties(&prop, ordinal), "hipGetDeviceProperties");
+        b->cus = prop.multiProcessorCount;
+        b->name = std::string("hip (") + prop.name + ", " + prop.gcnArchName + ")";
+        // One default-priority stream carries every operation of this backend.
+        sp::hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+        const std::string path = code_object_path();
+        if (hipModuleLoad(&b->module, path.c_str()) != hipSuccess)
+            throw sp::Error(SP_ERR_UNAVAILABLE, "HIP unavailable: cannot load code object " + path);
+        auto get = [&](hipFunction_t& f, const char* name) {
+            sp::hip_check(hipModuleGetFunction(&f, b->module, name), name);
+            b->kernel_fns.push_back(f);
+            b->kernel_names.push_back(name);
+            b->kernel_ms.push_back(0.0);
+            b->kernel_launches.push_back(0);
+        };
+        get(b->k_spmv, "sp_spmv_f64");
+        get(b->k_spmv_t_primal, "sp_spmv_t_primal_f64");
+        get(b->k_spmv_t_primal_halpern, "sp_spmv_t_primal_halpern_f64");
+        get(b->k_spmv_dual, "sp_spmv_dual_f64");
+        get(b->k_spmv_dual_halpern, "sp_spmv_dual_halpern_f64");
+        get(b->k_primal_step, "sp_primal_step_f64");
+        get(b->k_dual_step, "sp_dual_step_f64");
+        get(b->k_halpern, "sp_halpern_step_f64");
+        get(b->k_sub, "sp_sub_f64");
+        get(b->k_scale, "sp_scale_f64");
+        get(b->k_fill, "sp_fill_f64");
+        get(b->k_dot, "sp_red_dot_f64");
+        get(b->k_dist, "sp_red_distance_sq_f64");
+        get(b->k_primal_stats, "sp_red_primal_stats_f64");
+        get(b->k_dual_stats, "sp_red_dual_stats_f64");
+        get(b->k_step_dual, "sp_red_step_dual_f64");
+        get(b->k_primal_ray, "sp_red_primal_ray_f64");
+        get(b->k_dual_ray, "sp_red_dual_ray_f64");
+        b->red_partials = sp::dmalloc<double>((size_t)sp::Hip::kMaxReduceBlocks * sp::Hip::kMaxSlots);
+        b->red_ticket = sp::dmalloc<unsigned>(1);
+        b->red_result = sp::dmalloc<double>(sp::Hip::kMaxSlots);
+        sp::hip_check(hipMemsetAsync(b->red_ticket, 0, sizeof(unsigned), b->stream), "hipMemsetAsync");
+        sp::hip_check(hipHostMalloc(reinterpret_cast<void**>(&b->host_result), sizeof(double) * sp::Hip::kMaxSlots,
+                                    hipHostMallocDefault),
+                      "hipHostMalloc");
+        sp::hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    } catch (...) {
+        sp_backend_close(b);
+        throw;
+    }
+    return b;
+}
+
+void sp_backend_close(sp_backend* b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (const auto& pe : b->pending) b->event_pool.push_back(pe.start), b->event_pool.push_back(pe.stop);
+    for (hipEvent_t e : b->event_pool) (void)hipEventDestroy(e);
+    sp::dfree(b->red_partials);
+    sp::dfree(b->red_ticket);
+    sp::dfree(b->red_result);
+    if (b->host_result) (void)hipHostFree(b->host_result);
+    if (b->module) (void)hipModuleUnload(b->module);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}

@@ -1,0 +1,165 @@
+// The HIP backend: device state behind the opaque C handles, and the C++
+// methods the PDHG driver (solver.hpp) calls, one per Backend method of the
+// reference (crates/saddlepoint-core/src/backend.rs:43-137).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "host.hpp"
+#include "saddlepoint_hip.h"
+
+namespace sp {
+
+// Launch descriptor of one SpMV work item; mirrors `struct Desc` in kernels.hip.
+struct Desc {
+    int r0, r1, k0, k1;
+};
+
+// How an SpMV is cut into work items (see plan_spmv). The autotuner searches
+// these knobs per matrix.
+struct SpmvTuning {
+    int block_nnz = 2048;  // nonzeros staged per stream workgroup (<= kernel LDS capacity)
+    int long_row = 256;    // rows longer than this get workgroups of their own
+    int max_rows = 1024;   // rows per stream workgroup
+};
+
+// A CSR matrix on the device with int32 indices and its SpMV plan.
+struct DevCsr {
+    int nrows = 0, ncols = 0;
+    int64_t nnz = 0;
+    int* rp = nullptr;
+    int* ci = nullptr;
+    double* va = nullptr;
+    Desc* desc = nullptr;
+    int nblocks = 0;
+    double* partials = nullptr;  // one per chunk of a multi-chunk row
+    unsigned* tickets = nullptr;
+    int npartials = 0;
+    SpmvTuning tuning;
+};
+
+// Host-side planner: cuts rows into stream blocks and long-row chunks.
+std::vector<Desc> plan_spmv(const int64_t* row_ptr, int64_t nrows, const SpmvTuning& t, int* npartials);
+
+}  // namespace sp
+
+struct sp_vec {
+    sp_backend* b = nullptr;
+    double* d = nullptr;
+    int64_t len = 0;
+};
+
+struct sp_problem {
+    sp_backend* b = nullptr;
+    int64_t m = 0, n = 0;
+    sp::DevCsr a, at;
+    double *c = nullptr, *lc = nullptr, *uc = nullptr, *lv = nullptr, *uv = nullptr;
+    double *pf = nullptr, *df = nullptr;
+};
+
+struct sp_backend {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipModule_t module = nullptr;
+    int cus = 0;
+    std::string name;
+    std::string err;
+    // Reduction scratch: per-workgroup partials, the arrival ticket, the
+    // result slots and their pinned host mirror.
+    double* red_partials = nullptr;
+    unsigned* red_ticket = nullptr;
+    double* red_result = nullptr;
+    double* host_result = nullptr;
+
+    // Kernel timing (sp_backend_profile): while enabled, every launch is
+    // bracketed by two events on the backend's stream.
+    bool profiling = false;
+    struct Pending {
+        int kernel;
+        hipEvent_t start, stop;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::vector<std::string> kernel_names;   // index = kernel id
+    std::vector<hipFunction_t> kernel_fns;
+    std::vector<double> kernel_ms;
+    std::vector<int64_t> kernel_launches;
+
+    hipFunction_t k_spmv, k_spmv_t_primal, k_spmv_t_primal_halpern, k_spmv_dual, k_spmv_dual_halpern;
+    hipFunction_t k_primal_step, k_dual_step, k_halpern, k_sub, k_scale, k_fill;
+    hipFunction_t k_dot, k_dist, k_primal_stats, k_dual_stats, k_step_dual, k_primal_ray, k_dual_ray;
+};
+
+namespace sp {
+
+// Checks a HIP status and throws sp::Error(SP_ERR_HIP) with the call site.
+void hip_check(hipError_t e, const char* what);
+
+// The Backend methods. Each throws sp::Error on failure.
+struct Hip {
+    sp_backend* b;
+
+    static constexpr int kMaxReduceBlocks = 1024;
+    static constexpr int kMaxSlots = 8;
+
+    std::string name() const { return b->name; }
+    sp_problem* load(const ScaledProblem& s);
+    sp_problem* load_view(const sp_scaled_view& v);
+    sp_vec* upload(const double* data, int64_t len);
+    sp_vec* zeros(int64_t len);
+    void download(const sp_vec* v, double* out, int64_t len);
+    void copy(const sp_vec* src, sp_vec* dst);
+
+    void spmv(const sp_problem* p, const sp_vec* x, sp_vec* out);
+    void spmv_t(const sp_problem* p, const sp_vec* y, sp_vec* out);
+    void primal_step(const sp_problem* p, const sp_vec* x, const sp_vec* aty, double tau, sp_vec* x_t,
+                     sp_vec* x_bar);
+    void dual_step(const sp_problem* p, const sp_vec* y, const sp_vec* ax_bar, double sigma, sp_vec* y_t);
+    void halpern_step(sp_vec* z, const sp_vec* z_t, const sp_vec* anchor, double w, double rho);
+    double distance_sq(const sp_vec* a, const sp_vec* bv);
+    void sub(const sp_vec* a, const sp_vec* bv, sp_vec* out);
+    double norm_sq(const sp_vec* v);
+    double dot(const sp_vec* a, const sp_vec* bv);
+    void scale(sp_vec* v, double alpha);
+    sp_primal_stats_t primal_stats(const sp_problem* p, const sp_vec* x, const sp_vec* ax);
+    sp_dual_stats_t dual_stats(const sp_problem* p, const sp_vec* y, const sp_vec* aty);
+    double step_dual_residual_sq(const sp_problem* p, const sp_vec* x, const sp_vec* x_t, const sp_vec* aty,
+                                 const sp_vec* aty_t, double tau);
+    sp_ray_stats_t primal_ray_stats(const sp_problem* p, const sp_vec* dx, const sp_vec* adx);
+    sp_ray_stats_t dual_ray_stats(const sp_problem* p, const sp_vec* dy, const sp_vec* atdy);
+
+    // Fused forms used by the driver when params.fused is set. Each is
+    // bitwise equal to the sequence of Backend calls it replaces.
+    // spmv_t + primal_step
+    void spmv_t_primal(const sp_problem* p, const sp_vec* y, const sp_vec* x, double tau, sp_vec* aty,
+                       sp_vec* x_t, sp_vec* x_bar);
+    // spmv + dual_step
+    void spmv_dual(const sp_problem* p, const sp_vec* x_bar, const sp_vec* y, double sigma, sp_vec* ax,
+                   sp_vec* y_t);
+    // One whole iteration without a check: spmv_t + primal_step + halpern(x),
+    // then spmv + dual_step + halpern(y). x and y are updated in place;
+    // x_bar is overwritten; x_t, y_t, aty and ax are not written.
+    void plain_iteration(const sp_problem* p, sp_vec* x, sp_vec* y, const sp_vec* x_anchor, const sp_vec* y_anchor,
+                         sp_vec* x_bar, double tau, double sigma, double w, double rho);
+
+    void sync();
+
+   private:
+    int reduce_grid(int64_t n) const;
+    const double* read_result(int k);
+};
+
+void free_problem(sp_problem* p);
+void free_vec(sp_vec* v);
+// Folds finished event pairs into the per-kernel totals (synchronizes).
+void profile_collect(sp_backend* b);
+
+}  // namespace sp
+
+// Opens device `ordinal` and loads the code object; throws sp::Error.
+sp_backend* sp_backend_open(int ordinal);
+void sp_backend_close(sp_backend* b);

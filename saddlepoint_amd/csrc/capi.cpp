@@ -1,0 +1,414 @@
+// The C ABI of include/saddlepoint_hip.h: argument checks, error codes and
+// messages around the HIP backend (backend.cpp) and the driver (solver.hpp).
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <memory>
+
+#include "backend.hpp"
+#include "saddlepoint_hip.h"
+#include "solver.hpp"
+
+namespace {
+
+thread_local std::string g_create_error;
+
+// Runs f, mapping exceptions to an error code and a message on the backend.
+template <class F>
+int guarded(sp_backend* b, F&& f) {
+    if (!b) return SP_ERR_INVALID;
+    try {
+        f();
+        b->err.clear();
+        return SP_OK;
+    } catch (const sp::Error& e) {
+        b->err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        b->err = "host out of memory";
+        return SP_ERR_MEMORY;
+    } catch (const std::exception& e) {
+        b->err = e.what();
+        return SP_ERR_INVALID;
+    }
+}
+
+struct VecDel {
+    void operator()(sp_vec* v) const { sp::free_vec(v); }
+};
+struct ProbDel {
+    void operator()(sp_problem* p) const { sp::free_problem(p); }
+};
+
+// The backend as the driver sees it: owning handles for vectors and problems.
+struct DriverBackend : sp::Hip {
+    using Vector = std::unique_ptr<sp_vec, VecDel>;
+    using Problem = std::unique_ptr<sp_problem, ProbDel>;
+    Problem load(const sp::ScaledProblem& s) { return Problem(Hip::load(s)); }
+    Vector zeros(int64_t n) { return Vector(Hip::zeros(n)); }
+    Vector upload(const double* d, int64_t n) { return Vector(Hip::upload(d, n)); }
+};
+
+void check_problem(const sp_backend* b, const sp_problem* p) {
+    if (!p) throw sp::Error(SP_ERR_INVALID, "null problem");
+    if (p->b != b) throw sp::Error(SP_ERR_INVALID, "problem belongs to another backend");
+}
+
+sp::LpView view_of(const sp_lp_view* lp) {
+    if (!lp) throw sp::Error(SP_ERR_INVALID, "null problem");
+    sp::LpView v;
+    v.m = lp->a.nrows;
+    v.n = lp->a.ncols;
+    v.nnz = lp->a.nnz;
+    v.row_ptr = lp->a.row_ptr;
+    v.col_idx = lp->a.col_idx;
+    v.values = lp->a.values;
+    v.c = lp->c;
+    v.lc = lp->lc;
+    v.uc = lp->uc;
+    v.lv = lp->lv;
+    v.uv = lp->uv;
+    v.c0 = lp->c0;
+    v.maximize = lp->maximize != 0;
+    if (v.m < 0 || v.n < 0 || (v.n && (!v.c || !v.lv || !v.uv)) || (v.m && (!v.lc || !v.uc)))
+        throw sp::Error(SP_ERR_INVALID, "null array in problem");
+    return v;
+}
+
+sp::ScalingParams scaling_of(const sp_params& p) {
+    sp::ScalingParams s;
+    s.geometric_mean_passes = p.geometric_mean_passes;
+    s.ruiz_iterations = p.ruiz_iterations;
+    s.has_pock_chambolle = p.has_pock_chambolle != 0;
+    s.pock_chambolle_alpha = p.pock_chambolle_alpha;
+    s.bound_objective_rescaling = p.bound_objective_rescaling != 0;
+    return s;
+}
+
+}  // namespace
+
+struct sp_solver {
+    sp_backend* b;
+    DriverBackend db;
+    sp::Solver<DriverBackend> solver;
+    double setup_s;
+    sp_solver(sp_backend* bb, const sp::LpView& v, const sp_params& p, sp::ScaledProblem s, double setup)
+        : b(bb), db{{bb}}, solver(db, v, p, std::move(s)), setup_s(setup) {}
+};
+
+extern "C" {
+
+int sp_backend_create(int ordinal, sp_backend** out) {
+    if (!out) return SP_ERR_INVALID;
+    *out = nullptr;
+    try {
+        *out = sp_backend_open(ordinal);
+        g_create_error.clear();
+        return SP_OK;
+    } catch (const sp::Error& e) {
+        g_create_error = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_create_error = e.what();
+        return SP_ERR_UNAVAILABLE;
+    }
+}
+
+void sp_backend_destroy(sp_backend* b) { sp_backend_close(b); }
+
+const char* sp_backend_name(const sp_backend* b) { return b ? b->name.c_str() : ""; }
+
+const char* sp_last_error(const sp_backend* b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+
+int sp_problem_load(sp_backend* b, const sp_scaled_view* p, sp_problem** out) {
+    return guarded(b, [&] {
+        if (!p || !out) throw sp::Error(SP_ERR_INVALID, "load: null argument");
+        *out = nullptr;
+        *out = sp::Hip{b}.load_view(*p);
+    });
+}
+
+void sp_problem_free(sp_problem* p) { sp::free_problem(p); }
+
+int sp_vec_upload(sp_backend* b, const double* data, int64_t len, sp_vec** out) {
+    return guarded(b, [&] {
+        if (!out || (len && !data)) throw sp::Error(SP_ERR_INVALID, "upload: null argument");
+        *out = sp::Hip{b}.upload(data, len);
+    });
+}
+
+int sp_vec_zeros(sp_backend* b, int64_t len, sp_vec** out) {
+    return guarded(b, [&] {
+        if (!out) throw sp::Error(SP_ERR_INVALID, "zeros: null argument");
+        *out = sp::Hip{b}.zeros(len);
+    });
+}
+
+int sp_vec_download(sp_backend* b, const sp_vec* v, double* out, int64_t len) {
+    return guarded(b, [&] {
+        if (len && !out) throw sp::Error(SP_ERR_INVALID, "download: null argument");
+        sp::Hip{b}.download(v, out, len);
+    });
+}
+
+int sp_vec_copy(sp_backend* b, const sp_vec* src, sp_vec* dst) {
+    return guarded(b, [&] {
+        if (!src) throw sp::Error(SP_ERR_INVALID, "copy: null vector");
+        sp::Hip{b}.copy(src, dst);
+    });
+}
+
+int64_t sp_vec_len(const sp_vec* v) { return v ? v->len : -1; }
+
+void sp_vec_free(sp_vec* v) { sp::free_vec(v); }
+
+int sp_spmv(sp_backend* b, const sp_problem* p, const sp_vec* x, sp_vec* out) {
+    return guarded(b, [&] {
+        check_problem(b, p);
+        sp::Hip{b}.spmv(p, x, out);
+    });
+}
+
+int sp_spmv_t(sp_backend* b, const sp_problem* p, const sp_vec* y, sp_vec* out) {
+    return guarded(b, [&] {
+        check_problem(b, p);
+        sp::Hip{b}.spmv_t(p, y, out);
+    });
+}
+
+int sp_primal_step(sp_backend* b, const sp_problem* p, const sp_vec* x, const sp_vec* aty, double tau, sp_vec* x_t,
+                   sp_vec* x_bar) {
+    return guarded(b, [&] {
+        check_problem(b, p);
+        sp::Hip{b}.primal_step(p, x, aty, tau, x_t, x_bar);
+    });
+}
+
+int sp_dual_step(sp_backend* b, const sp_problem* p, const sp_vec* y, const sp_vec* ax_bar, double sigma,
+                 sp_vec* y_t) {
+    return guarded(b, [&] {
+        check_problem(b, p);
+        sp::Hip{b}.dual_step(p, y, ax_bar, sigma, y_t);
+    });
+}
+
+int sp_halpern_step(sp_backend* b, sp_vec* z, const sp_vec* z_t, const sp_vec* anchor, double w, double rho) {
+    return guarded(b, [&] { sp::Hip{b}.halpern_step(z, z_t, anchor, w, rho); });
+}
+
+int sp_distance_sq(sp_backend* b, const sp_vec* a, const sp_vec* bv, double* out) {
+    return guarded(b, [&] {
+        if (!out) throw sp::Error(SP_ERR_INVALID, "null output");
+        *out = sp::Hip{b}.distance_sq(a, bv);
+    });
+}
+
+int sp_sub(sp_backend* b, const sp_vec* a, const sp_vec* bv, sp_vec* out) {
+    return guarded(b, [&] { sp::Hip{b}.sub(a, bv, out); });
+}
+
+int sp_norm_sq(sp_backend* b, const sp_vec* v, double* out) {
+    return guarded(b, [&] {
+        if (!out) throw sp::Error(SP_ERR_INVALID, "null output");
+        *out = sp::Hip{b}.norm_sq(v);
+    });
+}
+
+int sp_dot(sp_backend* b, const sp_vec* a, const sp_vec* bv, double* out) {
+    return guarded(b, [&] {
+        if (!out) throw sp::Error(SP_ERR_INVALID, "null output");
+        *out = sp::Hip{b}.dot(a, bv);
+    });
+}
+
+int sp_scale(sp_backend* b, sp_vec* v, double alpha) {
+    return guarded(b, [&] { sp::Hip{b}.scale(v, alpha); });
+}
+
+int sp_primal_stats(sp_backend* b, const sp_problem* p, const sp_vec* x, const sp_vec* ax, sp_primal_stats_t* out) {
+    return guarded(b, [&] {
+        check_problem(b, p);
+        if (!out) throw sp::Error(SP_ERR_INVALID, "null output");
+        *out = sp::Hip{b}.primal_stats(p, x, ax);
+    });
+}
+
+int sp_dual_stats(sp_backend* b, const sp_problem* p, const sp_vec* y, const sp_vec* aty, sp_dual_stats_t* out) {
+    return guarded(b, [&] {
+        check_problem(b, p);
+        if (!out) throw sp::Error(SP_ERR_INVALID, "null output");
+        *out = sp::Hip{b}.dual_stats(p, y, aty);
+    });
+}
+
+int sp_step_dual_residual_sq(sp_backend* b, const sp_problem* p, const sp_vec* x, const sp_vec* x_t,
+                             const sp_vec* aty, const sp_vec* aty_t, double tau, double* out) {
+    return guarded(b, [&] {
+        check_problem(b, p);
+        if (!out) throw sp::Error(SP_ERR_INVALID, "null output");
+        *out = sp::Hip{b}.step_dual_residual_sq(p, x, x_t, aty, aty_t, tau);
+    });
+}
+
+int sp_primal_ray_stats(sp_backend* b, const sp_problem* p, const sp_vec* dx, const sp_vec* adx, sp_ray_stats_t* out) {
+    return guarded(b, [&] {
+        check_problem(b, p);
+        if (!out) throw sp::Error(SP_ERR_INVALID, "null output");
+        *out = sp::Hip{b}.primal_ray_stats(p, dx, adx);
+    });
+}
+
+int sp_dual_ray_stats(sp_backend* b, const sp_problem* p, const sp_vec* dy, const sp_vec* atdy, sp_ray_stats_t* out) {
+    return guarded(b, [&] {
+        check_problem(b, p);
+        if (!out) throw sp::Error(SP_ERR_INVALID, "null output");
+        *out = sp::Hip{b}.dual_ray_stats(p, dy, atdy);
+    });
+}
+
+void sp_default_params(sp_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof *p);
+    // SolverParams::default (solver.rs:70-93) and ScalingParams::default (scaling.rs:25-34).
+    p->tolerance = 1e-6;
+    p->infeasibility_tolerance = 1e-8;
+    p->max_iterations = 1000000;
+    p->time_limit_s = -1.0;
+    p->check_every = 64;
+    p->step_size_factor = 0.998;
+    p->reflection = 1.0;
+    p->restart_sufficient = 0.2;
+    p->restart_necessary = 0.5;
+    p->restart_artificial = 0.36;
+    p->pid_kp = 0.99;
+    p->pid_ki = 0.01;
+    p->pid_kd = 0.0;
+    p->pid_discount = 0.3;
+    p->power_iterations = 200;
+    p->geometric_mean_passes = 12;
+    p->ruiz_iterations = 10;
+    p->has_pock_chambolle = 1;
+    p->pock_chambolle_alpha = 1.0;
+    p->bound_objective_rescaling = 1;
+    p->termination = 0;
+    p->fused = 1;
+    p->log_every_checks = 0;
+}
+
+int sp_solve(sp_backend* b, const sp_lp_view* lp, const sp_params* params, sp_solution* out, double* x_out,
+             double* y_out) {
+    return guarded(b, [&] {
+        if (!params || !out) throw sp::Error(SP_ERR_INVALID, "solve: null argument");
+        if (params->check_every == 0) throw sp::Error(SP_ERR_INVALID, "solve: check_every must be positive");
+        const auto t0 = std::chrono::steady_clock::now();
+        const sp::LpView v = view_of(lp);
+        sp::validate(v);
+        sp::ScaledProblem scaled = sp::ScaledProblem::build(v, scaling_of(*params));
+        DriverBackend db{{b}};
+        sp::Solver<DriverBackend> solver(db, v, *params, std::move(scaled));
+        const double setup = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        *out = solver.run(x_out, y_out);
+        out->setup_time_s = setup;
+        out->total_time_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    });
+}
+
+int sp_solver_create(sp_backend* b, const sp_lp_view* lp, const sp_params* params, sp_solver** out) {
+    return guarded(b, [&] {
+        if (!params || !out) throw sp::Error(SP_ERR_INVALID, "solver_create: null argument");
+        if (params->check_every == 0) throw sp::Error(SP_ERR_INVALID, "solver_create: check_every must be positive");
+        *out = nullptr;
+        const auto t0 = std::chrono::steady_clock::now();
+        const sp::LpView v = view_of(lp);
+        sp::validate(v);
+        sp::ScaledProblem scaled = sp::ScaledProblem::build(v, scaling_of(*params));
+        const double setup = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        auto* s = new sp_solver(b, v, *params, std::move(scaled), setup);
+        try {
+            s->solver.start();
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+int sp_solver_advance(sp_solver* s, uint64_t iterations, int* done) {
+    if (!s) return SP_ERR_INVALID;
+    return guarded(s->b, [&] {
+        const bool d = s->solver.advance(iterations);
+        if (done) *done = d ? 1 : 0;
+    });
+}
+
+int sp_solver_result(sp_solver* s, sp_solution* out, double* x_out, double* y_out) {
+    if (!s) return SP_ERR_INVALID;
+    return guarded(s->b, [&] {
+        if (!out) throw sp::Error(SP_ERR_INVALID, "solver_result: null output");
+        *out = s->solver.finish(x_out, y_out);
+        out->setup_time_s = s->setup_s;
+        out->total_time_s = s->setup_s + out->solve_time_s;
+    });
+}
+
+void sp_solver_free(sp_solver* s) { delete s; }
+
+int sp_backend_profile(sp_backend* b, int enable) {
+    return guarded(b, [&] {
+        sp::profile_collect(b);
+        b->profiling = enable != 0;
+        if (b->profiling) {
+            std::fill(b->kernel_ms.begin(), b->kernel_ms.end(), 0.0);
+            std::fill(b->kernel_launches.begin(), b->kernel_launches.end(), 0);
+        }
+    });
+}
+
+int sp_backend_profile_read(sp_backend* b, int index, char* name, int name_len, double* total_ms,
+                            int64_t* launches) {
+    return guarded(b, [&] {
+        sp::profile_collect(b);
+        if (index < 0 || index >= (int)b->kernel_names.size()) throw sp::Error(SP_ERR_INVALID, "no such kernel");
+        if (name && name_len > 0) {
+            std::strncpy(name, b->kernel_names[(size_t)index].c_str(), (size_t)name_len - 1);
+            name[name_len - 1] = 0;
+        }
+        if (total_ms) *total_ms = b->kernel_ms[(size_t)index];
+        if (launches) *launches = b->kernel_launches[(size_t)index];
+    });
+}
+
+int sp_backend_sync(sp_backend* b) {
+    return guarded(b, [&] { sp::Hip{b}.sync(); });
+}
+
+int sp_scale_problem(const sp_lp_view* lp, const sp_params* params, double* a_values, double* c, double* lc,
+                     double* uc, double* lv, double* uv, double* row_scale, double* col_scale, double* bound_scale,
+                     double* obj_scale) {
+    try {
+        if (!params) return SP_ERR_INVALID;
+        const sp::LpView v = view_of(lp);
+        sp::validate(v);
+        const sp::ScaledProblem s = sp::ScaledProblem::build(v, scaling_of(*params));
+        std::copy(s.a.values.begin(), s.a.values.end(), a_values);
+        std::copy(s.c.begin(), s.c.end(), c);
+        std::copy(s.lc.begin(), s.lc.end(), lc);
+        std::copy(s.uc.begin(), s.uc.end(), uc);
+        std::copy(s.lv.begin(), s.lv.end(), lv);
+        std::copy(s.uv.begin(), s.uv.end(), uv);
+        std::copy(s.row_scale.begin(), s.row_scale.end(), row_scale);
+        std::copy(s.col_scale.begin(), s.col_scale.end(), col_scale);
+        *bound_scale = s.bound_scale;
+        *obj_scale = s.obj_scale;
+        return SP_OK;
+    } catch (const sp::Error& e) {
+        g_create_error = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_create_error = e.what();
+        return SP_ERR_INVALID;
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,68 @@
+// Host-side model of the LP and its preconditioning, in C++.
+//
+// Restates crates/saddlepoint-core/src/problem.rs (CsrMatrix, LpProblem) and
+// crates/saddlepoint-core/src/scaling.rs (ScaledProblem) so that the PDHG
+// driver in solver.hpp can run over the HIP backend without the Rust crates.
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace sp {
+
+// Errors cross the C ABI as a code plus a message.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// CsrMatrix (problem.rs:16-24) with usize indices as int64.
+struct Csr {
+    int64_t nrows = 0, ncols = 0;
+    std::vector<int64_t> row_ptr, col_idx;
+    std::vector<double> values;
+    int64_t nnz() const { return (int64_t)values.size(); }
+    Csr transpose() const;                                          // problem.rs:88-109
+    void scale(const std::vector<double>& row, const std::vector<double>& col);  // :120-126
+};
+
+// LpProblem (problem.rs:131-145), borrowed views of the caller's arrays.
+struct LpView {
+    int64_t m = 0, n = 0, nnz = 0;
+    const int64_t *row_ptr = nullptr, *col_idx = nullptr;
+    const double* values = nullptr;
+    const double *c = nullptr, *lc = nullptr, *uc = nullptr, *lv = nullptr, *uv = nullptr;
+    double c0 = 0.0;
+    bool maximize = false;
+};
+
+// ScalingParams (scaling.rs:15-34).
+struct ScalingParams {
+    int64_t geometric_mean_passes = 12;
+    int64_t ruiz_iterations = 10;
+    bool has_pock_chambolle = true;
+    double pock_chambolle_alpha = 1.0;
+    bool bound_objective_rescaling = true;
+};
+
+// ScaledProblem (scaling.rs:37-50).
+struct ScaledProblem {
+    Csr a, at;
+    std::vector<double> c, lc, uc, lv, uv, row_scale, col_scale;
+    double bound_scale = 1.0, obj_scale = 1.0;
+
+    static ScaledProblem build(const LpView& p, const ScalingParams& params);  // :53-103
+    std::vector<double> unscale_primal(const std::vector<double>& xs) const;    // :114-116
+    std::vector<double> unscale_dual(const std::vector<double>& ys) const;      // :119-121
+    std::vector<double> primal_residual_factors() const;                        // :124-126
+    std::vector<double> dual_residual_factors() const;                          // :129-131
+    double objective_factor() const { return 1.0 / (bound_scale * obj_scale); }  // :134-136
+};
+
+double norm2(const double* v, int64_t n);                                   // scaling.rs:186-188
+double finite_bound_norm(const double* lo, const double* hi, int64_t m);    // scaling.rs:191-201
+void validate(const LpView& p);                                             // problem.rs:166-194
+
+}  // namespace sp

@@ -1,0 +1,343 @@
+// The PDHG driver: restarted reflected-Halpern PDHG with the PID primal-weight
+// controller, restated from crates/saddlepoint-core/src/solver.rs:154-480 so
+// that it drives any backend with the reference's Backend method sequence.
+//
+// Templated on the backend. With params.fused, the iterations between checks
+// run through the backend's fused kernels instead (B::plain_iteration, and
+// B::spmv_t_primal / B::spmv_dual on check iterations), which compute the same
+// expressions in the same order and so give bitwise the same iterates.
+#pragma once
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <limits>
+#include <memory>
+
+#include "host.hpp"
+#include "saddlepoint_hip.h"
+
+namespace sp {
+
+// PidState (solver.rs:445-480).
+struct PidState {
+    double integral = 0.0;
+    bool has_prev = false;
+    double prev_error = 0.0;
+    double best_weight;
+    double best_balance = std::numeric_limits<double>::infinity();
+
+    explicit PidState(double omega) : best_weight(omega) {}
+
+    double update(const sp_params& p, double omega, double dx, double dy, double rel_primal, double rel_dual) {
+        auto sane = [](double v) { return v >= 1e-16 && v <= 1e12; };
+        const double ratio = rel_dual / rel_primal;
+        if (!sane(dx) || !sane(dy) || !(ratio >= 1e-8 && ratio <= 1e8)) {
+            integral = 0.0;
+            has_prev = false;
+            return best_weight;
+        }
+        // Positive error: the scaled primal displacement is too large.
+        const double e = std::log(omega * dx / dy);
+        integral = p.pid_discount * integral + e;
+        const double derivative = has_prev ? e - prev_error : 0.0;
+        has_prev = true;
+        prev_error = e;
+        const double next = std::exp(std::log(omega) - (p.pid_kp * e + p.pid_ki * integral + p.pid_kd * derivative));
+        const double balance = std::fabs(std::log10(ratio));
+        if (balance < best_balance) {
+            best_balance = balance;
+            best_weight = next;
+        }
+        return next;
+    }
+};
+
+// KKT measures of one candidate point, in the original space (solver.rs:133-152).
+struct Kkt {
+    double primal_obj = 0, dual_obj = 0, primal_res = 0, dual_res = 0;
+    double gap() const { return std::fabs(primal_obj - dual_obj); }
+    bool converged(double eps, double b_norm, double c_norm) const {
+        return primal_res <= eps * (1.0 + b_norm) && dual_res <= eps * (1.0 + c_norm) &&
+               gap() <= eps * (1.0 + std::fabs(primal_obj) + std::fabs(dual_obj));
+    }
+};
+
+template <class B>
+class Solver {
+    using Vec = typename B::Vector;  // owning handle
+    using Prob = typename B::Problem;
+
+   public:
+    // Solver::new (solver.rs:185-212): load, then thirteen vectors, zero except
+    // c. The LP itself is not kept: only m, n, c0, the sense and the two
+    // norms that run() needs.
+    Solver(B& b, const LpView& problem, const sp_params& params, ScaledProblem scaled)
+        : b_(b), p_(params), scaled_(std::move(scaled)), m_(problem.m), n_(problem.n), c0_(problem.c0),
+          maximize_(problem.maximize) {
+        b_norm_ = finite_bound_norm(problem.lc, problem.uc, problem.m);
+        c_norm_ = norm2(problem.c, problem.n);
+        dev_ = b_.load(scaled_);
+        const int64_t m = m_, n = n_;
+        x_ = b_.zeros(n), y_ = b_.zeros(m), x_t_ = b_.zeros(n), y_t_ = b_.zeros(m), x_bar_ = b_.zeros(n);
+        x_anchor_ = b_.zeros(n), y_anchor_ = b_.zeros(m), ax_ = b_.zeros(m), aty_ = b_.zeros(n);
+        tmp_n_ = b_.zeros(n), tmp_m_ = b_.zeros(m), tmp_m2_ = b_.zeros(m);
+        c_ = b_.upload(scaled_.c.data(), n);
+    }
+
+    // Solver::run (solver.rs:311-438). x_out/y_out receive the unscaled x_t, y_t.
+    sp_solution run(double* x_out, double* y_out) {
+        start();
+        while (!advance(UINT64_MAX)) {
+        }
+        return finish(x_out, y_out);
+    }
+
+    // The part of run() before the loop (solver.rs:312-330): the clock starts,
+    // power iteration sets η, and ω starts at 1 (or the norm ratio).
+    void start() {
+        start_ = clock::now();
+        started_ = true;
+        eta_ = p_.step_size_factor / estimate_norm();
+        omega_ = p_.bound_objective_rescaling ? 1.0 : (1.0 + c_norm_) / (1.0 + b_norm_);
+        pid_ = PidState(omega_);
+    }
+
+    // Runs the loop (solver.rs:332-420) for at most `budget` more iterations;
+    // returns true once it has terminated.
+    bool advance(uint64_t budget) {
+        const sp_params& p = p_;
+        for (uint64_t done = 0; !terminated_ && done < budget; ++done) {
+            const double tau = eta_ / omega_, sigma = eta_ * omega_;
+            const uint64_t next_total = total_ + 1;
+            const bool check = next_total % p.check_every == 0 || next_total >= p.max_iterations;
+            if (p.fused && !check && k_ != 0) {
+                // Nothing of this iteration is observed but x, y and x_bar.
+                const double w = (double)(k_ + 1) / (double)(k_ + 2);
+                b_.plain_iteration(dev_.get(), x_.get(), y_.get(), x_anchor_.get(), y_anchor_.get(), x_bar_.get(),
+                                   tau, sigma, w, p.reflection);
+                total_ = next_total;
+                k_ += 1;
+                continue;
+            }
+            if (p.fused) {
+                b_.spmv_t_primal(dev_.get(), y_.get(), x_.get(), tau, aty_.get(), x_t_.get(), x_bar_.get());
+                b_.spmv_dual(dev_.get(), x_bar_.get(), y_.get(), sigma, ax_.get(), y_t_.get());
+            } else {
+                b_.spmv_t(dev_.get(), y_.get(), aty_.get());
+                b_.primal_step(dev_.get(), x_.get(), aty_.get(), tau, x_t_.get(), x_bar_.get());
+                b_.spmv(dev_.get(), x_bar_.get(), ax_.get());
+                b_.dual_step(dev_.get(), y_.get(), ax_.get(), sigma, y_t_.get());
+            }
+            total_ = next_total;
+            // At k = 0 the iterate is the anchor, so this measures r(z_anchor).
+            const double residual = (check || k_ == 0) ? fixed_point_residual(eta_, omega_) : 0.0;
+            if (k_ == 0) anchor_residual_ = residual;
+            double step_dual = 0.0, step_obj = 0.0;
+            if (check) {
+                kkt_ = kkt();
+                step_dual = std::sqrt(
+                    b_.step_dual_residual_sq(dev_.get(), x_.get(), x_t_.get(), aty_.get(), tmp_n_.get(), tau));
+                if (p.termination == 1) step_obj = step_dual_objective(tau);
+            }
+            const double w = (double)(k_ + 1) / (double)(k_ + 2);
+            b_.halpern_step(x_.get(), x_t_.get(), x_anchor_.get(), w, p.reflection);
+            b_.halpern_step(y_.get(), y_t_.get(), y_anchor_.get(), w, p.reflection);
+            k_ += 1;
+            if (!check) continue;
+            checks_ += 1;
+            if (p.log_every_checks > 0 && checks_ % (uint64_t)p.log_every_checks == 0) {
+                std::fprintf(stderr, "%9llu %14.6e %14.6e %9.2e %9.2e %9.2e %9.2e %5llu %8.2fs %7llu %9.2e\n",
+                             (unsigned long long)total_, kkt_.primal_obj, kkt_.dual_obj, kkt_.primal_res,
+                             kkt_.dual_res, kkt_.gap(), omega_, (unsigned long long)restarts_, elapsed(),
+                             (unsigned long long)k_, residual / anchor_residual_);
+            }
+            Kkt test = kkt_;
+            if (p.termination == 1) {
+                test.dual_obj = step_obj;
+                test.dual_res = step_dual;
+            }
+            if (test.converged(p.tolerance, b_norm_, c_norm_)) {
+                stop(SP_OPTIMAL);
+                break;
+            }
+            const int inf = infeasibility();
+            if (inf >= 0) {
+                stop(inf);
+                break;
+            }
+            if (total_ >= p.max_iterations) {
+                stop(SP_ITERATION_LIMIT);
+                break;
+            }
+            if (p.time_limit_s >= 0.0 && elapsed() >= p.time_limit_s) {
+                stop(SP_TIME_LIMIT);
+                break;
+            }
+            // Restart decision on the fixed-point residual (solver.rs:396-419).
+            const double r0 = anchor_residual_;
+            const bool restart = residual <= p.restart_sufficient * r0 ||
+                                 (residual <= p.restart_necessary * r0 && residual > prev_residual_) ||
+                                 (double)k_ >= p.restart_artificial * (double)total_;
+            prev_residual_ = residual;
+            if (restart) {
+                const double dx = std::sqrt(b_.distance_sq(x_t_.get(), x_anchor_.get()));
+                const double dy = std::sqrt(b_.distance_sq(y_t_.get(), y_anchor_.get()));
+                const double rel_primal = kkt_.primal_res / (1.0 + b_norm_);
+                // The PID balances against the step-based dual residual.
+                const double rel_dual = step_dual / (1.0 + c_norm_);
+                omega_ = pid_.update(p, omega_, dx, dy, rel_primal, rel_dual);
+                b_.copy(x_t_.get(), x_anchor_.get());
+                b_.copy(y_t_.get(), y_anchor_.get());
+                b_.copy(x_t_.get(), x_.get());
+                b_.copy(y_t_.get(), y_.get());
+                k_ = 0;
+                restarts_ += 1;
+                prev_residual_ = INFINITY;
+            }
+        }
+        return terminated_;
+    }
+
+    uint64_t iterations() const { return total_; }
+    bool terminated() const { return terminated_; }
+
+    // solver.rs:422-437. Before termination, status is -1 and the figures are
+    // those of the last check.
+    sp_solution finish(double* x_out, double* y_out) {
+        if (x_out || y_out) {
+            std::vector<double> xs((size_t)n_), ys((size_t)m_);
+            b_.download(x_t_.get(), xs.data(), n_);
+            b_.download(y_t_.get(), ys.data(), m_);
+            const std::vector<double> x = scaled_.unscale_primal(xs);
+            const std::vector<double> y = scaled_.unscale_dual(ys);
+            if (x_out) std::copy(x.begin(), x.end(), x_out);
+            if (y_out) std::copy(y.begin(), y.end(), y_out);
+        }
+        const double sign = maximize_ ? -1.0 : 1.0;
+        sp_solution s{};
+        s.status = terminated_ ? status_ : -1;
+        s.objective = sign * kkt_.primal_obj;
+        s.dual_objective = sign * kkt_.dual_obj;
+        s.primal_residual = kkt_.primal_res;
+        s.dual_residual = kkt_.dual_res;
+        s.gap = kkt_.gap();
+        s.iterations = total_;
+        s.restarts = restarts_;
+        s.solve_time_s = terminated_ ? solve_time_ : elapsed();
+        s.final_omega = omega_;
+        return s;
+    }
+
+   private:
+    using clock = std::chrono::steady_clock;
+
+    double elapsed() const { return started_ ? std::chrono::duration<double>(clock::now() - start_).count() : 0.0; }
+
+    void stop(int status) {
+        status_ = status;
+        terminated_ = true;
+        solve_time_ = elapsed();
+    }
+
+    // Largest singular value of A' by power iteration on A'ᵀA' (solver.rs:215-242).
+    double estimate_norm() {
+        const int64_t n = n_;
+        if (n == 0 || scaled_.a.nnz() == 0) return 1.0;
+        std::vector<double> start((size_t)n);
+        for (int64_t j = 0; j < n; ++j) start[(size_t)j] = 1.0 + 0.01 * (double)(j % 7);
+        Vec v = b_.upload(start.data(), n);
+        const double s = 1.0 / std::sqrt(b_.norm_sq(v.get()));
+        b_.scale(v.get(), s);
+        double lambda = 0.0;
+        for (int64_t it = 0; it < p_.power_iterations; ++it) {
+            b_.spmv(dev_.get(), v.get(), tmp_m_.get());
+            b_.spmv_t(dev_.get(), tmp_m_.get(), v.get());
+            const double norm = std::sqrt(b_.norm_sq(v.get()));
+            if (std::isnan(norm) || norm <= 0.0 || std::isinf(norm))
+                throw Error(SP_ERR_NUMERICAL, "numerical breakdown: power iteration collapsed");
+            b_.scale(v.get(), 1.0 / norm);
+            const bool done = std::fabs(norm - lambda) <= 1e-6 * norm;
+            lambda = norm;
+            if (done) break;
+        }
+        // Power iteration underestimates; pad slightly so the step stays safe.
+        return std::sqrt(lambda) * 1.01;
+    }
+
+    // solver.rs:245-258. Leaves A'ᵀy_t in tmp_n.
+    Kkt kkt() {
+        const double f = scaled_.objective_factor();
+        b_.spmv(dev_.get(), x_t_.get(), tmp_m_.get());
+        const sp_primal_stats_t ps = b_.primal_stats(dev_.get(), x_t_.get(), tmp_m_.get());
+        b_.spmv_t(dev_.get(), y_t_.get(), tmp_n_.get());
+        const sp_dual_stats_t ds = b_.dual_stats(dev_.get(), y_t_.get(), tmp_n_.get());
+        Kkt k;
+        k.primal_obj = ps.objective * f + c0_;
+        k.dual_obj = ds.objective * f + c0_;
+        k.primal_res = std::sqrt(ps.residual_sq);
+        k.dual_res = std::sqrt(ds.residual_sq);
+        return k;
+    }
+
+    // cuPDLPx's dual objective of y_t (solver.rs:263-271).
+    double step_dual_objective(double tau) {
+        const double rows = b_.dual_stats(dev_.get(), y_t_.get(), c_.get()).objective;
+        const double lx = b_.dot(c_.get(), x_t_.get()) - b_.dot(aty_.get(), x_t_.get()) +
+                          (b_.dot(x_t_.get(), x_t_.get()) - b_.dot(x_.get(), x_t_.get())) / tau;
+        return (rows + lx) * scaled_.objective_factor() + c0_;
+    }
+
+    // ‖z - T(z)‖_P in the PDHG metric (solver.rs:275-288).
+    double fixed_point_residual(double eta, double omega) {
+        b_.sub(x_t_.get(), x_.get(), tmp_n_.get());
+        b_.sub(y_t_.get(), y_.get(), tmp_m2_.get());
+        b_.spmv(dev_.get(), tmp_n_.get(), tmp_m_.get());
+        const double dx2 = b_.norm_sq(tmp_n_.get());
+        const double dy2 = b_.norm_sq(tmp_m2_.get());
+        const double cross = b_.dot(tmp_m2_.get(), tmp_m_.get());
+        const double r2 = omega / eta * dx2 + dy2 / (eta * omega) + 2.0 * cross;
+        if (!std::isfinite(r2)) throw Error(SP_ERR_NUMERICAL, "numerical breakdown: iterates are not finite");
+        return std::sqrt(r2 > 0.0 ? r2 : 0.0);
+    }
+
+    // z_t - z_anchor as an infeasibility certificate (solver.rs:291-309); -1 if none.
+    int infeasibility() {
+        const double eps = p_.infeasibility_tolerance;
+        b_.sub(y_t_.get(), y_anchor_.get(), tmp_m_.get());
+        b_.spmv_t(dev_.get(), tmp_m_.get(), tmp_n_.get());
+        const sp_ray_stats_t d = b_.dual_ray_stats(dev_.get(), tmp_m_.get(), tmp_n_.get());
+        if (d.objective > 0.0 && std::sqrt(d.violation_sq) <= eps * d.objective &&
+            d.objective > eps * std::sqrt(d.norm_sq))
+            return SP_PRIMAL_INFEASIBLE;
+        b_.sub(x_t_.get(), x_anchor_.get(), tmp_n_.get());
+        b_.spmv(dev_.get(), tmp_n_.get(), tmp_m_.get());
+        const sp_ray_stats_t pr = b_.primal_ray_stats(dev_.get(), tmp_n_.get(), tmp_m_.get());
+        if (pr.objective < 0.0 && std::sqrt(pr.violation_sq) <= -eps * pr.objective &&
+            -pr.objective > eps * std::sqrt(pr.norm_sq))
+            return SP_DUAL_INFEASIBLE;
+        return -1;
+    }
+
+    B& b_;
+    const sp_params p_;
+    const ScaledProblem scaled_;
+    int64_t m_, n_;
+    double c0_;
+    bool maximize_;
+    double b_norm_ = 0.0, c_norm_ = 0.0;
+    // Loop state of run() (solver.rs:316-330).
+    clock::time_point start_{};
+    bool started_ = false, terminated_ = false;
+    int status_ = -1;
+    double solve_time_ = 0.0;
+    double eta_ = 1.0, omega_ = 1.0;
+    PidState pid_{1.0};
+    uint64_t total_ = 0, k_ = 0, restarts_ = 0, checks_ = 0;
+    double anchor_residual_ = INFINITY, prev_residual_ = INFINITY;
+    Kkt kkt_;
+    Prob dev_;
+    Vec x_, y_, x_t_, y_t_, x_bar_, x_anchor_, y_anchor_, ax_, aty_, tmp_n_, tmp_m_, tmp_m2_, c_;
+};
+
+}  // namespace sp

@@ -1,0 +1,150 @@
+"""The C-ABI library without a GPU: it loads, exports every function that
+include/*.h declares, reports a missing device as SP_ERR_UNAVAILABLE, and its
+host-side parts (validation, ScaledProblem) match the oracle bitwise."""
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from conftest import netlib
+from oracle import oracle as O
+from saddlepoint_amd import _lib, generate, mps
+from saddlepoint_amd.backend import ScaledProblem, default_params
+from saddlepoint_amd.problem import CsrMatrix, LpProblem
+
+ROOT = Path(__file__).resolve().parents[1]
+INF = float("inf")
+
+
+def declared_functions():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b(sp_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_exports_every_declared_function():
+    L = _lib.lib()
+    names = declared_functions()
+    assert len(names) >= 35
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert names == set(_lib.SIGNATURES), names ^ set(_lib.SIGNATURES)
+
+
+def test_missing_device_is_unavailable():
+    import ctypes as C
+
+    h = C.c_void_p()
+    rc = _lib.lib().sp_backend_create(0, C.byref(h))
+    if rc == _lib.SP_OK:
+        _lib.lib().sp_backend_destroy(h)
+        pytest.skip("a GPU is present")
+    assert rc == _lib.SP_ERR_UNAVAILABLE
+    assert _lib.lib().sp_last_error(None)
+
+
+@pytest.mark.parametrize("name", ["afiro", "israel", "shell", "25fv47", "perold"])
+def test_host_scaling_is_bitwise_the_oracle(name):
+    p = netlib(name)
+    g = ScaledProblem.new(p)
+    o = O.ScaledProblem.of(p).arrays()
+    assert np.array_equal(g.a.values, o["a"].values)
+    assert np.array_equal(g.at.values, o["at"].values) and np.array_equal(g.at.col_idx, o["at"].col_idx)
+    for k in ("c", "lc", "uc", "lv", "uv", "row_scale", "col_scale"):
+        assert np.array_equal(getattr(g, k), o[k]), k
+    assert g.bound_scale == o["bound_scale"] and g.obj_scale == o["obj_scale"]
+    assert np.array_equal(g.primal_residual_factors(), o["primal_factor"])
+    assert np.array_equal(g.dual_residual_factors(), o["dual_factor"])
+
+
+def test_host_scaling_config0_and_options():
+    p = generate.config0()
+    for kw in ({}, {"bound_objective_rescaling": 0}, {"has_pock_chambolle": 0, "ruiz_iterations": 3}):
+        g = ScaledProblem.new(p, default_params(**kw))
+        o = O.ScaledProblem.of(p, O.default_scaling(**kw)).arrays()
+        assert np.array_equal(g.a.values, o["a"].values)
+        assert np.array_equal(g.c, o["c"]) and np.array_equal(g.col_scale, o["col_scale"])
+
+
+def test_validation_errors():
+    a = CsrMatrix.from_triplets(1, 2, [0, 0], [0, 1], [1.0, 1.0])
+    good = dict(a=a, c=[1.0, 1.0], lc=[0.0], uc=[1.0], lv=[0.0, 0.0], uv=[1.0, 1.0])
+    ScaledProblem.new(LpProblem(**good))
+    for bad in ({"lv": [2.0, 0.0]}, {"uc": [float("nan")]}, {"c": [float("nan"), 1.0]}):
+        with pytest.raises(_lib.BackendError) as e:
+            ScaledProblem.new(LpProblem(**{**good, **bad}))
+        assert e.value.code == _lib.SP_ERR_PROBLEM
+
+
+# ---- problem model and MPS reader (io/src/lib.rs:324-382) ----------------------------
+
+SAMPLE = """\
+NAME          TESTLP
+OBJSENSE
+    MAX
+ROWS
+ N  COST
+ L  LIM1
+ G  LIM2
+ E  MYEQN
+ E  R4
+COLUMNS
+    X1        COST         1.0   LIM1         1.0
+    X1        LIM2         1.0
+    MARKER    'MARKER'     'INTORG'
+    X2        COST         2.0   LIM1         1.0
+    X2        MYEQN       -1.0
+    MARKER    'MARKER'     'INTEND'
+    X3        COST        -1.0   MYEQN        1.0
+    X3        R4           1.0
+RHS
+    RHS       COST        -3.5
+    RHS       LIM1         4.0   LIM2         1.0
+    RHS       MYEQN        7.0   R4           2.0
+RANGES
+    RNG       LIM1         2.5   R4          -1.0
+BOUNDS
+ UP BND       X1           4.0
+ MI BND       X2
+ UP BND       X2           1.0
+ FR BND       X3
+ENDATA
+"""
+
+
+def test_mps_parses_all_sections():
+    p = mps.read_mps(SAMPLE)
+    assert p.name == "TESTLP" and p.maximize
+    assert p.c.tolist() == [-1.0, -2.0, 1.0]
+    assert p.c0 == -3.5
+    assert p.lc.tolist() == [1.5, 1.0, 7.0, 1.0]
+    assert p.uc.tolist() == [4.0, INF, 7.0, 2.0]
+    assert p.lv.tolist() == [0.0, -INF, -INF]
+    assert p.uv.tolist() == [4.0, 1.0, INF]
+    assert p.a.nnz == 6
+    cols, vals = p.a.row(2)
+    assert cols.tolist() == [1, 2] and vals.tolist() == [-1.0, 1.0]
+    p.validate()
+
+
+def test_mps_rejects_unknown_rows():
+    with pytest.raises(mps.MpsError) as e:
+        mps.read_mps("NAME X\nROWS\n N obj\nCOLUMNS\n    x  nope  1\nENDATA\n")
+    assert e.value.line == 5
+
+
+def test_from_triplets_sums_duplicates_and_drops_zeros():
+    a = CsrMatrix.from_triplets(2, 3, [1, 0, 1, 1, 0], [2, 1, 0, 2, 1], [1.0, 2.0, 3.0, 4.0, -2.0])
+    assert a.row_ptr.tolist() == [0, 0, 2]
+    assert a.col_idx.tolist() == [0, 2] and a.values.tolist() == [3.0, 5.0]
+
+
+@pytest.mark.parametrize("name,m,n,nnz", [("afiro", 27, 32, 83), ("israel", 174, 142, 2269),
+                                          ("25fv47", 821, 1571, 10400), ("80bau3b", 2262, 9799, 21002)])
+def test_netlib_sizes(name, m, n, nnz):
+    p = netlib(name)
+    assert (p.nrows, p.ncols, p.a.nnz) == (m, n, nnz)

@@ -1,0 +1,191 @@
+"""Every Backend method on the GPU against the oracle's CPU backend, on scaled
+problems and deterministic pseudo-random vectors.
+
+Mirrors crates/saddlepoint-cuda/tests/ops.rs:29-146: the same xorshift
+vectors (every fifth entry zero so that every sign branch is taken), the same
+instances (afiro, israel, 25fv47, 80bau3b, perold), and the same bars: vectors
+max|a-b| / (1 + max|a|) < 1e-12, scalars |a-b| / (1 + max(|a|,|b|)) < 1e-11,
+copy bitwise. Added: the benchmark generators' shapes, including rows long
+enough to be split across workgroups.
+"""
+import numpy as np
+import pytest
+
+from conftest import netlib
+from oracle import oracle as O
+from saddlepoint_amd import generate
+from saddlepoint_amd.backend import ScaledProblem
+
+pytestmark = pytest.mark.gpu
+
+MASK = (1 << 64) - 1
+
+
+def vector(length: int, seed: int) -> np.ndarray:
+    """ops.rs:29-39: xorshift64, values in [-2, 2], every fifth one zero."""
+    s = ((seed * 0x9E3779B97F4A7C15) & MASK) | 1
+    out = np.zeros(length)
+    for i in range(length):
+        s ^= (s << 13) & MASK
+        s ^= s >> 7
+        s ^= (s << 17) & MASK
+        out[i] = 0.0 if i % 5 == 0 else (s >> 11) / float(1 << 53) * 4.0 - 2.0
+    return out
+
+
+def close(what, a, b):
+    err = abs(a - b) / (1.0 + max(abs(a), abs(b)))
+    assert err < 1e-11, f"{what}: cpu {a} vs gpu {b} (relative {err:.2e})"
+
+
+def close_vec(what, a, b):
+    assert a.shape == b.shape, f"{what}: length"
+    scale = 1.0 + (np.abs(a).max() if a.size else 0.0)
+    err = (np.abs(a - b).max() if a.size else 0.0) / scale
+    assert err < 1e-12, f"{what}: max relative difference {err:.2e}"
+
+
+def scaled_pair(p):
+    """The oracle's ScaledProblem, and the same arrays as the GPU's load input."""
+    so = O.ScaledProblem.of(p)
+    d = so.arrays()
+    sg = ScaledProblem(a=d["a"], at=d["at"], c=d["c"], lc=d["lc"], uc=d["uc"], lv=d["lv"], uv=d["uv"],
+                       row_scale=d["row_scale"], col_scale=d["col_scale"], bound_scale=d["bound_scale"],
+                       obj_scale=d["obj_scale"])
+    return so, sg
+
+
+def compare(name, p, gpu):
+    print(f"ops parity: {name} m={p.nrows} n={p.ncols} nnz={p.a.nnz}", flush=True)
+    cpu, sp = scaled_pair(p)
+    m, n = p.nrows, p.ncols
+    pg = gpu.load(sp)
+    up, down = gpu.upload, gpu.download
+    x, y, xa, ya = vector(n, 1), vector(m, 2), vector(n, 3), vector(m, 4)
+    gx, gy, gxa, gya = up(x), up(y), up(xa), up(ya)
+
+    # SpMV.
+    gm, gn = up(np.zeros(m)), up(np.zeros(n))
+    cm = cpu.spmv(x)
+    gpu.spmv(pg, gx, gm)
+    close_vec(f"{name} spmv", cm, down(gm))
+    cn = cpu.spmv_t(y)
+    gpu.spmv_t(pg, gy, gn)
+    close_vec(f"{name} spmv_t", cn, down(gn))
+
+    # Steps.
+    tau = 0.37
+    ct, cb = cpu.primal_step(x, cn, tau)
+    gt, gb = up(np.zeros(n)), up(np.zeros(n))
+    gpu.primal_step(pg, gx, gn, tau, gt, gb)
+    close_vec(f"{name} primal_step x_t", ct, down(gt))
+    close_vec(f"{name} primal_step x_bar", cb, down(gb))
+
+    sigma = 1.9
+    cyt = cpu.dual_step(y, cm, sigma)
+    gyt = up(np.zeros(m))
+    gpu.dual_step(pg, gy, gm, sigma, gyt)
+    close_vec(f"{name} dual_step", cyt, down(gyt))
+
+    cz = cpu.halpern_step(x, ct, xa, 0.8, 1.0)
+    gz = up(x)
+    gpu.halpern_step(gz, gt, gxa, 0.8, 1.0)
+    close_vec(f"{name} halpern_step", cz, down(gz))
+
+    # Vector helpers.
+    close(f"{name} distance_sq", cpu.distance_sq(x, xa), gpu.distance_sq(gx, gxa))
+    close(f"{name} norm_sq", cpu.norm_sq(y), gpu.norm_sq(gy))
+    close(f"{name} dot", cpu.dot(y, ya), gpu.dot(gy, gya))
+    cs = cpu.sub(x, xa)
+    gs = up(np.zeros(n))
+    gpu.sub(gx, gxa, gs)
+    close_vec(f"{name} sub", cs, down(gs))
+    cs = cpu.scale(cs, -1.5)
+    gpu.scale(gs, -1.5)
+    close_vec(f"{name} scale", cs, down(gs))
+
+    # Stats. Use the projected iterate so that the bounds are active.
+    ps = (cpu.primal_stats(ct, cm), gpu.primal_stats(pg, gt, gm))
+    close(f"{name} primal residual", ps[0].residual_sq, ps[1].residual_sq)
+    close(f"{name} primal objective", ps[0].objective, ps[1].objective)
+    ds = (cpu.dual_stats(y, cn), gpu.dual_stats(pg, gy, gn))
+    close(f"{name} dual residual", ds[0].residual_sq, ds[1].residual_sq)
+    close(f"{name} dual objective", ds[0].objective, ds[1].objective)
+    pr = (cpu.primal_ray_stats(x, cm), gpu.primal_ray_stats(pg, gx, gm))
+    close(f"{name} primal ray objective", pr[0].objective, pr[1].objective)
+    close(f"{name} primal ray violation", pr[0].violation_sq, pr[1].violation_sq)
+    close(f"{name} primal ray norm", pr[0].norm_sq, pr[1].norm_sq)
+    dr = (cpu.dual_ray_stats(y, cn), gpu.dual_ray_stats(pg, gy, gn))
+    close(f"{name} dual ray objective", dr[0].objective, dr[1].objective)
+    close(f"{name} dual ray violation", dr[0].violation_sq, dr[1].violation_sq)
+    close(f"{name} dual ray norm", dr[0].norm_sq, dr[1].norm_sq)
+
+    sd = (cpu.step_dual_residual_sq(x, ct, cn, cs, tau), gpu.step_dual_residual_sq(pg, gx, gt, gn, gs, tau))
+    close(f"{name} step dual residual", sd[0], sd[1])
+
+    # Copy and round trip.
+    gc = up(np.zeros(n))
+    gpu.copy(gx, gc)
+    assert np.array_equal(down(gc), x), f"{name} copy"
+
+
+@pytest.mark.parametrize("name", ["afiro", "israel", "25fv47", "80bau3b", "perold"])
+def test_backend_ops_match_cpu_netlib(gpu, name):
+    compare(name, netlib(name), gpu)
+
+
+def test_backend_ops_match_cpu_config0(gpu):
+    compare("config0", generate.config0(), gpu)
+
+
+def test_backend_ops_match_cpu_power_law(gpu):
+    # config1's generator at 1/10 scale: Zipf(1.3) columns and dense rows of
+    # 2k-6k nonzeros, so both stream blocks and multi-workgroup rows run.
+    compare("config1@0.1", generate.config1(scale=0.1), gpu)
+
+
+def test_spmv_is_deterministic(gpu):
+    p = generate.config1(scale=0.05)
+    _, sp = scaled_pair(p)
+    pg = gpu.load(sp)
+    x = gpu.upload(vector(p.ncols, 7))
+    outs = []
+    for _ in range(3):
+        o = gpu.zeros(p.nrows)
+        gpu.spmv(pg, x, o)
+        outs.append(gpu.download(o))
+    assert all(np.array_equal(outs[0], o) for o in outs[1:])
+    v = gpu.upload(vector(p.nrows, 8))
+    d = [gpu.dot(v, v) for _ in range(3)]
+    assert d[0] == d[1] == d[2]
+
+
+def test_short_rows_sum_left_to_right(gpu):
+    # Stream blocks with at least 256 rows reduce each row with one lane, left
+    # to right, exactly as the CPU backend does: the SpMV is then bitwise equal.
+    p = generate.config0()
+    cpu, sp = scaled_pair(p)
+    pg = gpu.load(sp)
+    x = vector(p.ncols, 11)
+    out = gpu.zeros(p.nrows)
+    gpu.spmv(pg, gpu.upload(x), out)
+    g = gpu.download(out)
+    c = cpu.spmv(x)
+    # Not every block qualifies, so require bitwise equality on most rows and
+    # 1e-12 on all.
+    close_vec("spmv", c, g)
+    assert np.mean(g == c) > 0.5
+
+
+def test_shape_errors_are_reported(gpu):
+    from saddlepoint_amd._lib import SP_ERR_SHAPE, BackendError
+
+    p = netlib("afiro")
+    _, sp = scaled_pair(p)
+    pg = gpu.load(sp)
+    with pytest.raises(BackendError) as e:
+        gpu.spmv(pg, gpu.zeros(p.ncols + 1), gpu.zeros(p.nrows))
+    assert e.value.code == SP_ERR_SHAPE
+    with pytest.raises(BackendError) as e:
+        gpu.dot(gpu.zeros(3), gpu.zeros(4))
+    assert e.value.code == SP_ERR_SHAPE
