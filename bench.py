@@ -44,6 +44,11 @@ def kernel_bytes(m: int, n: int, nnz: int) -> dict:
         # A side: values+indices, row pointers, gather x_bar, read y lc uc y_anchor, write y
         "sp_spmv_dual_halpern_f64": 12 * nnz + 4 * (m + 1) + 8 * n + 32 * m + 8 * m,
         "sp_spmv_f64": 12 * nnz + 4 * (m + 1) + 8 * n + 8 * m,
+        # Mixed precision (FP32 values and displacements, FP64 anchors/bounds/c and anchor products):
+        # Aᵀ side: values+indices 8·nnz, row pointers, gather dy, read dx xa c lv uv atya, write dx_bar dx
+        "sp_spmv_t_primal_mixed": 8 * nnz + 4 * (n + 1) + 4 * m + (4 + 8 * 5) * n + 8 * n,
+        # A side: values+indices, row pointers, gather dx_bar, read dy ya lc uc axa, write dy
+        "sp_spmv_dual_mixed": 8 * nnz + 4 * (m + 1) + 4 * n + (4 + 8 * 4) * m + 4 * m,
     }
 
 
@@ -73,6 +78,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=128)
     ap.add_argument("--scale", type=float, default=1.0, help="shrink the workload (tests only)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -91,7 +97,7 @@ def main() -> int:
     b = HipBackend(0)
     log(f"backend: {b.name()}")
     t0 = time.time()
-    s = Solver(b, p, default_params(tolerance=1e-6, fused=1))
+    s = Solver(b, p, default_params(tolerance=1e-6, fused=1, precision=int(args.precision == "mixed")))
     log(f"solver ready (scale + load + power iteration) in {time.time() - t0:.1f}s")
     s.advance(args.warmup)
     b.sync()
@@ -121,7 +127,7 @@ def main() -> int:
             per_kernel[name] = entry
     dom = max((k for k in per_kernel if k in kb), key=lambda k: per_kernel[k]["total_ms"])
     ach = per_kernel[dom]["gbs"]
-    b_iter = 24 * nnz + 52 * m + 68 * n
+    b_iter = 24 * nnz + 52 * m + 68 * n if args.precision == "fp64" else 16 * nnz + 48 * m + 60 * n
     roofline = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
                 "bytes_per_launch": kb[dom]}
@@ -137,7 +143,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic (seeded generator; random instance of the named shape)",
         "config": {"workload": "config1 plain-FP64 single solve: 400k x 900k LP, ~8M nnz, Zipf(1.3) columns + "
                                "20 dense rows", "m": m, "n": n, "nnz": nnz, "scale": args.scale,

@@ -149,6 +149,10 @@ typedef struct {
     int fused; /* 1: between checks, one fused Aᵀ-SpMV+primal+Halpern and one fused
                   A-SpMV+dual+Halpern kernel per iteration (bitwise equal to 0) */
     int log_every_checks; /* 0 = silent (solver.rs:49-50) */
+    int precision;        /* 0: FP64 throughout. 1: mixed (needs fused = 1): between checks the
+                             matrix values, iterates and SpMV sums are FP32, while the Halpern
+                             anchors, every check (KKT, restart residuals, PID, infeasibility)
+                             and the returned solution stay FP64 */
 } sp_params;
 
 /* Status (solver.rs:95-102). */
@@ -163,6 +167,9 @@ typedef struct {
     double total_time_s; /* scaling and upload included */
     double setup_time_s; /* scaling, transpose, upload */
     double final_omega;
+    /* Mixed precision: plain iterations that ran in FP64 after the solver
+       switched over (a near-certificate of infeasibility or a stall). */
+    uint64_t mixed_fp64_iterations;
 } sp_solution;
 
 void sp_default_params(sp_params *p);
@@ -193,6 +200,13 @@ int sp_backend_profile(sp_backend *b, int enable);
  * event time in milliseconds and launch count. SP_ERR_INVALID past the end. */
 int sp_backend_profile_read(sp_backend *b, int index, char *name, int name_len, double *total_ms,
                             int64_t *launches);
+/* Bandwidth probe (the counterpart of the reference's sp-bandwidth,
+ * crates/saddlepoint-cuda/src/probe.rs:13-44): average milliseconds per
+ * launch of kind 0 = copy of n doubles (16 bytes per element), 1 = triad
+ * c = a + s·b over n doubles (24 bytes per element), 2 = n random 8-byte
+ * gathers from a table of table_len doubles (indices streamed, results
+ * written: 12 bytes per element plus the gather). */
+int sp_probe(sp_backend *b, int kind, int64_t n, int64_t table_len, int reps, double *ms_per_launch);
 /* Waits for all work queued on the backend's stream. */
 int sp_backend_sync(sp_backend *b);
 

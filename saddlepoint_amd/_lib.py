@@ -63,13 +63,14 @@ class Params(C.Structure):
                 ("power_iterations", _i64), ("geometric_mean_passes", _i64), ("ruiz_iterations", _i64),
                 ("has_pock_chambolle", C.c_int), ("pock_chambolle_alpha", _d),
                 ("bound_objective_rescaling", C.c_int), ("termination", C.c_int), ("fused", C.c_int),
-                ("log_every_checks", C.c_int)]
+                ("log_every_checks", C.c_int), ("precision", C.c_int)]
 
 
 class Solution(C.Structure):
     _fields_ = [("status", C.c_int), ("objective", _d), ("dual_objective", _d), ("primal_residual", _d),
                 ("dual_residual", _d), ("gap", _d), ("iterations", _u64), ("restarts", _u64),
-                ("solve_time_s", _d), ("total_time_s", _d), ("setup_time_s", _d), ("final_omega", _d)]
+                ("solve_time_s", _d), ("total_time_s", _d), ("setup_time_s", _d), ("final_omega", _d),
+                ("mixed_fp64_iterations", _u64)]
 
 
 # name -> (restype, argtypes): every function include/saddlepoint_hip.h declares.
@@ -110,6 +111,7 @@ SIGNATURES = {
     "sp_backend_profile": (C.c_int, [_vp, C.c_int]),
     "sp_backend_profile_read": (C.c_int, [_vp, C.c_int, C.c_char_p, C.c_int, _pd, C.POINTER(_i64)]),
     "sp_backend_sync": (C.c_int, [_vp]),
+    "sp_probe": (C.c_int, [_vp, C.c_int, _i64, _i64, C.c_int, _pd]),
     "sp_scale_problem": (C.c_int, [C.POINTER(LpView), C.POINTER(Params), _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd,
                                    _pd, _pd]),
 }

@@ -237,6 +237,13 @@ class HipBackend:
     def sync(self) -> None:
         self._ok(lib().sp_backend_sync(self.h))
 
+    def probe(self, kind: str, n: int, table_len: int = 0, reps: int = 20) -> float:
+        """Milliseconds per launch of a bandwidth probe: "copy", "triad" or "gather"."""
+        ms = C.c_double()
+        k = {"copy": 0, "triad": 1, "gather": 2}[kind]
+        self._ok(lib().sp_probe(self.h, k, int(n), int(table_len), int(reps), C.byref(ms)))
+        return ms.value
+
     def distance_sq(self, a: Vector, b: Vector) -> float:
         return self._scalar(lib().sp_distance_sq, a.h, b.h)
 
@@ -295,6 +302,7 @@ class SolveResult:
     total_time: float
     setup_time: float
     final_omega: float
+    mixed_fp64_iterations: int = 0
 
 
 def solve(backend: HipBackend, p: LpProblem, params: Params | None = None, **kw) -> SolveResult:
@@ -309,7 +317,8 @@ def solve(backend: HipBackend, p: LpProblem, params: Params | None = None, **kw)
     backend._ok(rc)
     return SolveResult(STATUS[sol.status], x, y, sol.objective, sol.dual_objective, sol.primal_residual,
                        sol.dual_residual, sol.gap, int(sol.iterations), int(sol.restarts), sol.solve_time_s,
-                       sol.total_time_s, sol.setup_time_s, sol.final_omega)
+                       sol.total_time_s, sol.setup_time_s, sol.final_omega,
+                       int(sol.mixed_fp64_iterations))
 
 
 class Solver:
@@ -339,7 +348,8 @@ class Solver:
         self.b._ok(lib().sp_solver_result(self.h, C.byref(sol), px, py))
         return SolveResult(STATUS[sol.status], x, y, sol.objective, sol.dual_objective, sol.primal_residual,
                            sol.dual_residual, sol.gap, int(sol.iterations), int(sol.restarts), sol.solve_time_s,
-                           sol.total_time_s, sol.setup_time_s, sol.final_omega)
+                           sol.total_time_s, sol.setup_time_s, sol.final_omega,
+                       int(sol.mixed_fp64_iterations))
 
     def close(self):
         if getattr(self, "h", None):

@@ -24,6 +24,7 @@ namespace {
 constexpr int kWG = 256;
 constexpr int kChunk = 4096;  // SP_CHUNK in kernels.hip
 constexpr int kStreamNnzF64 = 2048;  // SP_NNZ_F64 in kernels.hip
+constexpr int kStreamRows = 4 * 256;  // SP_RPT * WG in kernels.hip
 
 template <class T>
 T* dmalloc(size_t count) {
@@ -81,6 +82,18 @@ unsigned ew_grid(int64_t n) {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, 8192));
 }
 
+// One SpMV with a fused epilogue: the SpMV kernel, then, if the matrix has
+// rows of several chunks, the finishing kernel with the same epilogue
+// arguments.
+template <class... E>
+void spmv_launch(sp_backend* b, hipFunction_t k, hipFunction_t kfin, const DevCsr& a, const void* va,
+                 const void* xin, E... epi) {
+    if (a.nblocks) launch(b, k, (unsigned)a.nblocks, a.desc, a.rp, a.ci, va, xin, (void*)a.partials, epi...);
+    if (a.nlong)
+        launch(b, kfin, (unsigned)((a.nlong + 3) / 4), (const LongRow*)a.longs, a.nlong, (const void*)a.partials,
+               epi...);
+}
+
 void need_len(const sp_vec* v, int64_t len, const char* what) {
     if (!v) throw Error(SP_ERR_INVALID, std::string(what) + ": null vector");
     if (v->len != len)
@@ -102,29 +115,31 @@ void upload_csr(sp_backend* b, DevCsr& d, const int64_t* row_ptr, const int64_t*
     d.rp = dupload(b->stream, rp.data(), rp.size());
     d.ci = dupload(b->stream, ci.data(), ci.size());
     d.va = dupload(b->stream, values, (size_t)nnz);
-    std::vector<Desc> desc = plan_spmv(row_ptr, nrows, d.tuning, &d.npartials);
-    d.nblocks = (int)desc.size();
-    d.desc = dupload(b->stream, desc.data(), desc.size());
+    SpmvPlan plan = plan_spmv(row_ptr, nrows, d.tuning);
+    d.nblocks = (int)plan.desc.size();
+    d.desc = dupload(b->stream, plan.desc.data(), plan.desc.size());
+    d.npartials = plan.npartials;
     d.partials = dmalloc<double>((size_t)std::max(1, d.npartials));
-    d.tickets = dmalloc<unsigned>((size_t)std::max(1, d.npartials));
-    hip_check(hipMemsetAsync(d.tickets, 0, sizeof(unsigned) * (size_t)std::max(1, d.npartials), b->stream),
-              "hipMemsetAsync");
+    d.nlong = (int)plan.longs.size();
+    d.longs = dupload(b->stream, plan.longs.data(), plan.longs.size());
     // The host staging vectors die at return: finish the copies first.
     hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
 }
 
 void free_csr(DevCsr& d) {
-    dfree(d.rp), dfree(d.ci), dfree(d.va), dfree(d.desc), dfree(d.partials), dfree(d.tickets);
+    dfree(d.rp), dfree(d.ci), dfree(d.va), dfree(d.va32), dfree(d.desc), dfree(d.partials), dfree(d.longs);
     d = DevCsr{};
 }
 
 }  // namespace
 
-std::vector<Desc> plan_spmv(const int64_t* rp, int64_t nrows, const SpmvTuning& t, int* npartials) {
-    std::vector<Desc> out;
+SpmvPlan plan_spmv(const int64_t* rp, int64_t nrows, const SpmvTuning& t) {
+    SpmvPlan plan;
+    std::vector<Desc>& out = plan.desc;
     int np = 0;
-    const int64_t block_nnz = std::min<int64_t>(t.block_nnz, kStreamNnzF64);
-    const int64_t long_row = std::min<int64_t>(t.long_row, block_nnz);
+    const int64_t block_nnz = std::max<int64_t>(1, std::min<int64_t>(t.block_nnz, kStreamNnzF64));
+    const int64_t long_row = std::max<int64_t>(1, std::min<int64_t>(t.long_row, block_nnz));
+    const int64_t max_rows = std::max<int64_t>(1, std::min<int64_t>(t.max_rows, kStreamRows));
     int64_t r = 0;
     while (r < nrows) {
         const int64_t len = rp[r + 1] - rp[r];
@@ -137,18 +152,21 @@ std::vector<Desc> plan_spmv(const int64_t* rp, int64_t nrows, const SpmvTuning& 
                     const int64_t k0 = rp[r] + c * kChunk, k1 = std::min(rp[r] + (c + 1) * kChunk, rp[r + 1]);
                     out.push_back(Desc{(int)r, -1 - (np + (int)c), (int)k0, (int)k1});
                 }
+                plan.longs.push_back(LongRow{(int)r, np, (int)nch, 0});
                 np += (int)nch;
             }
             ++r;
             continue;
         }
+        // A stream block: consecutive rows while the nonzeros and rows fit
+        // (the merge-path reduction balances any mix of row lengths).
         const int64_t start = r, k0 = rp[r];
-        while (r < nrows && rp[r + 1] - rp[r] <= long_row && rp[r + 1] - k0 <= block_nnz && r - start < t.max_rows)
+        while (r < nrows && rp[r + 1] - rp[r] <= long_row && rp[r + 1] - k0 <= block_nnz && r - start < max_rows)
             ++r;
         out.push_back(Desc{(int)start, (int)r, (int)k0, (int)rp[r]});
     }
-    *npartials = np;
-    return out;
+    plan.npartials = np;
+    return plan;
 }
 
 void profile_collect(sp_backend* b) {
@@ -173,6 +191,12 @@ void free_problem(sp_problem* p) {
     free_csr(p->at);
     for (double* v : {p->c, p->lc, p->uc, p->lv, p->uv, p->pf, p->df}) dfree(v);
     delete p;
+}
+
+void free_vec32(sp_vec32* v) {
+    if (!v) return;
+    dfree(v->d);
+    delete v;
 }
 
 void free_vec(sp_vec* v) {
@@ -279,6 +303,117 @@ void Hip::copy(const sp_vec* src, sp_vec* dst) {
                   "hipMemcpyAsync D2D");
 }
 
+// ---- mixed precision -------------------------------------------------------------------
+
+void Hip::ensure_fp32(sp_problem* p) {
+    if (p->has_fp32) return;
+    for (DevCsr* a : {&p->a, &p->at}) {
+        a->va32 = dmalloc<float>((size_t)a->nnz);
+        if (a->nnz) launch(b, b->k_to_f32, ew_grid(a->nnz), (const double*)a->va, a->va32, (long long)a->nnz);
+    }
+    p->has_fp32 = true;
+}
+
+sp_vec32* Hip::zeros32(int64_t len) {
+    auto* v = new sp_vec32;
+    v->len = len;
+    try {
+        v->d = dmalloc<float>((size_t)len);
+        hip_check(hipMemsetAsync(v->d, 0, sizeof(float) * (size_t)std::max<int64_t>(len, 1), b->stream),
+                  "hipMemsetAsync");
+    } catch (...) {
+        free_vec32(v);
+        throw;
+    }
+    return v;
+}
+
+void Hip::delta_from(const sp_vec* z, const sp_vec* anchor, sp_vec32* dz) {
+    if (!z || !anchor || !dz || z->len != dz->len || anchor->len != dz->len)
+        throw Error(SP_ERR_SHAPE, "delta_from: length mismatch");
+    if (z->len)
+        launch(b, b->k_delta_from, ew_grid(z->len), (const double*)z->d, (const double*)anchor->d, dz->d,
+               (long long)z->len);
+}
+
+void Hip::delta_to(const sp_vec32* dz, const sp_vec* anchor, sp_vec* z) {
+    if (!z || !anchor || !dz || z->len != dz->len || anchor->len != dz->len)
+        throw Error(SP_ERR_SHAPE, "delta_to: length mismatch");
+    if (z->len)
+        launch(b, b->k_delta_to, ew_grid(z->len), (const float*)dz->d, (const double*)anchor->d, z->d,
+               (long long)z->len);
+}
+
+void Hip::plain_iteration_mixed(const sp_problem* p, sp_vec32* dx, sp_vec32* dy, const sp_vec* x_anchor,
+                                const sp_vec* y_anchor, const sp_vec* atya, const sp_vec* axa, sp_vec32* dx_bar,
+                                double tau, double sigma, double w, double rho) {
+    if (!p->has_fp32) throw Error(SP_ERR_INVALID, "plain_iteration_mixed: FP32 values not prepared");
+    const DevCsr& at = p->at;
+    const DevCsr& a = p->a;
+    spmv_launch(b, b->k_spmv_t_primal_mixed, b->k_fin_t_primal_mixed, at, at.va32, dy->d, dx->d,
+                (const double*)x_anchor->d, (const double*)p->c, (const double*)p->lv, (const double*)p->uv,
+                (const double*)atya->d, tau, w, rho, dx_bar->d);
+    spmv_launch(b, b->k_spmv_dual_mixed, b->k_fin_dual_mixed, a, a.va32, dx_bar->d, dy->d,
+                (const double*)y_anchor->d, (const double*)p->lc, (const double*)p->uc, (const double*)axa->d, sigma,
+                w, rho);
+}
+
+// ---- bandwidth probe ---------------------------------------------------------------------
+
+double Hip::probe(int kind, int64_t n, int64_t table_len, int reps, uint64_t seed) {
+    if (n <= 0 || reps <= 0 || (kind == 2 && table_len <= 0)) throw Error(SP_ERR_INVALID, "probe: bad size");
+    double *a = nullptr, *bb = nullptr, *c = nullptr, *table = nullptr;
+    int* idx = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms = 0.0f;
+    try {
+        a = dmalloc<double>((size_t)n);
+        c = dmalloc<double>((size_t)n);
+        launch(b, b->k_fill, ew_grid(n), a, 1.0, (long long)n);
+        if (kind == 1) {
+            bb = dmalloc<double>((size_t)n);
+            launch(b, b->k_fill, ew_grid(n), bb, 2.0, (long long)n);
+        }
+        if (kind == 2) {
+            table = dmalloc<double>((size_t)table_len);
+            launch(b, b->k_fill, ew_grid(table_len), table, 3.0, (long long)table_len);
+            std::vector<int> h((size_t)n);
+            uint64_t st = seed * 0x9E3779B97F4A7C15ull | 1;
+            for (auto& v : h) {
+                st ^= st << 13, st ^= st >> 7, st ^= st << 17;
+                v = (int)(st % (uint64_t)table_len);
+            }
+            idx = dupload(b->stream, h.data(), h.size());
+            hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        }
+        const unsigned grid = (unsigned)std::min<int64_t>((n + kWG - 1) / kWG, 65536);
+        auto once = [&] {
+            if (kind == 0) launch(b, b->k_probe_copy, grid, (const double*)a, c, (long long)n);
+            else if (kind == 1) launch(b, b->k_probe_triad, grid, (const double*)a, (const double*)bb, c, 3.0, (long long)n);
+            else launch(b, b->k_probe_gather, grid, (const int*)idx, (const double*)table, c, (long long)n);
+        };
+        once();
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        hip_check(hipEventRecord(e0, b->stream), "hipEventRecord");
+        for (int r = 0; r < reps; ++r) once();
+        hip_check(hipEventRecord(e1, b->stream), "hipEventRecord");
+        hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    } catch (...) {
+        (void)hipStreamSynchronize(b->stream);
+        for (void* q : {(void*)a, (void*)bb, (void*)c, (void*)table, (void*)idx}) dfree(q);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipStreamSynchronize(b->stream);
+    for (void* q : {(void*)a, (void*)bb, (void*)c, (void*)table, (void*)idx}) dfree(q);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return (double)ms / reps;
+}
+
 void Hip::sync() { hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize"); }
 
 // ---- SpMV --------------------------------------------------------------------
@@ -286,19 +421,13 @@ void Hip::sync() { hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchron
 void Hip::spmv(const sp_problem* p, const sp_vec* x, sp_vec* out) {
     need_len(x, p->n, "spmv x");
     need_len(out, p->m, "spmv out");
-    const DevCsr& a = p->a;
-    if (a.nblocks)
-        launch(b, b->k_spmv, (unsigned)a.nblocks, a.desc, a.rp, a.ci, a.va, (const double*)x->d, a.partials,
-               a.tickets, out->d);
+    spmv_launch(b, b->k_spmv, b->k_fin, p->a, p->a.va, x->d, out->d);
 }
 
 void Hip::spmv_t(const sp_problem* p, const sp_vec* y, sp_vec* out) {
     need_len(y, p->m, "spmv_t y");
     need_len(out, p->n, "spmv_t out");
-    const DevCsr& a = p->at;
-    if (a.nblocks)
-        launch(b, b->k_spmv, (unsigned)a.nblocks, a.desc, a.rp, a.ci, a.va, (const double*)y->d, a.partials,
-               a.tickets, out->d);
+    spmv_launch(b, b->k_spmv, b->k_fin, p->at, p->at.va, y->d, out->d);
 }
 
 void Hip::spmv_t_primal(const sp_problem* p, const sp_vec* y, const sp_vec* x, double tau, sp_vec* aty,
@@ -306,36 +435,25 @@ void Hip::spmv_t_primal(const sp_problem* p, const sp_vec* y, const sp_vec* x, d
     need_len(y, p->m, "spmv_t_primal y");
     for (const sp_vec* v : {x, (const sp_vec*)aty, (const sp_vec*)x_t, (const sp_vec*)x_bar})
         need_len(v, p->n, "spmv_t_primal");
-    const DevCsr& a = p->at;
-    if (a.nblocks)
-        launch(b, b->k_spmv_t_primal, (unsigned)a.nblocks, a.desc, a.rp, a.ci, a.va, (const double*)y->d,
-               a.partials, a.tickets, (const double*)x->d, (const double*)p->c, (const double*)p->lv,
-               (const double*)p->uv, tau, aty->d, x_t->d, x_bar->d);
+    spmv_launch(b, b->k_spmv_t_primal, b->k_fin_t_primal, p->at, p->at.va, y->d, (const double*)x->d,
+                (const double*)p->c, (const double*)p->lv, (const double*)p->uv, tau, aty->d, x_t->d, x_bar->d);
 }
 
 void Hip::spmv_dual(const sp_problem* p, const sp_vec* x_bar, const sp_vec* y, double sigma, sp_vec* ax,
                     sp_vec* y_t) {
     need_len(x_bar, p->n, "spmv_dual x_bar");
     for (const sp_vec* v : {y, (const sp_vec*)ax, (const sp_vec*)y_t}) need_len(v, p->m, "spmv_dual");
-    const DevCsr& a = p->a;
-    if (a.nblocks)
-        launch(b, b->k_spmv_dual, (unsigned)a.nblocks, a.desc, a.rp, a.ci, a.va, (const double*)x_bar->d,
-               a.partials, a.tickets, (const double*)y->d, (const double*)p->lc, (const double*)p->uc, sigma,
-               ax->d, y_t->d);
+    spmv_launch(b, b->k_spmv_dual, b->k_fin_dual, p->a, p->a.va, x_bar->d, (const double*)y->d,
+                (const double*)p->lc, (const double*)p->uc, sigma, ax->d, y_t->d);
 }
 
 void Hip::plain_iteration(const sp_problem* p, sp_vec* x, sp_vec* y, const sp_vec* x_anchor,
                           const sp_vec* y_anchor, sp_vec* x_bar, double tau, double sigma, double w, double rho) {
-    const DevCsr& at = p->at;
-    const DevCsr& a = p->a;
-    if (at.nblocks)
-        launch(b, b->k_spmv_t_primal_halpern, (unsigned)at.nblocks, at.desc, at.rp, at.ci, at.va,
-               (const double*)y->d, at.partials, at.tickets, x->d, (const double*)p->c, (const double*)p->lv,
-               (const double*)p->uv, (const double*)x_anchor->d, tau, w, rho, x_bar->d);
-    if (a.nblocks)
-        launch(b, b->k_spmv_dual_halpern, (unsigned)a.nblocks, a.desc, a.rp, a.ci, a.va, (const double*)x_bar->d,
-               a.partials, a.tickets, y->d, (const double*)p->lc, (const double*)p->uc, (const double*)y_anchor->d,
-               sigma, w, rho);
+    spmv_launch(b, b->k_spmv_t_primal_halpern, b->k_fin_t_primal_halpern, p->at, p->at.va, y->d, x->d,
+                (const double*)p->c, (const double*)p->lv, (const double*)p->uv, (const double*)x_anchor->d, tau, w,
+                rho, x_bar->d);
+    spmv_launch(b, b->k_spmv_dual_halpern, b->k_fin_dual_halpern, p->a, p->a.va, x_bar->d, y->d,
+                (const double*)p->lc, (const double*)p->uc, (const double*)y_anchor->d, sigma, w, rho);
 }
 
 // ---- element-wise ------------------------------------------------------------------
@@ -381,11 +499,12 @@ void Hip::scale(sp_vec* v, double alpha) {
 
 int Hip::reduce_grid(int64_t n) const {
     // A fixed function of n, so a given call always reduces in the same order.
-    int64_t g = (n + 4 * kWG - 1) / (4 * kWG);
+    int64_t g = (n + 8 * kWG - 1) / (8 * kWG);
     return (int)std::max<int64_t>(1, std::min<int64_t>(g, kMaxReduceBlocks));
 }
 
-const double* Hip::read_result(int k) {
+const double* Hip::read_result(int k, int grid) {
+    if (grid > 1) launch(b, b->k_red_finish, 1u, (const double*)b->red_partials, grid, k, b->red_result);
     hip_check(hipMemcpyAsync(b->host_result, b->red_result, sizeof(double) * (size_t)k, hipMemcpyDeviceToHost,
                              b->stream),
               "hipMemcpyAsync D2H");
@@ -397,8 +516,8 @@ double Hip::dot(const sp_vec* a, const sp_vec* bv) {
     if (!a) throw Error(SP_ERR_INVALID, "dot: null vector");
     need_len(bv, a->len, "dot");
     launch(b, b->k_dot, (unsigned)reduce_grid(a->len), (const double*)a->d, (const double*)bv->d, (long long)a->len,
-           b->red_partials, b->red_ticket, b->red_result);
-    return read_result(1)[0];
+           b->red_partials, b->red_result);
+    return read_result(1, reduce_grid(a->len))[0];
 }
 
 double Hip::norm_sq(const sp_vec* v) { return dot(v, v); }
@@ -407,8 +526,8 @@ double Hip::distance_sq(const sp_vec* a, const sp_vec* bv) {
     if (!a) throw Error(SP_ERR_INVALID, "distance_sq: null vector");
     need_len(bv, a->len, "distance_sq");
     launch(b, b->k_dist, (unsigned)reduce_grid(a->len), (const double*)a->d, (const double*)bv->d,
-           (long long)a->len, b->red_partials, b->red_ticket, b->red_result);
-    return read_result(1)[0];
+           (long long)a->len, b->red_partials, b->red_result);
+    return read_result(1, reduce_grid(a->len))[0];
 }
 
 sp_primal_stats_t Hip::primal_stats(const sp_problem* p, const sp_vec* x, const sp_vec* ax) {
@@ -416,8 +535,8 @@ sp_primal_stats_t Hip::primal_stats(const sp_problem* p, const sp_vec* x, const 
     need_len(ax, p->m, "primal_stats ax");
     launch(b, b->k_primal_stats, (unsigned)reduce_grid(p->m + p->n), (const double*)ax->d, (const double*)p->lc,
            (const double*)p->uc, (const double*)p->pf, (long long)p->m, (const double*)p->c, (const double*)x->d,
-           (long long)p->n, b->red_partials, b->red_ticket, b->red_result);
-    const double* r = read_result(2);
+           (long long)p->n, b->red_partials, b->red_result);
+    const double* r = read_result(2, reduce_grid(p->m + p->n));
     return sp_primal_stats_t{r[0], r[1]};
 }
 
@@ -426,9 +545,8 @@ sp_dual_stats_t Hip::dual_stats(const sp_problem* p, const sp_vec* y, const sp_v
     need_len(aty, p->n, "dual_stats aty");
     launch(b, b->k_dual_stats, (unsigned)reduce_grid(p->m + p->n), (const double*)y->d, (const double*)p->lc,
            (const double*)p->uc, (long long)p->m, (const double*)p->c, (const double*)aty->d, (const double*)p->lv,
-           (const double*)p->uv, (const double*)p->df, (long long)p->n, b->red_partials, b->red_ticket,
-           b->red_result);
-    const double* r = read_result(3);
+           (const double*)p->uv, (const double*)p->df, (long long)p->n, b->red_partials, b->red_result);
+    const double* r = read_result(3, reduce_grid(p->m + p->n));
     // cpu/src/lib.rs:223: objective = rows + cols.
     return sp_dual_stats_t{r[1], r[0] + r[2]};
 }
@@ -439,8 +557,8 @@ double Hip::step_dual_residual_sq(const sp_problem* p, const sp_vec* x, const sp
     const double inv_tau = 1.0 / tau;  // cpu/src/lib.rs:235: multiply, don't divide
     launch(b, b->k_step_dual, (unsigned)reduce_grid(p->n), (const double*)x->d, (const double*)x_t->d,
            (const double*)aty->d, (const double*)aty_t->d, (const double*)p->df, inv_tau, (long long)p->n,
-           b->red_partials, b->red_ticket, b->red_result);
-    return read_result(1)[0];
+           b->red_partials, b->red_result);
+    return read_result(1, reduce_grid(p->n))[0];
 }
 
 sp_ray_stats_t Hip::primal_ray_stats(const sp_problem* p, const sp_vec* dx, const sp_vec* adx) {
@@ -448,8 +566,8 @@ sp_ray_stats_t Hip::primal_ray_stats(const sp_problem* p, const sp_vec* dx, cons
     need_len(adx, p->m, "primal_ray_stats adx");
     launch(b, b->k_primal_ray, (unsigned)reduce_grid(p->m + p->n), (const double*)adx->d, (const double*)p->lc,
            (const double*)p->uc, (long long)p->m, (const double*)dx->d, (const double*)p->c, (const double*)p->lv,
-           (const double*)p->uv, (long long)p->n, b->red_partials, b->red_ticket, b->red_result);
-    const double* r = read_result(4);
+           (const double*)p->uv, (long long)p->n, b->red_partials, b->red_result);
+    const double* r = read_result(4, reduce_grid(p->m + p->n));
     // cpu/src/lib.rs:245: violation = rows + cols.
     return sp_ray_stats_t{r[3], r[0] + r[1], r[2]};
 }
@@ -459,8 +577,8 @@ sp_ray_stats_t Hip::dual_ray_stats(const sp_problem* p, const sp_vec* dy, const 
     need_len(atdy, p->n, "dual_ray_stats atdy");
     launch(b, b->k_dual_ray, (unsigned)reduce_grid(p->m + p->n), (const double*)dy->d, (const double*)p->lc,
            (const double*)p->uc, (long long)p->m, (const double*)atdy->d, (const double*)p->lv, (const double*)p->uv,
-           (long long)p->n, b->red_partials, b->red_ticket, b->red_result);
-    const double* r = read_result(5);
+           (long long)p->n, b->red_partials, b->red_result);
+    const double* r = read_result(5, reduce_grid(p->m + p->n));
     // cpu/src/lib.rs:258: objective = row_obj + col_obj, violation = row_viol + col_viol.
     return sp_ray_stats_t{r[1] + r[4], r[0] + r[3], r[2]};
 }
@@ -511,6 +629,15 @@ sp_backend* sp_backend_open(int ordinal) {
             b->kernel_launches.push_back(0);
         };
         get(b->k_spmv, "sp_spmv_f64");
+        get(b->k_fin, "sp_finish_f64");
+        get(b->k_fin_t_primal, "sp_finish_t_primal_f64");
+        get(b->k_fin_t_primal_halpern, "sp_finish_t_primal_halpern_f64");
+        get(b->k_fin_dual, "sp_finish_dual_f64");
+        get(b->k_fin_dual_halpern, "sp_finish_dual_halpern_f64");
+        get(b->k_fin32, "sp_finish_f32");
+        get(b->k_fin_t_primal_mixed, "sp_finish_t_primal_mixed");
+        get(b->k_fin_dual_mixed, "sp_finish_dual_mixed");
+        get(b->k_red_finish, "sp_red_finish");
         get(b->k_spmv_t_primal, "sp_spmv_t_primal_f64");
         get(b->k_spmv_t_primal_halpern, "sp_spmv_t_primal_halpern_f64");
         get(b->k_spmv_dual, "sp_spmv_dual_f64");
@@ -521,6 +648,15 @@ sp_backend* sp_backend_open(int ordinal) {
         get(b->k_sub, "sp_sub_f64");
         get(b->k_scale, "sp_scale_f64");
         get(b->k_fill, "sp_fill_f64");
+        get(b->k_spmv32, "sp_spmv_f32");
+        get(b->k_probe_copy, "sp_probe_copy");
+        get(b->k_probe_triad, "sp_probe_triad");
+        get(b->k_probe_gather, "sp_probe_gather");
+        get(b->k_spmv_t_primal_mixed, "sp_spmv_t_primal_mixed");
+        get(b->k_spmv_dual_mixed, "sp_spmv_dual_mixed");
+        get(b->k_to_f32, "sp_f64_to_f32");
+        get(b->k_delta_from, "sp_delta_from");
+        get(b->k_delta_to, "sp_delta_to");
         get(b->k_dot, "sp_red_dot_f64");
         get(b->k_dist, "sp_red_distance_sq_f64");
         get(b->k_primal_stats, "sp_red_primal_stats_f64");
@@ -529,9 +665,7 @@ sp_backend* sp_backend_open(int ordinal) {
         get(b->k_primal_ray, "sp_red_primal_ray_f64");
         get(b->k_dual_ray, "sp_red_dual_ray_f64");
         b->red_partials = sp::dmalloc<double>((size_t)sp::Hip::kMaxReduceBlocks * sp::Hip::kMaxSlots);
-        b->red_ticket = sp::dmalloc<unsigned>(1);
         b->red_result = sp::dmalloc<double>(sp::Hip::kMaxSlots);
-        sp::hip_check(hipMemsetAsync(b->red_ticket, 0, sizeof(unsigned), b->stream), "hipMemsetAsync");
         sp::hip_check(hipHostMalloc(reinterpret_cast<void**>(&b->host_result), sizeof(double) * sp::Hip::kMaxSlots,
                                     hipHostMallocDefault),
                       "hipHostMalloc");
@@ -549,7 +683,6 @@ void sp_backend_close(sp_backend* b) {
     for (const auto& pe : b->pending) b->event_pool.push_back(pe.start), b->event_pool.push_back(pe.stop);
     for (hipEvent_t e : b->event_pool) (void)hipEventDestroy(e);
     sp::dfree(b->red_partials);
-    sp::dfree(b->red_ticket);
     sp::dfree(b->red_result);
     if (b->host_result) (void)hipHostFree(b->host_result);
     if (b->module) (void)hipModuleUnload(b->module);

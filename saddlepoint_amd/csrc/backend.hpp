@@ -19,11 +19,17 @@ struct Desc {
     int r0, r1, k0, k1;
 };
 
+// A row split into several chunks: the finishing kernel adds its chunk sums
+// partials[pbase .. pbase + nch). Mirrors `struct LongRow` in kernels.hip.
+struct LongRow {
+    int row, pbase, nch, pad;
+};
+
 // How an SpMV is cut into work items (see plan_spmv). The autotuner searches
 // these knobs per matrix.
 struct SpmvTuning {
     int block_nnz = 2048;  // nonzeros staged per stream workgroup (<= kernel LDS capacity)
-    int long_row = 256;    // rows longer than this get workgroups of their own
+    int long_row = 2048;   // rows longer than this get workgroups of their own
     int max_rows = 1024;   // rows per stream workgroup
 };
 
@@ -34,16 +40,23 @@ struct DevCsr {
     int* rp = nullptr;
     int* ci = nullptr;
     double* va = nullptr;
+    float* va32 = nullptr;  // FP32 copy of the values, made on first mixed-precision use
     Desc* desc = nullptr;
     int nblocks = 0;
     double* partials = nullptr;  // one per chunk of a multi-chunk row
-    unsigned* tickets = nullptr;
     int npartials = 0;
+    LongRow* longs = nullptr;    // the multi-chunk rows
+    int nlong = 0;
     SpmvTuning tuning;
 };
 
 // Host-side planner: cuts rows into stream blocks and long-row chunks.
-std::vector<Desc> plan_spmv(const int64_t* row_ptr, int64_t nrows, const SpmvTuning& t, int* npartials);
+struct SpmvPlan {
+    std::vector<Desc> desc;
+    std::vector<LongRow> longs;
+    int npartials = 0;
+};
+SpmvPlan plan_spmv(const int64_t* row_ptr, int64_t nrows, const SpmvTuning& t);
 
 }  // namespace sp
 
@@ -59,6 +72,13 @@ struct sp_problem {
     sp::DevCsr a, at;
     double *c = nullptr, *lc = nullptr, *uc = nullptr, *lv = nullptr, *uv = nullptr;
     double *pf = nullptr, *df = nullptr;
+    bool has_fp32 = false;  // a.va32 / at.va32 are made on first mixed-precision use
+};
+
+// An FP32 device vector (driver-internal: the mixed-precision shadows).
+struct sp_vec32 {
+    float* d = nullptr;
+    int64_t len = 0;
 };
 
 struct sp_backend {
@@ -68,10 +88,9 @@ struct sp_backend {
     int cus = 0;
     std::string name;
     std::string err;
-    // Reduction scratch: per-workgroup partials, the arrival ticket, the
-    // result slots and their pinned host mirror.
+    // Reduction scratch: per-workgroup partials, the result slots and their
+    // pinned host mirror.
     double* red_partials = nullptr;
-    unsigned* red_ticket = nullptr;
     double* red_result = nullptr;
     double* host_result = nullptr;
 
@@ -89,8 +108,12 @@ struct sp_backend {
     std::vector<double> kernel_ms;
     std::vector<int64_t> kernel_launches;
 
+    hipFunction_t k_fin, k_fin_t_primal, k_fin_t_primal_halpern, k_fin_dual, k_fin_dual_halpern, k_fin32,
+        k_fin_t_primal_mixed, k_fin_dual_mixed, k_red_finish;
     hipFunction_t k_spmv, k_spmv_t_primal, k_spmv_t_primal_halpern, k_spmv_dual, k_spmv_dual_halpern;
     hipFunction_t k_primal_step, k_dual_step, k_halpern, k_sub, k_scale, k_fill;
+    hipFunction_t k_probe_copy, k_probe_triad, k_probe_gather;
+    hipFunction_t k_spmv32, k_spmv_t_primal_mixed, k_spmv_dual_mixed, k_to_f32, k_delta_from, k_delta_to;
     hipFunction_t k_dot, k_dist, k_primal_stats, k_dual_stats, k_step_dual, k_primal_ray, k_dual_ray;
 };
 
@@ -146,15 +169,32 @@ struct Hip {
     void plain_iteration(const sp_problem* p, sp_vec* x, sp_vec* y, const sp_vec* x_anchor, const sp_vec* y_anchor,
                          sp_vec* x_bar, double tau, double sigma, double w, double rho);
 
+    // Mixed precision (sp_params.precision = 1): iterates as FP64 anchor + FP32
+    // displacement between checks.
+    void ensure_fp32(sp_problem* p);
+    sp_vec32* zeros32(int64_t len);
+    void delta_from(const sp_vec* z, const sp_vec* anchor, sp_vec32* dz);  // dz = z - anchor
+    void delta_to(const sp_vec32* dz, const sp_vec* anchor, sp_vec* z);    // z = anchor + dz
+    // plain_iteration in displacement form; atya = A'ᵀy_anchor, axa = A'x_anchor.
+    void plain_iteration_mixed(const sp_problem* p, sp_vec32* dx, sp_vec32* dy, const sp_vec* x_anchor,
+                               const sp_vec* y_anchor, const sp_vec* atya, const sp_vec* axa, sp_vec32* dx_bar,
+                               double tau, double sigma, double w, double rho);
+
+    // Average milliseconds of one probe kernel over `reps` launches: kind 0
+    // copies n doubles, 1 is a triad over n doubles, 2 gathers n doubles at
+    // pseudo-random positions of a table of table_len doubles.
+    double probe(int kind, int64_t n, int64_t table_len, int reps, uint64_t seed);
+
     void sync();
 
    private:
     int reduce_grid(int64_t n) const;
-    const double* read_result(int k);
+    const double* read_result(int k, int grid);
 };
 
 void free_problem(sp_problem* p);
 void free_vec(sp_vec* v);
+void free_vec32(sp_vec32* v);
 // Folds finished event pairs into the per-kernel totals (synchronizes).
 void profile_collect(sp_backend* b);
 

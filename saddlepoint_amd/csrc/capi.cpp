@@ -39,12 +39,17 @@ struct VecDel {
 struct ProbDel {
     void operator()(sp_problem* p) const { sp::free_problem(p); }
 };
+struct Vec32Del {
+    void operator()(sp_vec32* v) const { sp::free_vec32(v); }
+};
 
 // The backend as the driver sees it: owning handles for vectors and problems.
 struct DriverBackend : sp::Hip {
     using Vector = std::unique_ptr<sp_vec, VecDel>;
     using Problem = std::unique_ptr<sp_problem, ProbDel>;
+    using Vector32 = std::unique_ptr<sp_vec32, Vec32Del>;
     Problem load(const sp::ScaledProblem& s) { return Problem(Hip::load(s)); }
+    Vector32 zeros32(int64_t n) { return Vector32(Hip::zeros32(n)); }
     Vector zeros(int64_t n) { return Vector(Hip::zeros(n)); }
     Vector upload(const double* d, int64_t n) { return Vector(Hip::upload(d, n)); }
 };
@@ -293,6 +298,7 @@ void sp_default_params(sp_params* p) {
     p->termination = 0;
     p->fused = 1;
     p->log_every_checks = 0;
+    p->precision = 0;
 }
 
 int sp_solve(sp_backend* b, const sp_lp_view* lp, const sp_params* params, sp_solution* out, double* x_out,
@@ -376,6 +382,13 @@ int sp_backend_profile_read(sp_backend* b, int index, char* name, int name_len, 
         }
         if (total_ms) *total_ms = b->kernel_ms[(size_t)index];
         if (launches) *launches = b->kernel_launches[(size_t)index];
+    });
+}
+
+int sp_probe(sp_backend* b, int kind, int64_t n, int64_t table_len, int reps, double* ms_per_launch) {
+    return guarded(b, [&] {
+        if (!ms_per_launch || kind < 0 || kind > 2) throw sp::Error(SP_ERR_INVALID, "probe: bad argument");
+        *ms_per_launch = sp::Hip{b}.probe(kind, n, table_len, reps, 1);
     });
 }
 

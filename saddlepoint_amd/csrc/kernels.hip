@@ -58,38 +58,6 @@ __device__ __forceinline__ double recession_violation(double v, double lo, doubl
     return excess(v, l, h);
 }
 
-// ---- cross-workgroup publication (agent scope: the 8 XCDs have separate L2s)
-
-__device__ __forceinline__ void store_agent(double* p, double v) {
-    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v),
-                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ double load_agent(const double* p) {
-    return __longlong_as_double(
-        (long long)__hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT));
-}
-__device__ __forceinline__ void store_agent(float* p, float v) {
-    __hip_atomic_store(reinterpret_cast<unsigned*>(p), __float_as_uint(v), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ float load_agent(const float* p) {
-    return __uint_as_float(
-        __hip_atomic_load(reinterpret_cast<const unsigned*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-}
-// Returns true in exactly one workgroup: the last of `count` to arrive. The
-// ticket is reset by that workgroup so the next launch starts from zero.
-__device__ __forceinline__ bool arrive_last(unsigned* ticket, unsigned count) {
-    __threadfence();
-    const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    const bool last = (t == count - 1);
-    if (last) {
-        __threadfence();
-        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    return last;
-}
-
 // ---- fixed-order block reductions -------------------------------------------
 
 template <class A>
@@ -125,86 +93,178 @@ __device__ __forceinline__ void block_sum(A (&s)[K]) {
 
 // ---- SpMV with a fused per-row epilogue --------------------------------------
 //
-// out_row = Σ_k val[k] * x[col[k]] over the row, then epi(row, sum).
-// Stream blocks stage the products in LDS with fully coalesced loads and then
-// reduce each row with T lanes (T a power of two chosen from the block's row
-// count, so the order is fixed for a given matrix). T = 1 sums a row left to
-// right, as the CPU backend does. Long rows are summed by a whole workgroup
-// per chunk; chunks of one row are combined in chunk order by the last
-// workgroup to finish.
+// out_row = Σ_k val[k] * x[col[k]] over the row, then the epilogue on the sum.
+//
+// Stream blocks (rows [r0, r1) holding at most NNZ nonzeros and at most
+// SP_ROWS rows) stage the products in LDS with fully coalesced loads, then
+// reduce rows with a merge-path split (Merrill & Garland, "Merge-based
+// parallel sparse matrix-vector multiplication", SC'16): the merged sequence
+// of row ends and nonzeros is cut into WG equal pieces, so every thread does
+// the same work whatever the row lengths. A row that crosses pieces gets the
+// partial sums of the preceding pieces added in order by the thread that
+// finishes it, so the summation order is fixed for a given matrix and plan.
+// Every load that does not depend on another one is issued up front: column
+// indices, values, row pointers and the epilogue's per-row operands, then the
+// gathers.
+//
+// Long rows (more than SP_NNZ nonzeros) are summed by a whole workgroup per
+// chunk; chunks of one row are combined in chunk order by the last workgroup
+// to finish.
+#define SP_RPT 4                // rows per thread in the epilogue
+#define SP_ROWS (SP_RPT * WG)   // most rows in one stream block
+
 template <int NNZ, class V, class X, class A, class Epi>
 __device__ __forceinline__ void spmv_body(const Desc* __restrict__ desc, const int* __restrict__ rp,
                                           const int* __restrict__ ci, const V* __restrict__ va,
-                                          const X* __restrict__ x, A* __restrict__ partials,
-                                          unsigned* __restrict__ tickets, const Epi& epi) {
-    __shared__ A lds[NNZ];
+                                          const X* __restrict__ x, A* __restrict__ partials, const Epi& epi) {
+    __shared__ A prod[NNZ];
+    __shared__ A sums[SP_ROWS];
+    __shared__ int rend[SP_ROWS];
+    __shared__ int crow[WG];
+    __shared__ A cval[WG];
     const Desc d = desc[blockIdx.x];
     const int tid = threadIdx.x;
     if (d.r1 >= 0) {
         const int r0 = d.r0, nr = d.r1 - d.r0, k0 = d.k0, n = d.k1 - d.k0;
         constexpr int PER = NNZ / WG;
+        // Straight-line loads: out-of-range slots read a clamped, in-bounds
+        // address and are masked at use, so no load sits behind a branch and
+        // the compiler keeps them all in flight.
+        const int last = n > 0 ? n - 1 : 0;
+        const int kb = n > 0 ? k0 : 0;
         int c[PER];
         V v[PER];
+        if (n > 0) {
 #pragma unroll
-        for (int u = 0; u < PER; ++u) {
-            const int p = tid + u * WG;
-            if (p < n) {
-                c[u] = __builtin_nontemporal_load(ci + k0 + p);
-                v[u] = __builtin_nontemporal_load(va + k0 + p);
+            for (int u = 0; u < PER; ++u) c[u] = __builtin_nontemporal_load(ci + kb + min(tid + u * WG, last));
+#pragma unroll
+            for (int u = 0; u < PER; ++u) v[u] = __builtin_nontemporal_load(va + kb + min(tid + u * WG, last));
+        }
+        int re[SP_RPT];
+        typename Epi::Pre pre[SP_RPT];
+#pragma unroll
+        for (int j = 0; j < SP_RPT; ++j) {
+            if (j * WG < nr) {  // uniform across the workgroup
+                const int rr = r0 + min(tid + j * WG, nr - 1);
+                re[j] = rp[rr + 1];
+                pre[j] = epi.load(rr);
             }
         }
+        if (n > 0) {
+            X xv[PER];
 #pragma unroll
-        for (int u = 0; u < PER; ++u) {
-            const int p = tid + u * WG;
-            if (p < n) lds[p] = (A)v[u] * (A)x[c[u]];
+            for (int u = 0; u < PER; ++u) xv[u] = x[c[u]];
+#pragma unroll
+            for (int u = 0; u < PER; ++u)
+                if (tid + u * WG < n) prod[tid + u * WG] = (A)v[u] * (A)xv[u];
+        }
+#pragma unroll
+        for (int j = 0; j < SP_RPT; ++j)
+            if (j * WG < nr && tid + j * WG < nr) rend[tid + j * WG] = re[j] - k0;
+        __syncthreads();
+
+        // Merge path: this thread's piece of the nr + n merged items.
+        const int L = nr + n;
+        const int d0 = (int)(((long long)tid * L) / WG), d1 = (int)(((long long)(tid + 1) * L) / WG);
+        int lo = max(0, d0 - n), hi = min(d0, nr);
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (rend[mid] <= d0 - mid - 1) lo = mid + 1;
+            else hi = mid;
+        }
+        int i = lo, j = d0 - lo;
+        int first = -1;  // the first row this thread finishes (it may have begun earlier)
+        A acc = 0;
+        int e = i < nr ? rend[i] : 0;
+        for (int step = d0; step < d1; ++step) {
+            if (i < nr && j < e) {
+                acc += prod[j];
+                ++j;
+            } else if (i < nr) {
+                if (first < 0) first = i;
+                sums[i] = acc;
+                acc = 0;
+                ++i;
+                e = i < nr ? rend[i] : 0;
+            }
+        }
+        crow[tid] = i < nr ? i : -1;  // the row left open, with its partial sum
+        cval[tid] = acc;
+        __syncthreads();
+        if (first >= 0 && tid > 0 && crow[tid - 1] == first) {
+            int q = tid - 1;
+            while (q > 0 && crow[q - 1] == first) --q;
+            A t = 0;
+            for (; q < tid; ++q) t += cval[q];
+            sums[first] = t + sums[first];
         }
         __syncthreads();
-        int T = WAVE;
-        while (T > 1 && T * nr > WG) T >>= 1;
-        const int g = tid / T, l = tid % T, ng = WG / T;
-        for (int base = 0; base < nr; base += ng) {
-            const int rr = base + g;
-            const bool act = rr < nr;
-            A s = 0;
-            if (act) {
-                const int a = rp[r0 + rr] - k0, b = rp[r0 + rr + 1] - k0;
-                for (int q = a + l; q < b; q += T) s += lds[q];
-            }
-            for (int o = T / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, WAVE);
-            if (act && l == 0) epi(r0 + rr, s);
+#pragma unroll
+        for (int jj = 0; jj < SP_RPT; ++jj) {
+            if (jj * WG >= nr) break;  // uniform
+            const int rr = tid + jj * WG;
+            if (rr < nr) epi.apply(r0 + rr, sums[rr], pre[jj]);
         }
     } else {
         const int r = d.r0, pslot = -1 - d.r1, k0 = d.k0, n = d.k1 - d.k0;
+        // One chunk (n <= SP_CHUNK, n > 0): the same straight-line pattern as
+        // the stream blocks, with the products summed in registers.
+        constexpr int PERL = SP_CHUNK / WG;
+        const int lastl = n - 1;
+        int c[PERL];
+        V v[PERL];
+#pragma unroll
+        for (int u = 0; u < PERL; ++u) c[u] = __builtin_nontemporal_load(ci + k0 + min(tid + u * WG, lastl));
+#pragma unroll
+        for (int u = 0; u < PERL; ++u) v[u] = __builtin_nontemporal_load(va + k0 + min(tid + u * WG, lastl));
+        X xv[PERL];
+#pragma unroll
+        for (int u = 0; u < PERL; ++u) xv[u] = x[c[u]];
         A s[1] = {0};
-#pragma unroll 4
-        for (int p = tid; p < n; p += WG) {
-            const int cc = __builtin_nontemporal_load(ci + k0 + p);
-            const V vv = __builtin_nontemporal_load(va + k0 + p);
-            s[0] += (A)vv * (A)x[cc];
-        }
+#pragma unroll
+        for (int u = 0; u < PERL; ++u)
+            if (tid + u * WG < n) s[0] += (A)v[u] * (A)xv[u];
         block_sum<1>(s);
         const int rs = rp[r], len = rp[r + 1] - rs;
         const int nch = (len + SP_CHUNK - 1) / SP_CHUNK;
         if (nch == 1) {
-            if (tid == 0) epi(r, s[0]);
+            if (tid == 0) epi.apply(r, s[0], epi.load(r));
             return;
         }
-        if (tid == 0) {
-            const int pbase = pslot - (k0 - rs) / SP_CHUNK;
-            store_agent(partials + pslot, s[0]);
-            if (arrive_last(tickets + pbase, (unsigned)nch)) {
-                A tot = 0;
-                for (int q = 0; q < nch; ++q) tot += load_agent(partials + pbase + q);
-                epi(r, tot);
-            }
-        }
+        // A row of several chunks: publish this chunk's sum; a finishing
+        // kernel adds the chunks in order and applies the epilogue.
+        if (tid == 0) partials[pslot] = s[0];
     }
 }
 
+// The finishing pass for rows of several chunks: one wave per row adds the
+// row's chunk sums in a fixed order and applies the epilogue. (Combining in
+// the SpMV itself would need an agent-scope release fence per workgroup, and
+// on this part, with one L2 per XCD, that is an L2 write-back each time.)
+struct LongRow {
+    int row, pbase, nch, pad;
+};
+
+template <class A, class Epi>
+__device__ __forceinline__ void finish_body(const LongRow* __restrict__ longs, int nlong,
+                                            const A* __restrict__ partials, const Epi& epi) {
+    const int w = (int)((blockIdx.x * WG + threadIdx.x) / WAVE), lane = threadIdx.x % WAVE;
+    if (w >= nlong) return;  // uniform per wave
+    const LongRow L = longs[w];
+    A s = 0;
+    for (int q = lane; q < L.nch; q += WAVE) s += partials[L.pbase + q];
+    s = wave_sum(s);
+    if (lane == 0) epi.apply(L.row, s, epi.load(L.row));
+}
+
+// Epilogues: load(row) fetches the row's operands (issued early), apply(row,
+// sum, operands) computes and stores.
 template <class A>
 struct EpiStore {
     A* out;
-    __device__ void operator()(int r, A s) const { out[r] = s; }
+    struct Pre {};
+    __device__ Pre load(int) const { return {}; }
+    __device__ void apply(int r, A s, const Pre&) const { out[r] = s; }
 };
 
 // spmv_t + primal_step: aty = A'ᵀy; x_t = clamp(x - τ(c - aty), lv, uv);
@@ -213,12 +273,15 @@ struct EpiPrimal {
     const double *x, *c, *lv, *uv;
     double tau;
     double *aty, *x_t, *x_bar;
-    __device__ void operator()(int j, double s) const {
-        const double xj = x[j];
-        const double t = clampv(xj - tau * (c[j] - s), lv[j], uv[j]);
+    struct Pre {
+        double x, c, lv, uv;
+    };
+    __device__ Pre load(int j) const { return {x[j], c[j], lv[j], uv[j]}; }
+    __device__ void apply(int j, double s, const Pre& q) const {
+        const double t = clampv(q.x - tau * (q.c - s), q.lv, q.uv);
         aty[j] = s;
         x_t[j] = t;
-        x_bar[j] = 2.0 * t - xj;
+        x_bar[j] = 2.0 * t - q.x;
     }
 };
 
@@ -229,11 +292,14 @@ struct EpiPrimalHalpern {
     const double *c, *lv, *uv, *anchor;
     double tau, w, rho;
     double* x_bar;
-    __device__ void operator()(int j, double s) const {
-        const double xj = x[j];
-        const double t = clampv(xj - tau * (c[j] - s), lv[j], uv[j]);
-        x_bar[j] = 2.0 * t - xj;
-        x[j] = w * ((1.0 + rho) * t - rho * xj) + (1.0 - w) * anchor[j];
+    struct Pre {
+        double x, c, lv, uv, anchor;
+    };
+    __device__ Pre load(int j) const { return {x[j], c[j], lv[j], uv[j], anchor[j]}; }
+    __device__ void apply(int j, double s, const Pre& q) const {
+        const double t = clampv(q.x - tau * (q.c - s), q.lv, q.uv);
+        x_bar[j] = 2.0 * t - q.x;
+        x[j] = w * ((1.0 + rho) * t - rho * q.x) + (1.0 - w) * q.anchor;
     }
 };
 
@@ -243,11 +309,14 @@ struct EpiDual {
     const double *y, *lc, *uc;
     double sigma;
     double *ax, *y_t;
-    __device__ void operator()(int i, double s) const {
-        const double yi = y[i];
-        const double v = s - yi / sigma;
+    struct Pre {
+        double y, lc, uc;
+    };
+    __device__ Pre load(int i) const { return {y[i], lc[i], uc[i]}; }
+    __device__ void apply(int i, double s, const Pre& q) const {
+        const double v = s - q.y / sigma;
         ax[i] = s;
-        y_t[i] = yi - sigma * (s - clampv(v, lc[i], uc[i]));
+        y_t[i] = q.y - sigma * (s - clampv(v, q.lc, q.uc));
     }
 };
 
@@ -255,21 +324,23 @@ struct EpiDualHalpern {
     double* y;
     const double *lc, *uc, *anchor;
     double sigma, w, rho;
-    __device__ void operator()(int i, double s) const {
-        const double yi = y[i];
-        const double v = s - yi / sigma;
-        const double t = yi - sigma * (s - clampv(v, lc[i], uc[i]));
-        y[i] = w * ((1.0 + rho) * t - rho * yi) + (1.0 - w) * anchor[i];
+    struct Pre {
+        double y, lc, uc, anchor;
+    };
+    __device__ Pre load(int i) const { return {y[i], lc[i], uc[i], anchor[i]}; }
+    __device__ void apply(int i, double s, const Pre& q) const {
+        const double v = s - q.y / sigma;
+        const double t = q.y - sigma * (s - clampv(v, q.lc, q.uc));
+        y[i] = w * ((1.0 + rho) * t - rho * q.y) + (1.0 - w) * q.anchor;
     }
 };
 
 #define SPMV_ARGS                                                                                    \
     const Desc *__restrict__ desc, const int *__restrict__ rp, const int *__restrict__ ci,          \
-        const double *__restrict__ va, const double *__restrict__ xin, double *__restrict__ partials, \
-        unsigned *__restrict__ tickets
+        const double *__restrict__ va, const double *__restrict__ xin, double *__restrict__ partials
 
 extern "C" __global__ __launch_bounds__(WG) void sp_spmv_f64(SPMV_ARGS, double* __restrict__ out) {
-    spmv_body<SP_NNZ_F64, double, double, double>(desc, rp, ci, va, xin, partials, tickets, EpiStore<double>{out});
+    spmv_body<SP_NNZ_F64, double, double, double>(desc, rp, ci, va, xin, partials, EpiStore<double>{out});
 }
 
 extern "C" __global__ __launch_bounds__(WG) void sp_spmv_t_primal_f64(SPMV_ARGS, const double* __restrict__ x,
@@ -279,7 +350,7 @@ extern "C" __global__ __launch_bounds__(WG) void sp_spmv_t_primal_f64(SPMV_ARGS,
                                                                       double* __restrict__ aty,
                                                                       double* __restrict__ x_t,
                                                                       double* __restrict__ x_bar) {
-    spmv_body<SP_NNZ_F64, double, double, double>(desc, rp, ci, va, xin, partials, tickets,
+    spmv_body<SP_NNZ_F64, double, double, double>(desc, rp, ci, va, xin, partials,
                                                   EpiPrimal{x, c, lv, uv, tau, aty, x_t, x_bar});
 }
 
@@ -287,7 +358,7 @@ extern "C" __global__ __launch_bounds__(WG) void sp_spmv_t_primal_halpern_f64(
     SPMV_ARGS, double* __restrict__ x, const double* __restrict__ c, const double* __restrict__ lv,
     const double* __restrict__ uv, const double* __restrict__ anchor, double tau, double w, double rho,
     double* __restrict__ x_bar) {
-    spmv_body<SP_NNZ_F64, double, double, double>(desc, rp, ci, va, xin, partials, tickets,
+    spmv_body<SP_NNZ_F64, double, double, double>(desc, rp, ci, va, xin, partials,
                                                   EpiPrimalHalpern{x, c, lv, uv, anchor, tau, w, rho, x_bar});
 }
 
@@ -296,15 +367,143 @@ extern "C" __global__ __launch_bounds__(WG) void sp_spmv_dual_f64(SPMV_ARGS, con
                                                                   const double* __restrict__ uc, double sigma,
                                                                   double* __restrict__ ax,
                                                                   double* __restrict__ y_t) {
-    spmv_body<SP_NNZ_F64, double, double, double>(desc, rp, ci, va, xin, partials, tickets,
+    spmv_body<SP_NNZ_F64, double, double, double>(desc, rp, ci, va, xin, partials,
                                                   EpiDual{y, lc, uc, sigma, ax, y_t});
 }
 
 extern "C" __global__ __launch_bounds__(WG) void sp_spmv_dual_halpern_f64(
     SPMV_ARGS, double* __restrict__ y, const double* __restrict__ lc, const double* __restrict__ uc,
     const double* __restrict__ anchor, double sigma, double w, double rho) {
-    spmv_body<SP_NNZ_F64, double, double, double>(desc, rp, ci, va, xin, partials, tickets,
+    spmv_body<SP_NNZ_F64, double, double, double>(desc, rp, ci, va, xin, partials,
                                                   EpiDualHalpern{y, lc, uc, anchor, sigma, w, rho});
+}
+
+
+#define FIN_ARGS(A) const LongRow *__restrict__ longs, int nlong, const A *__restrict__ partials
+
+extern "C" __global__ __launch_bounds__(WG) void sp_finish_f64(FIN_ARGS(double), double* __restrict__ out) {
+    finish_body<double>(longs, nlong, partials, EpiStore<double>{out});
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_finish_t_primal_f64(FIN_ARGS(double), const double* __restrict__ x,
+                                                                        const double* __restrict__ c,
+                                                                        const double* __restrict__ lv,
+                                                                        const double* __restrict__ uv, double tau,
+                                                                        double* __restrict__ aty,
+                                                                        double* __restrict__ x_t,
+                                                                        double* __restrict__ x_bar) {
+    finish_body<double>(longs, nlong, partials, EpiPrimal{x, c, lv, uv, tau, aty, x_t, x_bar});
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_finish_t_primal_halpern_f64(
+    FIN_ARGS(double), double* __restrict__ x, const double* __restrict__ c, const double* __restrict__ lv,
+    const double* __restrict__ uv, const double* __restrict__ anchor, double tau, double w, double rho,
+    double* __restrict__ x_bar) {
+    finish_body<double>(longs, nlong, partials, EpiPrimalHalpern{x, c, lv, uv, anchor, tau, w, rho, x_bar});
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_finish_dual_f64(FIN_ARGS(double), const double* __restrict__ y,
+                                                                    const double* __restrict__ lc,
+                                                                    const double* __restrict__ uc, double sigma,
+                                                                    double* __restrict__ ax, double* __restrict__ y_t) {
+    finish_body<double>(longs, nlong, partials, EpiDual{y, lc, uc, sigma, ax, y_t});
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_finish_dual_halpern_f64(
+    FIN_ARGS(double), double* __restrict__ y, const double* __restrict__ lc, const double* __restrict__ uc,
+    const double* __restrict__ anchor, double sigma, double w, double rho) {
+    finish_body<double>(longs, nlong, partials, EpiDualHalpern{y, lc, uc, anchor, sigma, w, rho});
+}
+
+// ---- mixed precision: FP32 matrix and displacements, FP64 anchors ------------
+//
+// Between checks the iterates are held as z = z_anchor + dz with the Halpern
+// anchor z_anchor in FP64 and the displacement dz in FP32. With
+// A'ᵀy = A'ᵀy_anchor + A'ᵀdy and A'x_bar = A'x_anchor + A'dx_bar, the FP64
+// products of the anchors are formed once per restart epoch and the per-
+// iteration SpMVs run on FP32 values with FP32 sums. Steps, projections and
+// the Halpern update are evaluated in FP64 on z_anchor + dz, so the rounding
+// is relative to the displacement, not to the iterate. In displacement form
+// the Halpern update z ← w((1+ρ)z_t - ρz) + (1-w)z_anchor reads
+// dz ← w((1+ρ)(z_t - z_anchor) - ρ dz).
+
+struct EpiPrimalMixed {
+    float* dx;
+    const double *xa, *c, *lv, *uv, *atya;
+    double tau, w, rho;
+    float* dx_bar;
+    struct Pre {
+        float dx;
+        double xa, c, lv, uv, atya;
+    };
+    __device__ Pre load(int j) const { return {dx[j], xa[j], c[j], lv[j], uv[j], atya[j]}; }
+    __device__ void apply(int j, float s, const Pre& q) const {
+        const double d = (double)q.dx;
+        const double aty = q.atya + (double)s;
+        const double t = clampv(q.xa + d - tau * (q.c - aty), q.lv, q.uv) - q.xa;
+        dx_bar[j] = (float)(2.0 * t - d);
+        dx[j] = (float)(w * ((1.0 + rho) * t - rho * d));
+    }
+};
+
+struct EpiDualMixed {
+    float* dy;
+    const double *ya, *lc, *uc, *axa;
+    double sigma, w, rho;
+    struct Pre {
+        float dy;
+        double ya, lc, uc, axa;
+    };
+    __device__ Pre load(int i) const { return {dy[i], ya[i], lc[i], uc[i], axa[i]}; }
+    __device__ void apply(int i, float s, const Pre& q) const {
+        const double d = (double)q.dy;
+        const double y = q.ya + d;
+        const double ax = q.axa + (double)s;
+        const double v = ax - y / sigma;
+        const double t = y - sigma * (ax - clampv(v, q.lc, q.uc)) - q.ya;
+        dy[i] = (float)(w * ((1.0 + rho) * t - rho * d));
+    }
+};
+
+#define SPMV32_ARGS                                                                                  \
+    const Desc *__restrict__ desc, const int *__restrict__ rp, const int *__restrict__ ci,          \
+        const float *__restrict__ va, const float *__restrict__ xin, float *__restrict__ partials
+
+extern "C" __global__ __launch_bounds__(WG) void sp_spmv_f32(SPMV32_ARGS, float* __restrict__ out) {
+    spmv_body<SP_NNZ_F64, float, float, float>(desc, rp, ci, va, xin, partials, EpiStore<float>{out});
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_spmv_t_primal_mixed(
+    SPMV32_ARGS, float* __restrict__ dx, const double* __restrict__ xa, const double* __restrict__ c,
+    const double* __restrict__ lv, const double* __restrict__ uv, const double* __restrict__ atya, double tau,
+    double w, double rho, float* __restrict__ dx_bar) {
+    spmv_body<SP_NNZ_F64, float, float, float>(desc, rp, ci, va, xin, partials,
+                                               EpiPrimalMixed{dx, xa, c, lv, uv, atya, tau, w, rho, dx_bar});
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_spmv_dual_mixed(
+    SPMV32_ARGS, float* __restrict__ dy, const double* __restrict__ ya, const double* __restrict__ lc,
+    const double* __restrict__ uc, const double* __restrict__ axa, double sigma, double w, double rho) {
+    spmv_body<SP_NNZ_F64, float, float, float>(desc, rp, ci, va, xin, partials,
+                                               EpiDualMixed{dy, ya, lc, uc, axa, sigma, w, rho});
+}
+
+
+extern "C" __global__ __launch_bounds__(WG) void sp_finish_f32(FIN_ARGS(float), float* __restrict__ out) {
+    finish_body<float>(longs, nlong, partials, EpiStore<float>{out});
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_finish_t_primal_mixed(
+    FIN_ARGS(float), float* __restrict__ dx, const double* __restrict__ xa, const double* __restrict__ c,
+    const double* __restrict__ lv, const double* __restrict__ uv, const double* __restrict__ atya, double tau,
+    double w, double rho, float* __restrict__ dx_bar) {
+    finish_body<float>(longs, nlong, partials, EpiPrimalMixed{dx, xa, c, lv, uv, atya, tau, w, rho, dx_bar});
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_finish_dual_mixed(
+    FIN_ARGS(float), float* __restrict__ dy, const double* __restrict__ ya, const double* __restrict__ lc,
+    const double* __restrict__ uc, const double* __restrict__ axa, double sigma, double w, double rho) {
+    finish_body<float>(longs, nlong, partials, EpiDualMixed{dy, ya, lc, uc, axa, sigma, w, rho});
 }
 
 // ---- element-wise (cpu/src/lib.rs:147-209) ------------------------------------
@@ -359,6 +558,25 @@ extern "C" __global__ __launch_bounds__(WG) void sp_fill_f64(double* __restrict_
     GRID_LOOP(i, n) { v[i] = value; }
 }
 
+// Precision conversions (round to nearest; ±inf and NaN carry over).
+extern "C" __global__ __launch_bounds__(WG) void sp_f64_to_f32(const double* __restrict__ a, float* __restrict__ b,
+                                                               i64 n) {
+    GRID_LOOP(i, n) { b[i] = (float)a[i]; }
+}
+
+// dz = (float)(z - anchor) and back: z = anchor + dz.
+extern "C" __global__ __launch_bounds__(WG) void sp_delta_from(const double* __restrict__ z,
+                                                               const double* __restrict__ anchor,
+                                                               float* __restrict__ dz, i64 n) {
+    GRID_LOOP(i, n) { dz[i] = (float)(z[i] - anchor[i]); }
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_delta_to(const float* __restrict__ dz,
+                                                             const double* __restrict__ anchor,
+                                                             double* __restrict__ z, i64 n) {
+    GRID_LOOP(i, n) { z[i] = anchor[i] + (double)dz[i]; }
+}
+
 // ---- fused reductions (cpu/src/lib.rs:189-259) ---------------------------------
 //
 // Each kernel runs over a combined index range (rows, then columns when a
@@ -369,38 +587,47 @@ extern "C" __global__ __launch_bounds__(WG) void sp_fill_f64(double* __restrict_
 
 template <int K, class F>
 __device__ __forceinline__ void reduce_body(i64 n, const F& f, double* __restrict__ partials,
-                                            unsigned* __restrict__ ticket, double* __restrict__ result) {
+                                            double* __restrict__ result) {
     double s[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) s[k] = 0.0;
-    GRID_LOOP(i, n) { f(i, s); }
-    block_sum<K>(s);
-    if (gridDim.x == 1) {
-        if (threadIdx.x == 0)
+    // Tiles of 4·WG elements; a thread takes elements tid, tid+WG, ... of a tile,
+    // so its four loads are independent and issue together.
+    for (i64 base = (i64)blockIdx.x * 4 * WG; base < n; base += (i64)gridDim.x * 4 * WG) {
+        if (base + 4 * WG <= n) {
 #pragma unroll
-            for (int k = 0; k < K; ++k) result[k] = s[k];
-        return;
+            for (int u = 0; u < 4; ++u) f(base + threadIdx.x + u * WG, s);
+        } else {
+            for (int u = 0; u < 4; ++u) {
+                const i64 i = base + threadIdx.x + u * WG;
+                if (i < n) f(i, s);
+            }
+        }
     }
-    __shared__ bool last;
+    block_sum<K>(s);
     if (threadIdx.x == 0) {
+        double* out = gridDim.x == 1 ? result : partials + (i64)blockIdx.x * K;
 #pragma unroll
-        for (int k = 0; k < K; ++k) store_agent(partials + (i64)blockIdx.x * K + k, s[k]);
-        last = arrive_last(ticket, gridDim.x);
+        for (int k = 0; k < K; ++k) out[k] = s[k];
     }
-    __syncthreads();
-    if (!last) return;
+}
+
+// Adds the per-workgroup partials of a reduction (G workgroups, K sums each)
+// in workgroup order: thread t takes workgroups t, t + WG, ..., then the block
+// sums in its fixed order. One workgroup.
+extern "C" __global__ __launch_bounds__(WG) void sp_red_finish(const double* __restrict__ partials, int G, int K,
+                                                               double* __restrict__ result) {
+    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < G; b += WG)
 #pragma unroll
-    for (int k = 0; k < K; ++k) s[k] = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += WG)
-#pragma unroll
-        for (int k = 0; k < K; ++k) s[k] += load_agent(partials + (i64)b * K + k);
-    block_sum<K>(s);
+        for (int k = 0; k < 5; ++k)
+            if (k < K) s[k] += partials[(i64)b * K + k];
+    block_sum<5>(s);
     if (threadIdx.x == 0)
-#pragma unroll
         for (int k = 0; k < K; ++k) result[k] = s[k];
 }
 
-#define RED_TAIL double *__restrict__ partials, unsigned *__restrict__ ticket, double *__restrict__ result
+#define RED_TAIL double *__restrict__ partials, double *__restrict__ result
 
 struct FDot {
     const double *a, *b;
@@ -408,7 +635,7 @@ struct FDot {
 };
 extern "C" __global__ __launch_bounds__(WG) void sp_red_dot_f64(const double* __restrict__ a,
                                                                 const double* __restrict__ b, i64 n, RED_TAIL) {
-    reduce_body<1>(n, FDot{a, b}, partials, ticket, result);
+    reduce_body<1>(n, FDot{a, b}, partials, result);
 }
 
 struct FDist {
@@ -421,7 +648,7 @@ struct FDist {
 extern "C" __global__ __launch_bounds__(WG) void sp_red_distance_sq_f64(const double* __restrict__ a,
                                                                         const double* __restrict__ b, i64 n,
                                                                         RED_TAIL) {
-    reduce_body<1>(n, FDist{a, b}, partials, ticket, result);
+    reduce_body<1>(n, FDist{a, b}, partials, result);
 }
 
 // primal_stats: [Σ_i (excess(ax, lc, uc) pf)², Σ_j c x]
@@ -442,7 +669,7 @@ extern "C" __global__ __launch_bounds__(WG) void sp_red_primal_stats_f64(
     const double* __restrict__ ax, const double* __restrict__ lc, const double* __restrict__ uc,
     const double* __restrict__ pf, i64 m, const double* __restrict__ c, const double* __restrict__ x, i64 n,
     RED_TAIL) {
-    reduce_body<2>(m + n, FPrimal{ax, lc, uc, pf, c, x, m}, partials, ticket, result);
+    reduce_body<2>(m + n, FPrimal{ax, lc, uc, pf, c, x, m}, partials, result);
 }
 
 // dual_stats: [Σ_i bound_term(y, lc, uc), Σ_j (forbidden(r) df)², Σ_j bound_term(r, lv, uv)], r = c - aty
@@ -465,7 +692,7 @@ extern "C" __global__ __launch_bounds__(WG) void sp_red_dual_stats_f64(
     const double* __restrict__ y, const double* __restrict__ lc, const double* __restrict__ uc, i64 m,
     const double* __restrict__ c, const double* __restrict__ aty, const double* __restrict__ lv,
     const double* __restrict__ uv, const double* __restrict__ df, i64 n, RED_TAIL) {
-    reduce_body<3>(m + n, FDual{y, lc, uc, c, aty, lv, uv, df, m}, partials, ticket, result);
+    reduce_body<3>(m + n, FDual{y, lc, uc, c, aty, lv, uv, df, m}, partials, result);
 }
 
 // step_dual_residual_sq: Σ ((aty - aty_t - (x_t - x) inv_tau) df)²
@@ -480,7 +707,7 @@ struct FStep {
 extern "C" __global__ __launch_bounds__(WG) void sp_red_step_dual_f64(
     const double* __restrict__ x, const double* __restrict__ x_t, const double* __restrict__ aty,
     const double* __restrict__ aty_t, const double* __restrict__ df, double inv_tau, i64 n, RED_TAIL) {
-    reduce_body<1>(n, FStep{x, x_t, aty, aty_t, df, inv_tau}, partials, ticket, result);
+    reduce_body<1>(n, FStep{x, x_t, aty, aty_t, df, inv_tau}, partials, result);
 }
 
 // primal_ray_stats: [Σ_i rv(adx, lc, uc)², Σ_j rv(dx, lv, uv)², Σ_j dx², Σ_j c dx]
@@ -505,7 +732,7 @@ extern "C" __global__ __launch_bounds__(WG) void sp_red_primal_ray_f64(
     const double* __restrict__ adx, const double* __restrict__ lc, const double* __restrict__ uc, i64 m,
     const double* __restrict__ dx, const double* __restrict__ c, const double* __restrict__ lv,
     const double* __restrict__ uv, i64 n, RED_TAIL) {
-    reduce_body<4>(m + n, FPrimalRay{adx, lc, uc, dx, c, lv, uv, m}, partials, ticket, result);
+    reduce_body<4>(m + n, FPrimalRay{adx, lc, uc, dx, c, lv, uv, m}, partials, result);
 }
 
 // dual_ray_stats: [Σ_i forbidden(dy)², Σ_i bound_term(dy), Σ_i dy², Σ_j forbidden(r)², Σ_j bound_term(r)], r = -atdy
@@ -532,5 +759,28 @@ extern "C" __global__ __launch_bounds__(WG) void sp_red_dual_ray_f64(
     const double* __restrict__ dy, const double* __restrict__ lc, const double* __restrict__ uc, i64 m,
     const double* __restrict__ atdy, const double* __restrict__ lv, const double* __restrict__ uv, i64 n,
     RED_TAIL) {
-    reduce_body<5>(m + n, FDualRay{dy, lc, uc, atdy, lv, uv, m}, partials, ticket, result);
+    reduce_body<5>(m + n, FDualRay{dy, lc, uc, atdy, lv, uv, m}, partials, result);
+}
+
+// ---- bandwidth probes (the counterpart of the reference's sp-bandwidth) ------------
+//
+// Achievable HBM copy and triad rates, and the rate of random 8-byte gathers
+// from a table of a given size, to put the SpMV's figures in context.
+
+extern "C" __global__ __launch_bounds__(WG) void sp_probe_copy(const double* __restrict__ a, double* __restrict__ b,
+                                                               i64 n) {
+    GRID_LOOP(i, n) { b[i] = __builtin_nontemporal_load(a + i); }
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_probe_triad(const double* __restrict__ a,
+                                                                const double* __restrict__ b, double* __restrict__ c,
+                                                                double s, i64 n) {
+    GRID_LOOP(i, n) { c[i] = __builtin_nontemporal_load(a + i) + s * __builtin_nontemporal_load(b + i); }
+}
+
+// out[i] = table[idx[i]]: streams idx and out, gathers from table.
+extern "C" __global__ __launch_bounds__(WG) void sp_probe_gather(const int* __restrict__ idx,
+                                                                 const double* __restrict__ table,
+                                                                 double* __restrict__ out, i64 n) {
+    GRID_LOOP(i, n) { out[i] = table[__builtin_nontemporal_load(idx + i)]; }
 }

@@ -10,6 +10,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <algorithm>
 #include <cstdio>
 #include <limits>
 #include <memory>
@@ -83,6 +84,12 @@ class Solver {
         x_anchor_ = b_.zeros(n), y_anchor_ = b_.zeros(m), ax_ = b_.zeros(m), aty_ = b_.zeros(n);
         tmp_n_ = b_.zeros(n), tmp_m_ = b_.zeros(m), tmp_m2_ = b_.zeros(m);
         c_ = b_.upload(scaled_.c.data(), n);
+        if (params.precision == 1) {
+            if (!params.fused) throw Error(SP_ERR_INVALID, "mixed precision needs the fused path");
+            b_.ensure_fp32(dev_.get());
+            dx32_ = b_.zeros32(n), dy32_ = b_.zeros32(m), dx_bar32_ = b_.zeros32(n);
+            atya_ = b_.zeros(n), axa_ = b_.zeros(m);
+        }
     }
 
     // Solver::run (solver.rs:311-438). x_out/y_out receive the unscaled x_t, y_t.
@@ -114,11 +121,34 @@ class Solver {
             if (p.fused && !check && k_ != 0) {
                 // Nothing of this iteration is observed but x, y and x_bar.
                 const double w = (double)(k_ + 1) / (double)(k_ + 2);
-                b_.plain_iteration(dev_.get(), x_.get(), y_.get(), x_anchor_.get(), y_anchor_.get(), x_bar_.get(),
-                                   tau, sigma, w, p.reflection);
+                if (p.precision == 1 && !fp64_fallback_) {
+                    if (!shadow_live_) {
+                        if (anchor_epoch_ != restarts_) {
+                            // The anchors moved: refresh their FP64 products.
+                            b_.spmv_t(dev_.get(), y_anchor_.get(), atya_.get());
+                            b_.spmv(dev_.get(), x_anchor_.get(), axa_.get());
+                            anchor_epoch_ = restarts_;
+                        }
+                        b_.delta_from(x_.get(), x_anchor_.get(), dx32_.get());
+                        b_.delta_from(y_.get(), y_anchor_.get(), dy32_.get());
+                        shadow_live_ = true;
+                    }
+                    b_.plain_iteration_mixed(dev_.get(), dx32_.get(), dy32_.get(), x_anchor_.get(), y_anchor_.get(),
+                                             atya_.get(), axa_.get(), dx_bar32_.get(), tau, sigma, w, p.reflection);
+                } else {
+                    b_.plain_iteration(dev_.get(), x_.get(), y_.get(), x_anchor_.get(), y_anchor_.get(),
+                                       x_bar_.get(), tau, sigma, w, p.reflection);
+                    if (p.precision == 1) ++fp64_plain_;
+                }
                 total_ = next_total;
                 k_ += 1;
                 continue;
+            }
+            if (shadow_live_) {
+                // Checks and restarts work on the FP64 iterates.
+                b_.delta_to(dx32_.get(), x_anchor_.get(), x_.get());
+                b_.delta_to(dy32_.get(), y_anchor_.get(), y_.get());
+                shadow_live_ = false;
             }
             if (p.fused) {
                 b_.spmv_t_primal(dev_.get(), y_.get(), x_.get(), tau, aty_.get(), x_t_.get(), x_bar_.get());
@@ -161,6 +191,7 @@ class Solver {
                 stop(SP_OPTIMAL);
                 break;
             }
+            watch_progress(test);
             const int inf = infeasibility();
             if (inf >= 0) {
                 stop(inf);
@@ -226,6 +257,7 @@ class Solver {
         s.restarts = restarts_;
         s.solve_time_s = terminated_ ? solve_time_ : elapsed();
         s.final_omega = omega_;
+        s.mixed_fp64_iterations = fp64_plain_;
         return s;
     }
 
@@ -310,13 +342,42 @@ class Solver {
         if (d.objective > 0.0 && std::sqrt(d.violation_sq) <= eps * d.objective &&
             d.objective > eps * std::sqrt(d.norm_sq))
             return SP_PRIMAL_INFEASIBLE;
+        near_certificate(d.objective, d.violation_sq, d.norm_sq);
         b_.sub(x_t_.get(), x_anchor_.get(), tmp_n_.get());
         b_.spmv(dev_.get(), tmp_n_.get(), tmp_m_.get());
         const sp_ray_stats_t pr = b_.primal_ray_stats(dev_.get(), tmp_n_.get(), tmp_m_.get());
         if (pr.objective < 0.0 && std::sqrt(pr.violation_sq) <= -eps * pr.objective &&
             -pr.objective > eps * std::sqrt(pr.norm_sq))
             return SP_DUAL_INFEASIBLE;
+        near_certificate(-pr.objective, pr.violation_sq, pr.norm_sq);
         return -1;
+    }
+
+    // Mixed precision only: a ray that satisfies the certificate test with a
+    // looser violation bound (kNearCertificate instead of the infeasibility
+    // tolerance) cannot be completed with FP32 displacements, whose rounding
+    // is far above 1e-8 relative; the rest of the solve then runs in FP64.
+    static constexpr double kNearCertificate = 1e-4;
+    void near_certificate(double objective, double violation_sq, double norm_sq) {
+        if (p_.precision != 1 || fp64_fallback_) return;
+        if (objective > 0.0 && std::sqrt(violation_sq) <= kNearCertificate * objective &&
+            objective > p_.infeasibility_tolerance * std::sqrt(norm_sq))
+            fp64_fallback_ = true;
+    }
+
+    // Mixed precision only: FP64 takes over when the relative KKT error has not
+    // improved on its best for kStallChecks checks in a row.
+    static constexpr uint64_t kStallChecks = 64;
+    void watch_progress(const Kkt& k) {
+        if (p_.precision != 1 || fp64_fallback_) return;
+        const double err = std::max({k.primal_res / (1.0 + b_norm_), k.dual_res / (1.0 + c_norm_),
+                                     k.gap() / (1.0 + std::fabs(k.primal_obj) + std::fabs(k.dual_obj))});
+        if (err < 0.5 * best_err_) {
+            best_err_ = err;
+            stalled_checks_ = 0;
+        } else if (++stalled_checks_ >= kStallChecks) {
+            fp64_fallback_ = true;
+        }
     }
 
     B& b_;
@@ -338,6 +399,16 @@ class Solver {
     Kkt kkt_;
     Prob dev_;
     Vec x_, y_, x_t_, y_t_, x_bar_, x_anchor_, y_anchor_, ax_, aty_, tmp_n_, tmp_m_, tmp_m2_, c_;
+    // Mixed precision: FP32 displacements from the anchors, live between
+    // checks, and the anchors' FP64 products A'ᵀy_anchor, A'x_anchor.
+    typename B::Vector32 dx32_, dy32_, dx_bar32_;
+    Vec atya_, axa_;
+    bool shadow_live_ = false;
+    uint64_t anchor_epoch_ = UINT64_MAX;
+    bool fp64_fallback_ = false;
+    double best_err_ = INFINITY;
+    uint64_t stalled_checks_ = 0;
+    uint64_t fp64_plain_ = 0;  // plain iterations that ran in FP64 in mixed mode
 };
 
 }  // namespace sp

@@ -160,9 +160,9 @@ def test_spmv_is_deterministic(gpu):
     assert d[0] == d[1] == d[2]
 
 
-def test_short_rows_sum_left_to_right(gpu):
-    # Stream blocks with at least 256 rows reduce each row with one lane, left
-    # to right, exactly as the CPU backend does: the SpMV is then bitwise equal.
+def test_rows_inside_one_piece_sum_left_to_right(gpu):
+    # A row that falls inside one thread's merge-path piece is summed left to
+    # right, exactly as the CPU backend does, so those rows are bitwise equal.
     p = generate.config0()
     cpu, sp = scaled_pair(p)
     pg = gpu.load(sp)
@@ -171,10 +171,10 @@ def test_short_rows_sum_left_to_right(gpu):
     gpu.spmv(pg, gpu.upload(x), out)
     g = gpu.download(out)
     c = cpu.spmv(x)
-    # Not every block qualifies, so require bitwise equality on most rows and
-    # 1e-12 on all.
+    # Rows that cross pieces are summed in another fixed order: 1e-12 on all,
+    # bitwise on a good share.
     close_vec("spmv", c, g)
-    assert np.mean(g == c) > 0.5
+    assert np.mean(g == c) > 0.3
 
 
 def test_shape_errors_are_reported(gpu):

@@ -89,3 +89,40 @@ def test_iteration_limit_and_restarts_are_reported(gpu):
     c, _, _ = O.solve(p, O.default_params(max_iterations=640))
     assert g.status == "IterationLimit" == O.STATUS[c.status]
     assert g.iterations == 640 and g.restarts == c.restarts
+
+
+# ---- mixed precision: FP32 matrix/iterates between checks, FP64 checks -------------
+
+@pytest.mark.parametrize("name", ["afiro", "adlittle", "israel", "sctest", "shell", "etamacro", "standata",
+                                  "standgub", "standmps", "forest6", "woodinfe"])
+def test_mixed_precision_netlib(gpu, name):
+    """Same status as the FP64 oracle; an optimal answer passes the independent
+    KKT check at 2e-6 and its objective is within 2e-6 of the FP64 one (both are
+    within the 1e-6 tolerance of the optimum)."""
+    p = netlib(name)
+    c, _, _ = O.solve(p, O.default_params(tolerance=TOL))
+    g = solve(gpu, p, default_params(tolerance=TOL, precision=1))
+    rep = O.check(p, g.x, g.y) if g.status == "Optimal" else None
+    print(f"{name:>10}: fp64 {c.iterations} it; mixed {g.status} {g.iterations} it "
+          f"({g.mixed_fp64_iterations} in fp64 after switch-over); "
+          f"kkt {rep.max_relative() if rep else float('nan'):.2e}", flush=True)
+    assert g.status == O.STATUS[c.status]
+    if g.status == "Optimal":
+        rep.verify(2 * TOL)
+        assert abs(c.objective - g.objective) / (1 + abs(c.objective) + abs(g.objective)) <= 2 * TOL
+
+
+def test_mixed_precision_config0(gpu):
+    p = generate.config0()
+    c, _, _ = O.solve(p, O.default_params(tolerance=TOL))
+    g = solve(gpu, p, default_params(tolerance=TOL, precision=1))
+    assert g.status == "Optimal"
+    O.check(p, g.x, g.y).verify(2 * TOL)
+    assert abs(c.objective - g.objective) / (1 + abs(c.objective) + abs(g.objective)) <= 2 * TOL
+
+
+def test_mixed_precision_needs_fused(gpu):
+    from saddlepoint_amd._lib import BackendError
+
+    with pytest.raises(BackendError):
+        solve(gpu, netlib("afiro"), default_params(precision=1, fused=0))
