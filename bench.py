@@ -52,6 +52,21 @@ def kernel_bytes(m: int, n: int, nnz: int) -> dict:
     }
 
 
+def read_traffic(path: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by the factor
+    the same counter reads on a copy kernel of known size), or None."""
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) >= 2 and parts[0] == kernel + ":":
+                    return float(parts[1])
+    except OSError:
+        return None
+    return None
+
+
 def cpu_baseline(problem, seconds_budget: float) -> dict:
     from oracle import oracle as O
 
@@ -80,6 +95,10 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--probe", choices=["copy", "triad", "gather"],
+                    help="run only a bandwidth probe (copy: 2^27 doubles, the HBM counter calibration)")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_config1.txt"),
+                    help="measured HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE summary)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported (this backend has no "
@@ -88,6 +107,16 @@ def main() -> int:
 
     from saddlepoint_amd import generate
     from saddlepoint_amd.backend import HipBackend, Solver, default_params
+
+    if args.probe:
+        b = HipBackend(0)
+        n = 1 << 27
+        table = 900_000 if args.probe == "gather" else 0
+        ms = b.probe(args.probe, n, table, reps=10)
+        per = {"copy": 16, "triad": 24, "gather": 12}[args.probe]
+        print(json.dumps({"probe": args.probe, "elements": n, "table": table, "ms_per_launch": ms,
+                          "bytes_per_launch": per * n, "gbs": per * n / ms / 1e6}), flush=True)
+        return 0
 
     t0 = time.time()
     p = generate.config1(scale=args.scale)
@@ -103,18 +132,23 @@ def main() -> int:
     b.sync()
     log(f"warmup {args.warmup} iterations done")
 
-    b.profile(True)
+    # Timed region: K iterations, nothing but the solver on the stream.
     it0 = s.result(with_solution=False).iterations
     t0 = time.perf_counter()
     done = s.advance(args.steps)
     b.sync()
     t1 = time.perf_counter()
-    prof = b.profile_read()
-    b.profile(False)
     res = s.result(with_solution=False)
     steps = res.iterations - it0
     elapsed = t1 - t0
     log(f"timed {steps} iterations in {elapsed:.3f}s (terminated={done}, status={res.status})")
+    # The same number of iterations again with a HIP event pair around every
+    # launch on the solver's stream, for per-kernel durations.
+    b.profile(True)
+    s.advance(args.steps)
+    b.sync()
+    prof = b.profile_read()
+    b.profile(False)
 
     kb = kernel_bytes(m, n, nnz)
     per_kernel = {}
@@ -128,8 +162,9 @@ def main() -> int:
     dom = max((k for k in per_kernel if k in kb), key=lambda k: per_kernel[k]["total_ms"])
     ach = per_kernel[dom]["gbs"]
     b_iter = 24 * nnz + 52 * m + 68 * n if args.precision == "fp64" else 16 * nnz + 48 * m + 60 * n
+    traffic = read_traffic(args.traffic, dom) if args.scale == 1.0 and args.precision == "fp64" else None
     roofline = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "bytes_per_launch": kb[dom]}
 
     out = {

@@ -135,7 +135,7 @@ void free_csr(DevCsr& d) {
 
 SpmvPlan plan_spmv(const int64_t* rp, int64_t nrows, const SpmvTuning& t) {
     SpmvPlan plan;
-    std::vector<Desc>& out = plan.desc;
+    std::vector<Desc> longs, stream;
     int np = 0;
     const int64_t block_nnz = std::max<int64_t>(1, std::min<int64_t>(t.block_nnz, kStreamNnzF64));
     const int64_t long_row = std::max<int64_t>(1, std::min<int64_t>(t.long_row, block_nnz));
@@ -144,13 +144,15 @@ SpmvPlan plan_spmv(const int64_t* rp, int64_t nrows, const SpmvTuning& t) {
     while (r < nrows) {
         const int64_t len = rp[r + 1] - rp[r];
         if (len > long_row) {
+            // Workgroups of its own, one per chunk; a row of several chunks
+            // is finished by the finishing kernel.
             const int64_t nch = (len + kChunk - 1) / kChunk;
             if (nch == 1) {
-                out.push_back(Desc{(int)r, -1, (int)rp[r], (int)rp[r + 1]});
+                longs.push_back(Desc{(int)r, -1, (int)rp[r], (int)rp[r + 1]});
             } else {
                 for (int64_t c = 0; c < nch; ++c) {
                     const int64_t k0 = rp[r] + c * kChunk, k1 = std::min(rp[r] + (c + 1) * kChunk, rp[r + 1]);
-                    out.push_back(Desc{(int)r, -1 - (np + (int)c), (int)k0, (int)k1});
+                    longs.push_back(Desc{(int)r, -1 - (np + (int)c), (int)k0, (int)k1});
                 }
                 plan.longs.push_back(LongRow{(int)r, np, (int)nch, 0});
                 np += (int)nch;
@@ -163,8 +165,11 @@ SpmvPlan plan_spmv(const int64_t* rp, int64_t nrows, const SpmvTuning& t) {
         const int64_t start = r, k0 = rp[r];
         while (r < nrows && rp[r + 1] - rp[r] <= long_row && rp[r + 1] - k0 <= block_nnz && r - start < max_rows)
             ++r;
-        out.push_back(Desc{(int)start, (int)r, (int)k0, (int)rp[r]});
+        stream.push_back(Desc{(int)start, (int)r, (int)k0, (int)rp[r]});
     }
+    // Long rows first: they take the longest, so they are dispatched first.
+    plan.desc = std::move(longs);
+    plan.desc.insert(plan.desc.end(), stream.begin(), stream.end());
     plan.npartials = np;
     return plan;
 }
@@ -503,8 +508,16 @@ int Hip::reduce_grid(int64_t n) const {
     return (int)std::max<int64_t>(1, std::min<int64_t>(g, kMaxReduceBlocks));
 }
 
+// Launches a reduction over `n` elements whose K totals land in `dst`
+// (device memory): per-workgroup partials, then the finishing pass.
+template <class... Args>
+void reduce_into(sp_backend* b, hipFunction_t k, int grid, int K, double* dst, Args... args) {
+    launch(b, k, (unsigned)grid, args..., b->red_partials, dst);
+    if (grid > 1) launch(b, b->k_red_finish, 1u, (const double*)b->red_partials, grid, K, dst);
+}
+
 const double* Hip::read_result(int k, int grid) {
-    if (grid > 1) launch(b, b->k_red_finish, 1u, (const double*)b->red_partials, grid, k, b->red_result);
+    (void)grid;
     hip_check(hipMemcpyAsync(b->host_result, b->red_result, sizeof(double) * (size_t)k, hipMemcpyDeviceToHost,
                              b->stream),
               "hipMemcpyAsync D2H");
@@ -512,75 +525,119 @@ const double* Hip::read_result(int k, int grid) {
     return b->host_result;
 }
 
-double Hip::dot(const sp_vec* a, const sp_vec* bv) {
+double* Hip::slot(int i) {
+    if (i < 0 || i >= kSlots) throw Error(SP_ERR_INVALID, "result slot out of range");
+    return b->slots + i;
+}
+
+const double* Hip::fetch(int count) {
+    if (count < 0 || count > kSlots) throw Error(SP_ERR_INVALID, "fetch: bad slot count");
+    hip_check(hipMemcpyAsync(b->host_slots, b->slots, sizeof(double) * (size_t)count, hipMemcpyDeviceToHost,
+                             b->stream),
+              "hipMemcpyAsync D2H");
+    hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    return b->host_slots;
+}
+
+// Raw sums -> the trait's structs (cpu/src/lib.rs:211-259).
+sp_primal_stats_t Hip::primal_from(const double* r) { return sp_primal_stats_t{r[0], r[1]}; }
+// dual_stats: objective = rows + cols (cpu/src/lib.rs:223).
+sp_dual_stats_t Hip::dual_from(const double* r) { return sp_dual_stats_t{r[1], r[0] + r[2]}; }
+// primal_ray_stats: violation = rows + cols (cpu/src/lib.rs:245).
+sp_ray_stats_t Hip::primal_ray_from(const double* r) { return sp_ray_stats_t{r[3], r[0] + r[1], r[2]}; }
+// dual_ray_stats: objective = row_obj + col_obj, violation = row_viol + col_viol (cpu/src/lib.rs:258).
+sp_ray_stats_t Hip::dual_ray_from(const double* r) { return sp_ray_stats_t{r[1] + r[4], r[0] + r[3], r[2]}; }
+
+void Hip::dot_into(const sp_vec* a, const sp_vec* bv, double* dst) {
     if (!a) throw Error(SP_ERR_INVALID, "dot: null vector");
     need_len(bv, a->len, "dot");
-    launch(b, b->k_dot, (unsigned)reduce_grid(a->len), (const double*)a->d, (const double*)bv->d, (long long)a->len,
-           b->red_partials, b->red_result);
-    return read_result(1, reduce_grid(a->len))[0];
+    reduce_into(b, b->k_dot, reduce_grid(a->len), 1, dst, (const double*)a->d, (const double*)bv->d,
+                (long long)a->len);
+}
+
+void Hip::distance_sq_into(const sp_vec* a, const sp_vec* bv, double* dst) {
+    if (!a) throw Error(SP_ERR_INVALID, "distance_sq: null vector");
+    need_len(bv, a->len, "distance_sq");
+    reduce_into(b, b->k_dist, reduce_grid(a->len), 1, dst, (const double*)a->d, (const double*)bv->d,
+                (long long)a->len);
+}
+
+void Hip::primal_stats_into(const sp_problem* p, const sp_vec* x, const sp_vec* ax, double* dst) {
+    need_len(x, p->n, "primal_stats x");
+    need_len(ax, p->m, "primal_stats ax");
+    reduce_into(b, b->k_primal_stats, reduce_grid(p->m + p->n), 2, dst, (const double*)ax->d, (const double*)p->lc,
+                (const double*)p->uc, (const double*)p->pf, (long long)p->m, (const double*)p->c,
+                (const double*)x->d, (long long)p->n);
+}
+
+void Hip::dual_stats_into(const sp_problem* p, const sp_vec* y, const sp_vec* aty, double* dst) {
+    need_len(y, p->m, "dual_stats y");
+    need_len(aty, p->n, "dual_stats aty");
+    reduce_into(b, b->k_dual_stats, reduce_grid(p->m + p->n), 3, dst, (const double*)y->d, (const double*)p->lc,
+                (const double*)p->uc, (long long)p->m, (const double*)p->c, (const double*)aty->d,
+                (const double*)p->lv, (const double*)p->uv, (const double*)p->df, (long long)p->n);
+}
+
+void Hip::step_dual_residual_sq_into(const sp_problem* p, const sp_vec* x, const sp_vec* x_t, const sp_vec* aty,
+                                     const sp_vec* aty_t, double tau, double* dst) {
+    for (const sp_vec* v : {x, x_t, aty, aty_t}) need_len(v, p->n, "step_dual_residual_sq");
+    const double inv_tau = 1.0 / tau;  // cpu/src/lib.rs:235: multiply, don't divide
+    reduce_into(b, b->k_step_dual, reduce_grid(p->n), 1, dst, (const double*)x->d, (const double*)x_t->d,
+                (const double*)aty->d, (const double*)aty_t->d, (const double*)p->df, inv_tau, (long long)p->n);
+}
+
+void Hip::primal_ray_stats_into(const sp_problem* p, const sp_vec* dx, const sp_vec* adx, double* dst) {
+    need_len(dx, p->n, "primal_ray_stats dx");
+    need_len(adx, p->m, "primal_ray_stats adx");
+    reduce_into(b, b->k_primal_ray, reduce_grid(p->m + p->n), 4, dst, (const double*)adx->d, (const double*)p->lc,
+                (const double*)p->uc, (long long)p->m, (const double*)dx->d, (const double*)p->c,
+                (const double*)p->lv, (const double*)p->uv, (long long)p->n);
+}
+
+void Hip::dual_ray_stats_into(const sp_problem* p, const sp_vec* dy, const sp_vec* atdy, double* dst) {
+    need_len(dy, p->m, "dual_ray_stats dy");
+    need_len(atdy, p->n, "dual_ray_stats atdy");
+    reduce_into(b, b->k_dual_ray, reduce_grid(p->m + p->n), 5, dst, (const double*)dy->d, (const double*)p->lc,
+                (const double*)p->uc, (long long)p->m, (const double*)atdy->d, (const double*)p->lv,
+                (const double*)p->uv, (long long)p->n);
+}
+
+double Hip::dot(const sp_vec* a, const sp_vec* bv) {
+    dot_into(a, bv, b->red_result);
+    return read_result(1, 0)[0];
 }
 
 double Hip::norm_sq(const sp_vec* v) { return dot(v, v); }
 
 double Hip::distance_sq(const sp_vec* a, const sp_vec* bv) {
-    if (!a) throw Error(SP_ERR_INVALID, "distance_sq: null vector");
-    need_len(bv, a->len, "distance_sq");
-    launch(b, b->k_dist, (unsigned)reduce_grid(a->len), (const double*)a->d, (const double*)bv->d,
-           (long long)a->len, b->red_partials, b->red_result);
-    return read_result(1, reduce_grid(a->len))[0];
+    distance_sq_into(a, bv, b->red_result);
+    return read_result(1, 0)[0];
 }
 
 sp_primal_stats_t Hip::primal_stats(const sp_problem* p, const sp_vec* x, const sp_vec* ax) {
-    need_len(x, p->n, "primal_stats x");
-    need_len(ax, p->m, "primal_stats ax");
-    launch(b, b->k_primal_stats, (unsigned)reduce_grid(p->m + p->n), (const double*)ax->d, (const double*)p->lc,
-           (const double*)p->uc, (const double*)p->pf, (long long)p->m, (const double*)p->c, (const double*)x->d,
-           (long long)p->n, b->red_partials, b->red_result);
-    const double* r = read_result(2, reduce_grid(p->m + p->n));
-    return sp_primal_stats_t{r[0], r[1]};
+    primal_stats_into(p, x, ax, b->red_result);
+    return primal_from(read_result(2, 0));
 }
 
 sp_dual_stats_t Hip::dual_stats(const sp_problem* p, const sp_vec* y, const sp_vec* aty) {
-    need_len(y, p->m, "dual_stats y");
-    need_len(aty, p->n, "dual_stats aty");
-    launch(b, b->k_dual_stats, (unsigned)reduce_grid(p->m + p->n), (const double*)y->d, (const double*)p->lc,
-           (const double*)p->uc, (long long)p->m, (const double*)p->c, (const double*)aty->d, (const double*)p->lv,
-           (const double*)p->uv, (const double*)p->df, (long long)p->n, b->red_partials, b->red_result);
-    const double* r = read_result(3, reduce_grid(p->m + p->n));
-    // cpu/src/lib.rs:223: objective = rows + cols.
-    return sp_dual_stats_t{r[1], r[0] + r[2]};
+    dual_stats_into(p, y, aty, b->red_result);
+    return dual_from(read_result(3, 0));
 }
 
 double Hip::step_dual_residual_sq(const sp_problem* p, const sp_vec* x, const sp_vec* x_t, const sp_vec* aty,
                                   const sp_vec* aty_t, double tau) {
-    for (const sp_vec* v : {x, x_t, aty, aty_t}) need_len(v, p->n, "step_dual_residual_sq");
-    const double inv_tau = 1.0 / tau;  // cpu/src/lib.rs:235: multiply, don't divide
-    launch(b, b->k_step_dual, (unsigned)reduce_grid(p->n), (const double*)x->d, (const double*)x_t->d,
-           (const double*)aty->d, (const double*)aty_t->d, (const double*)p->df, inv_tau, (long long)p->n,
-           b->red_partials, b->red_result);
-    return read_result(1, reduce_grid(p->n))[0];
+    step_dual_residual_sq_into(p, x, x_t, aty, aty_t, tau, b->red_result);
+    return read_result(1, 0)[0];
 }
 
 sp_ray_stats_t Hip::primal_ray_stats(const sp_problem* p, const sp_vec* dx, const sp_vec* adx) {
-    need_len(dx, p->n, "primal_ray_stats dx");
-    need_len(adx, p->m, "primal_ray_stats adx");
-    launch(b, b->k_primal_ray, (unsigned)reduce_grid(p->m + p->n), (const double*)adx->d, (const double*)p->lc,
-           (const double*)p->uc, (long long)p->m, (const double*)dx->d, (const double*)p->c, (const double*)p->lv,
-           (const double*)p->uv, (long long)p->n, b->red_partials, b->red_result);
-    const double* r = read_result(4, reduce_grid(p->m + p->n));
-    // cpu/src/lib.rs:245: violation = rows + cols.
-    return sp_ray_stats_t{r[3], r[0] + r[1], r[2]};
+    primal_ray_stats_into(p, dx, adx, b->red_result);
+    return primal_ray_from(read_result(4, 0));
 }
 
 sp_ray_stats_t Hip::dual_ray_stats(const sp_problem* p, const sp_vec* dy, const sp_vec* atdy) {
-    need_len(dy, p->m, "dual_ray_stats dy");
-    need_len(atdy, p->n, "dual_ray_stats atdy");
-    launch(b, b->k_dual_ray, (unsigned)reduce_grid(p->m + p->n), (const double*)dy->d, (const double*)p->lc,
-           (const double*)p->uc, (long long)p->m, (const double*)atdy->d, (const double*)p->lv, (const double*)p->uv,
-           (long long)p->n, b->red_partials, b->red_result);
-    const double* r = read_result(5, reduce_grid(p->m + p->n));
-    // cpu/src/lib.rs:258: objective = row_obj + col_obj, violation = row_viol + col_viol.
-    return sp_ray_stats_t{r[1] + r[4], r[0] + r[3], r[2]};
+    dual_ray_stats_into(p, dy, atdy, b->red_result);
+    return dual_ray_from(read_result(5, 0));
 }
 
 }  // namespace sp
@@ -666,6 +723,10 @@ sp_backend* sp_backend_open(int ordinal) {
         get(b->k_dual_ray, "sp_red_dual_ray_f64");
         b->red_partials = sp::dmalloc<double>((size_t)sp::Hip::kMaxReduceBlocks * sp::Hip::kMaxSlots);
         b->red_result = sp::dmalloc<double>(sp::Hip::kMaxSlots);
+        b->slots = sp::dmalloc<double>(sp::Hip::kSlots);
+        sp::hip_check(hipHostMalloc(reinterpret_cast<void**>(&b->host_slots), sizeof(double) * sp::Hip::kSlots,
+                                    hipHostMallocDefault),
+                      "hipHostMalloc");
         sp::hip_check(hipHostMalloc(reinterpret_cast<void**>(&b->host_result), sizeof(double) * sp::Hip::kMaxSlots,
                                     hipHostMallocDefault),
                       "hipHostMalloc");
@@ -685,6 +746,8 @@ void sp_backend_close(sp_backend* b) {
     sp::dfree(b->red_partials);
     sp::dfree(b->red_result);
     if (b->host_result) (void)hipHostFree(b->host_result);
+    if (b->host_slots) (void)hipHostFree(b->host_slots);
+    sp::dfree(b->slots);
     if (b->module) (void)hipModuleUnload(b->module);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;

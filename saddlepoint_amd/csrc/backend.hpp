@@ -93,6 +93,10 @@ struct sp_backend {
     double* red_partials = nullptr;
     double* red_result = nullptr;
     double* host_result = nullptr;
+    // Result slots for reductions whose values are read back together
+    // (one transfer per check iteration in the fused driver).
+    double* slots = nullptr;
+    double* host_slots = nullptr;
 
     // Kernel timing (sp_backend_profile): while enabled, every launch is
     // bracketed by two events on the backend's stream.
@@ -128,6 +132,7 @@ struct Hip {
 
     static constexpr int kMaxReduceBlocks = 1024;
     static constexpr int kMaxSlots = 8;
+    static constexpr int kSlots = 64;
 
     std::string name() const { return b->name; }
     sp_problem* load(const ScaledProblem& s);
@@ -184,6 +189,22 @@ struct Hip {
     // copies n doubles, 1 is a triad over n doubles, 2 gathers n doubles at
     // pseudo-random positions of a table of table_len doubles.
     double probe(int kind, int64_t n, int64_t table_len, int reps, uint64_t seed);
+
+    // Reductions into device result slots, read back together by fetch().
+    double* slot(int i);
+    const double* fetch(int count);
+    void dot_into(const sp_vec* a, const sp_vec* bv, double* dst);
+    void distance_sq_into(const sp_vec* a, const sp_vec* bv, double* dst);
+    void primal_stats_into(const sp_problem* p, const sp_vec* x, const sp_vec* ax, double* dst);      // 2 sums
+    void dual_stats_into(const sp_problem* p, const sp_vec* y, const sp_vec* aty, double* dst);       // 3 sums
+    void step_dual_residual_sq_into(const sp_problem* p, const sp_vec* x, const sp_vec* x_t, const sp_vec* aty,
+                                    const sp_vec* aty_t, double tau, double* dst);
+    void primal_ray_stats_into(const sp_problem* p, const sp_vec* dx, const sp_vec* adx, double* dst);  // 4 sums
+    void dual_ray_stats_into(const sp_problem* p, const sp_vec* dy, const sp_vec* atdy, double* dst);   // 5 sums
+    static sp_primal_stats_t primal_from(const double* raw);
+    static sp_dual_stats_t dual_from(const double* raw);
+    static sp_ray_stats_t primal_ray_from(const double* raw);
+    static sp_ray_stats_t dual_ray_from(const double* raw);
 
     void sync();
 

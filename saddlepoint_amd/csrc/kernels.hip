@@ -20,11 +20,12 @@ typedef long long i64;
 #define WAVE 64
 #define NWAVES (WG / WAVE)
 
-// Stream blocks stage at most this many products in LDS; a longer row is split
-// into chunks of SP_CHUNK nonzeros, one workgroup each.
+// Stream blocks stage at most this many products in LDS. A longer row gets
+// workgroups of its own, one per SP_CHUNK nonzeros.
 #define SP_NNZ_F64 2048
 #define SP_NNZ_F32 4096
-#define SP_CHUNK 4096
+#define SP_CHUNK 4096   // nonzeros of a long row per workgroup
+#define SP_BATCH 4096   // nonzeros a workgroup loads at once
 
 // One SpMV work item. Stream block: rows [r0, r1) whose nonzeros are [k0, k1).
 // Long-row chunk: row r0, nonzeros [k0, k1) of it, r1 = -1 - partial slot.
@@ -41,6 +42,14 @@ __device__ __forceinline__ T clampv(T v, T lo, T hi) {
     if (v > hi) v = hi;
     return v;
 }
+// Epilogue outputs are written with non-temporal stores: they stream out
+// during the kernel instead of sitting dirty in the XCD's L2 until the
+// kernel-end write-back.
+template <class T>
+__device__ __forceinline__ void st(T* p, T v) {
+    __builtin_nontemporal_store(v, p);
+}
+
 __device__ __forceinline__ double excess(double v, double lo, double hi) {
     return v < lo ? v - lo : (v > hi ? v - hi : 0.0);
 }
@@ -206,24 +215,27 @@ __device__ __forceinline__ void spmv_body(const Desc* __restrict__ desc, const i
             if (rr < nr) epi.apply(r0 + rr, sums[rr], pre[jj]);
         }
     } else {
-        const int r = d.r0, pslot = -1 - d.r1, k0 = d.k0, n = d.k1 - d.k0;
-        // One chunk (n <= SP_CHUNK, n > 0): the same straight-line pattern as
-        // the stream blocks, with the products summed in registers.
-        constexpr int PERL = SP_CHUNK / WG;
-        const int lastl = n - 1;
-        int c[PERL];
-        V v[PERL];
-#pragma unroll
-        for (int u = 0; u < PERL; ++u) c[u] = __builtin_nontemporal_load(ci + k0 + min(tid + u * WG, lastl));
-#pragma unroll
-        for (int u = 0; u < PERL; ++u) v[u] = __builtin_nontemporal_load(va + k0 + min(tid + u * WG, lastl));
-        X xv[PERL];
-#pragma unroll
-        for (int u = 0; u < PERL; ++u) xv[u] = x[c[u]];
+        const int r = d.r0, pslot = -1 - d.r1, k0 = d.k0, k1 = d.k1;
+        // Up to SP_CHUNK nonzeros of one row, in batches of SP_BATCH with the
+        // same straight-line pattern as the stream blocks; each thread keeps
+        // its running sum in a register across batches.
+        constexpr int PERL = SP_BATCH / WG;
         A s[1] = {0};
+        for (int base = k0; base < k1; base += SP_BATCH) {
+            const int n = min(SP_BATCH, k1 - base), lastl = n - 1;
+            int c[PERL];
+            V v[PERL];
 #pragma unroll
-        for (int u = 0; u < PERL; ++u)
-            if (tid + u * WG < n) s[0] += (A)v[u] * (A)xv[u];
+            for (int u = 0; u < PERL; ++u) c[u] = __builtin_nontemporal_load(ci + base + min(tid + u * WG, lastl));
+#pragma unroll
+            for (int u = 0; u < PERL; ++u) v[u] = __builtin_nontemporal_load(va + base + min(tid + u * WG, lastl));
+            X xv[PERL];
+#pragma unroll
+            for (int u = 0; u < PERL; ++u) xv[u] = x[c[u]];
+#pragma unroll
+            for (int u = 0; u < PERL; ++u)
+                if (tid + u * WG < n) s[0] += (A)v[u] * (A)xv[u];
+        }
         block_sum<1>(s);
         const int rs = rp[r], len = rp[r + 1] - rs;
         const int nch = (len + SP_CHUNK - 1) / SP_CHUNK;
@@ -264,7 +276,7 @@ struct EpiStore {
     A* out;
     struct Pre {};
     __device__ Pre load(int) const { return {}; }
-    __device__ void apply(int r, A s, const Pre&) const { out[r] = s; }
+    __device__ void apply(int r, A s, const Pre&) const { st(out + r, s); }
 };
 
 // spmv_t + primal_step: aty = A'ᵀy; x_t = clamp(x - τ(c - aty), lv, uv);
@@ -279,9 +291,9 @@ struct EpiPrimal {
     __device__ Pre load(int j) const { return {x[j], c[j], lv[j], uv[j]}; }
     __device__ void apply(int j, double s, const Pre& q) const {
         const double t = clampv(q.x - tau * (q.c - s), q.lv, q.uv);
-        aty[j] = s;
-        x_t[j] = t;
-        x_bar[j] = 2.0 * t - q.x;
+        st(aty + j, s);
+        st(x_t + j, t);
+        st(x_bar + j, 2.0 * t - q.x);
     }
 };
 
@@ -298,8 +310,8 @@ struct EpiPrimalHalpern {
     __device__ Pre load(int j) const { return {x[j], c[j], lv[j], uv[j], anchor[j]}; }
     __device__ void apply(int j, double s, const Pre& q) const {
         const double t = clampv(q.x - tau * (q.c - s), q.lv, q.uv);
-        x_bar[j] = 2.0 * t - q.x;
-        x[j] = w * ((1.0 + rho) * t - rho * q.x) + (1.0 - w) * q.anchor;
+        st(x_bar + j, 2.0 * t - q.x);
+        st(x + j, w * ((1.0 + rho) * t - rho * q.x) + (1.0 - w) * q.anchor);
     }
 };
 
@@ -315,8 +327,8 @@ struct EpiDual {
     __device__ Pre load(int i) const { return {y[i], lc[i], uc[i]}; }
     __device__ void apply(int i, double s, const Pre& q) const {
         const double v = s - q.y / sigma;
-        ax[i] = s;
-        y_t[i] = q.y - sigma * (s - clampv(v, q.lc, q.uc));
+        st(ax + i, s);
+        st(y_t + i, q.y - sigma * (s - clampv(v, q.lc, q.uc)));
     }
 };
 
@@ -331,7 +343,7 @@ struct EpiDualHalpern {
     __device__ void apply(int i, double s, const Pre& q) const {
         const double v = s - q.y / sigma;
         const double t = q.y - sigma * (s - clampv(v, q.lc, q.uc));
-        y[i] = w * ((1.0 + rho) * t - rho * q.y) + (1.0 - w) * q.anchor;
+        st(y + i, w * ((1.0 + rho) * t - rho * q.y) + (1.0 - w) * q.anchor);
     }
 };
 
@@ -441,8 +453,8 @@ struct EpiPrimalMixed {
         const double d = (double)q.dx;
         const double aty = q.atya + (double)s;
         const double t = clampv(q.xa + d - tau * (q.c - aty), q.lv, q.uv) - q.xa;
-        dx_bar[j] = (float)(2.0 * t - d);
-        dx[j] = (float)(w * ((1.0 + rho) * t - rho * d));
+        st(dx_bar + j, (float)(2.0 * t - d));
+        st(dx + j, (float)(w * ((1.0 + rho) * t - rho * d)));
     }
 };
 
@@ -461,7 +473,7 @@ struct EpiDualMixed {
         const double ax = q.axa + (double)s;
         const double v = ax - y / sigma;
         const double t = y - sigma * (ax - clampv(v, q.lc, q.uc)) - q.ya;
-        dy[i] = (float)(w * ((1.0 + rho) * t - rho * d));
+        st(dy + i, (float)(w * ((1.0 + rho) * t - rho * d)));
     }
 };
 

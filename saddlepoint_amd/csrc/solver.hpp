@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <limits>
 #include <memory>
+#include <utility>
 
 #include "host.hpp"
 #include "saddlepoint_hip.h"
@@ -151,8 +152,8 @@ class Solver {
                 shadow_live_ = false;
             }
             if (p.fused) {
-                b_.spmv_t_primal(dev_.get(), y_.get(), x_.get(), tau, aty_.get(), x_t_.get(), x_bar_.get());
-                b_.spmv_dual(dev_.get(), x_bar_.get(), y_.get(), sigma, ax_.get(), y_t_.get());
+                if (deferred_step(check, tau, sigma, next_total)) break;
+                continue;
             } else {
                 b_.spmv_t(dev_.get(), y_.get(), aty_.get());
                 b_.primal_step(dev_.get(), x_.get(), aty_.get(), tau, x_t_.get(), x_bar_.get());
@@ -176,12 +177,7 @@ class Solver {
             k_ += 1;
             if (!check) continue;
             checks_ += 1;
-            if (p.log_every_checks > 0 && checks_ % (uint64_t)p.log_every_checks == 0) {
-                std::fprintf(stderr, "%9llu %14.6e %14.6e %9.2e %9.2e %9.2e %9.2e %5llu %8.2fs %7llu %9.2e\n",
-                             (unsigned long long)total_, kkt_.primal_obj, kkt_.dual_obj, kkt_.primal_res,
-                             kkt_.dual_res, kkt_.gap(), omega_, (unsigned long long)restarts_, elapsed(),
-                             (unsigned long long)k_, residual / anchor_residual_);
-            }
+            log_check(residual);
             Kkt test = kkt_;
             if (p.termination == 1) {
                 test.dual_obj = step_obj;
@@ -205,27 +201,10 @@ class Solver {
                 stop(SP_TIME_LIMIT);
                 break;
             }
-            // Restart decision on the fixed-point residual (solver.rs:396-419).
-            const double r0 = anchor_residual_;
-            const bool restart = residual <= p.restart_sufficient * r0 ||
-                                 (residual <= p.restart_necessary * r0 && residual > prev_residual_) ||
-                                 (double)k_ >= p.restart_artificial * (double)total_;
-            prev_residual_ = residual;
-            if (restart) {
-                const double dx = std::sqrt(b_.distance_sq(x_t_.get(), x_anchor_.get()));
-                const double dy = std::sqrt(b_.distance_sq(y_t_.get(), y_anchor_.get()));
-                const double rel_primal = kkt_.primal_res / (1.0 + b_norm_);
-                // The PID balances against the step-based dual residual.
-                const double rel_dual = step_dual / (1.0 + c_norm_);
-                omega_ = pid_.update(p, omega_, dx, dy, rel_primal, rel_dual);
-                b_.copy(x_t_.get(), x_anchor_.get());
-                b_.copy(y_t_.get(), y_anchor_.get());
-                b_.copy(x_t_.get(), x_.get());
-                b_.copy(y_t_.get(), y_.get());
-                k_ = 0;
-                restarts_ += 1;
-                prev_residual_ = INFINITY;
-            }
+            restart_decision(residual, step_dual, [&] {
+                return std::make_pair(std::sqrt(b_.distance_sq(x_t_.get(), x_anchor_.get())),
+                                      std::sqrt(b_.distance_sq(y_t_.get(), y_anchor_.get())));
+            });
         }
         return terminated_;
     }
@@ -260,6 +239,199 @@ class Solver {
         s.mixed_fp64_iterations = fp64_plain_;
         return s;
     }
+
+   private:
+    // Result slots of the deferred check (fused path): everything a check
+    // needs is reduced on the device and read back in one transfer.
+    enum Slot {
+        kFix = 0,         // dx², dy², cross of the fixed-point residual (3)
+        kAnchorFix = 3,   // the same for a k = 0 iteration between checks (3)
+        kPrimal = 6,      // primal_stats sums (2)
+        kDual = 8,        // dual_stats sums (3)
+        kStep = 11,       // step_dual_residual_sq (1)
+        kStepRows = 12,   // dual_stats(y_t, c) sums, step-implied termination (3)
+        kStepDots = 15,   // c·x_t, aty·x_t, x_t·x_t, x·x_t (4)
+        kDualRay = 19,    // dual_ray_stats sums (5)
+        kPrimalRay = 24,  // primal_ray_stats sums (4)
+        kDistX = 28,      // ‖x_t - x_anchor‖² (1)
+        kDistY = 29,      // ‖y_t - y_anchor‖² (1)
+        kSlotCount = 30
+    };
+
+    void enqueue_fixed_point(int base) {
+        b_.sub(x_t_.get(), x_.get(), tmp_n_.get());
+        b_.sub(y_t_.get(), y_.get(), tmp_m2_.get());
+        b_.spmv(dev_.get(), tmp_n_.get(), tmp_m_.get());
+        b_.dot_into(tmp_n_.get(), tmp_n_.get(), b_.slot(base));
+        b_.dot_into(tmp_m2_.get(), tmp_m2_.get(), b_.slot(base + 1));
+        b_.dot_into(tmp_m2_.get(), tmp_m_.get(), b_.slot(base + 2));
+    }
+
+    // fixed_point_residual's closing arithmetic (solver.rs:283-287).
+    static double residual_of(const double* r, double eta, double omega) {
+        const double r2 = omega / eta * r[0] + r[1] / (eta * omega) + 2.0 * r[2];
+        if (!std::isfinite(r2)) throw Error(SP_ERR_NUMERICAL, "numerical breakdown: iterates are not finite");
+        return std::sqrt(r2 > 0.0 ? r2 : 0.0);
+    }
+
+    // One non-plain iteration of the fused path: the reference's calls in
+    // the reference's order, with the reductions written to result slots and
+    // the infeasibility rays and restart distances formed speculatively, so
+    // that a check costs one device-to-host transfer. The decisions that
+    // follow are those of solver.rs:340-419, on the same values. Returns true
+    // when the loop has ended.
+    bool deferred_step(bool check, double tau, double sigma, uint64_t next_total) {
+        const sp_params& p = p_;
+        b_.spmv_t_primal(dev_.get(), y_.get(), x_.get(), tau, aty_.get(), x_t_.get(), x_bar_.get());
+        b_.spmv_dual(dev_.get(), x_bar_.get(), y_.get(), sigma, ax_.get(), y_t_.get());
+        total_ = next_total;
+        const bool k_was_zero = k_ == 0;
+        if (check) {
+            enqueue_fixed_point(kFix);
+        } else if (k_was_zero) {
+            // r(z_anchor), needed at the next check (solver.rs:341-346).
+            enqueue_fixed_point(kAnchorFix);
+            anchor_pending_ = true;
+            anchor_omega_ = omega_;
+        }
+        if (check) {
+            b_.spmv(dev_.get(), x_t_.get(), tmp_m_.get());
+            b_.primal_stats_into(dev_.get(), x_t_.get(), tmp_m_.get(), b_.slot(kPrimal));
+            b_.spmv_t(dev_.get(), y_t_.get(), tmp_n_.get());
+            b_.dual_stats_into(dev_.get(), y_t_.get(), tmp_n_.get(), b_.slot(kDual));
+            b_.step_dual_residual_sq_into(dev_.get(), x_.get(), x_t_.get(), aty_.get(), tmp_n_.get(), tau,
+                                          b_.slot(kStep));
+            if (p.termination == 1) {
+                b_.dual_stats_into(dev_.get(), y_t_.get(), c_.get(), b_.slot(kStepRows));
+                b_.dot_into(c_.get(), x_t_.get(), b_.slot(kStepDots));
+                b_.dot_into(aty_.get(), x_t_.get(), b_.slot(kStepDots + 1));
+                b_.dot_into(x_t_.get(), x_t_.get(), b_.slot(kStepDots + 2));
+                b_.dot_into(x_.get(), x_t_.get(), b_.slot(kStepDots + 3));
+            }
+        }
+        const double w = (double)(k_ + 1) / (double)(k_ + 2);
+        b_.halpern_step(x_.get(), x_t_.get(), x_anchor_.get(), w, p.reflection);
+        b_.halpern_step(y_.get(), y_t_.get(), y_anchor_.get(), w, p.reflection);
+        k_ += 1;
+        if (!check) return false;
+
+        // Speculative: the infeasibility rays (solver.rs:291-309) and the
+        // restart displacements (solver.rs:403-404).
+        b_.sub(y_t_.get(), y_anchor_.get(), tmp_m_.get());
+        b_.spmv_t(dev_.get(), tmp_m_.get(), tmp_n_.get());
+        b_.dual_ray_stats_into(dev_.get(), tmp_m_.get(), tmp_n_.get(), b_.slot(kDualRay));
+        b_.sub(x_t_.get(), x_anchor_.get(), tmp_n_.get());
+        b_.spmv(dev_.get(), tmp_n_.get(), tmp_m_.get());
+        b_.primal_ray_stats_into(dev_.get(), tmp_n_.get(), tmp_m_.get(), b_.slot(kPrimalRay));
+        b_.distance_sq_into(x_t_.get(), x_anchor_.get(), b_.slot(kDistX));
+        b_.distance_sq_into(y_t_.get(), y_anchor_.get(), b_.slot(kDistY));
+        const double* r = b_.fetch(kSlotCount);
+
+        if (anchor_pending_) {
+            anchor_residual_ = residual_of(r + kAnchorFix, eta_, anchor_omega_);
+            anchor_pending_ = false;
+        }
+        const double residual = residual_of(r + kFix, eta_, omega_);
+        if (k_was_zero) anchor_residual_ = residual;
+        const double f = scaled_.objective_factor();
+        const sp_primal_stats_t ps = B::primal_from(r + kPrimal);
+        const sp_dual_stats_t ds = B::dual_from(r + kDual);
+        kkt_.primal_obj = ps.objective * f + c0_;
+        kkt_.dual_obj = ds.objective * f + c0_;
+        kkt_.primal_res = std::sqrt(ps.residual_sq);
+        kkt_.dual_res = std::sqrt(ds.residual_sq);
+        const double step_dual = std::sqrt(r[kStep]);
+        checks_ += 1;
+        log_check(residual);
+        Kkt test = kkt_;
+        if (p.termination == 1) {
+            const double rows = B::dual_from(r + kStepRows).objective;
+            const double* d = r + kStepDots;
+            const double lx = d[0] - d[1] + (d[2] - d[3]) / tau;
+            test.dual_obj = (rows + lx) * f + c0_;
+            test.dual_res = step_dual;
+        }
+        if (test.converged(p.tolerance, b_norm_, c_norm_)) {
+            stop(SP_OPTIMAL);
+            return true;
+        }
+        watch_progress(test);
+        const int inf = infeasibility_of(B::dual_ray_from(r + kDualRay), B::primal_ray_from(r + kPrimalRay));
+        if (inf >= 0) {
+            stop(inf);
+            return true;
+        }
+        if (total_ >= p.max_iterations) {
+            stop(SP_ITERATION_LIMIT);
+            return true;
+        }
+        if (p.time_limit_s >= 0.0 && elapsed() >= p.time_limit_s) {
+            stop(SP_TIME_LIMIT);
+            return true;
+        }
+        restart_decision(residual, step_dual,
+                         [&] { return std::make_pair(std::sqrt(r[kDistX]), std::sqrt(r[kDistY])); });
+        return false;
+    }
+
+    // The progress line of solver.rs:367-378.
+    void log_check(double residual) {
+        if (p_.log_every_checks > 0 && checks_ % (uint64_t)p_.log_every_checks == 0) {
+            std::fprintf(stderr, "%9llu %14.6e %14.6e %9.2e %9.2e %9.2e %9.2e %5llu %8.2fs %7llu %9.2e\n",
+                         (unsigned long long)total_, kkt_.primal_obj, kkt_.dual_obj, kkt_.primal_res, kkt_.dual_res,
+                         kkt_.gap(), omega_, (unsigned long long)restarts_, elapsed(), (unsigned long long)k_,
+                         residual / anchor_residual_);
+        }
+    }
+
+    // Restart decision on the fixed-point residual (solver.rs:396-419).
+    // `distances` yields (‖x_t - x_anchor‖, ‖y_t - y_anchor‖).
+    template <class D>
+    void restart_decision(double residual, double step_dual, D&& distances) {
+        const sp_params& p = p_;
+        const double r0 = anchor_residual_;
+        const bool restart = residual <= p.restart_sufficient * r0 ||
+                             (residual <= p.restart_necessary * r0 && residual > prev_residual_) ||
+                             (double)k_ >= p.restart_artificial * (double)total_;
+        prev_residual_ = residual;
+        if (!restart) return;
+        const auto [dx, dy] = distances();
+        const double rel_primal = kkt_.primal_res / (1.0 + b_norm_);
+        // The PID balances against the step-based dual residual.
+        const double rel_dual = step_dual / (1.0 + c_norm_);
+        omega_ = pid_.update(p, omega_, dx, dy, rel_primal, rel_dual);
+        b_.copy(x_t_.get(), x_anchor_.get());
+        b_.copy(y_t_.get(), y_anchor_.get());
+        b_.copy(x_t_.get(), x_.get());
+        b_.copy(y_t_.get(), y_.get());
+        k_ = 0;
+        restarts_ += 1;
+        prev_residual_ = INFINITY;
+    }
+
+    // The certificate tests of solver.rs:298 and :305.
+    bool dual_ray_certifies(const sp_ray_stats_t& d) const {
+        const double eps = p_.infeasibility_tolerance;
+        return d.objective > 0.0 && std::sqrt(d.violation_sq) <= eps * d.objective &&
+               d.objective > eps * std::sqrt(d.norm_sq);
+    }
+    bool primal_ray_certifies(const sp_ray_stats_t& pr) const {
+        const double eps = p_.infeasibility_tolerance;
+        return pr.objective < 0.0 && std::sqrt(pr.violation_sq) <= -eps * pr.objective &&
+               -pr.objective > eps * std::sqrt(pr.norm_sq);
+    }
+
+    // infeasibility() on rays that were already reduced.
+    int infeasibility_of(const sp_ray_stats_t& d, const sp_ray_stats_t& pr) {
+        if (dual_ray_certifies(d)) return SP_PRIMAL_INFEASIBLE;
+        near_certificate(d.objective, d.violation_sq, d.norm_sq);
+        if (primal_ray_certifies(pr)) return SP_DUAL_INFEASIBLE;
+        near_certificate(-pr.objective, pr.violation_sq, pr.norm_sq);
+        return -1;
+    }
+
+    bool anchor_pending_ = false;
+    double anchor_omega_ = 1.0;
 
    private:
     using clock = std::chrono::steady_clock;
@@ -335,20 +507,15 @@ class Solver {
 
     // z_t - z_anchor as an infeasibility certificate (solver.rs:291-309); -1 if none.
     int infeasibility() {
-        const double eps = p_.infeasibility_tolerance;
         b_.sub(y_t_.get(), y_anchor_.get(), tmp_m_.get());
         b_.spmv_t(dev_.get(), tmp_m_.get(), tmp_n_.get());
         const sp_ray_stats_t d = b_.dual_ray_stats(dev_.get(), tmp_m_.get(), tmp_n_.get());
-        if (d.objective > 0.0 && std::sqrt(d.violation_sq) <= eps * d.objective &&
-            d.objective > eps * std::sqrt(d.norm_sq))
-            return SP_PRIMAL_INFEASIBLE;
+        if (dual_ray_certifies(d)) return SP_PRIMAL_INFEASIBLE;
         near_certificate(d.objective, d.violation_sq, d.norm_sq);
         b_.sub(x_t_.get(), x_anchor_.get(), tmp_n_.get());
         b_.spmv(dev_.get(), tmp_n_.get(), tmp_m_.get());
         const sp_ray_stats_t pr = b_.primal_ray_stats(dev_.get(), tmp_n_.get(), tmp_m_.get());
-        if (pr.objective < 0.0 && std::sqrt(pr.violation_sq) <= -eps * pr.objective &&
-            -pr.objective > eps * std::sqrt(pr.norm_sq))
-            return SP_DUAL_INFEASIBLE;
+        if (primal_ray_certifies(pr)) return SP_DUAL_INFEASIBLE;
         near_certificate(-pr.objective, pr.violation_sq, pr.norm_sq);
         return -1;
     }

@@ -144,6 +144,23 @@ def test_backend_ops_match_cpu_power_law(gpu):
     compare("config1@0.1", generate.config1(scale=0.1), gpu)
 
 
+def test_backend_ops_match_cpu_very_long_rows(gpu):
+    # Two rows far longer than one workgroup's chunk (4,096 nonzeros), so they are
+    # split across workgroups and combined by the finishing kernel.
+    rng = np.random.default_rng(21)
+    m, n = 3000, 150_000
+    cols = np.repeat(np.arange(n), 3)
+    rows = rng.integers(2, m, cols.size)
+    long_cols = np.concatenate([rng.choice(n, 140_000, replace=False), rng.choice(n, 70_000, replace=False)])
+    long_rows = np.concatenate([np.zeros(140_000, np.int64), np.ones(70_000, np.int64)])
+    r = np.concatenate([rows, long_rows])
+    c = np.concatenate([cols, long_cols])
+    a = generate._csr_from_rows_cols(m, n, r, c, generate._values(rng, r.size, -2.0, 2.0))
+    p = generate._finish(rng, a, eq_frac=0.6)
+    assert np.diff(p.a.row_ptr).max() > 4096
+    compare("very-long-rows", p, gpu)
+
+
 def test_spmv_is_deterministic(gpu):
     p = generate.config1(scale=0.05)
     _, sp = scaled_pair(p)

@@ -57,11 +57,15 @@ def test_netlib_slow_gpu_matches_cpu(gpu, name):
     run_pair(gpu, name, netlib(name))
 
 
-@pytest.mark.parametrize("name", ["afiro", "israel", "shell", "forest6"])
-def test_fused_path_is_bitwise_the_method_path(gpu, name):
+@pytest.mark.parametrize("name", ["afiro", "israel", "shell", "forest6", "woodinfe"])
+@pytest.mark.parametrize("termination", ["projected", "step_implied"])
+def test_fused_path_is_bitwise_the_method_path(gpu, name, termination):
+    """The fused kernels and the one-readback check (fused = 1) give exactly
+    the iterates, counts and answer of the one-kernel-per-method path that
+    follows the reference's call sequence (fused = 0)."""
     p = netlib(name)
-    a = solve(gpu, p, default_params(fused=0))
-    b = solve(gpu, p, default_params(fused=1))
+    a = solve(gpu, p, default_params(fused=0, termination=termination))
+    b = solve(gpu, p, default_params(fused=1, termination=termination))
     assert a.status == b.status and a.iterations == b.iterations and a.restarts == b.restarts
     assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
     assert a.objective == b.objective
