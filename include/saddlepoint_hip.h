@@ -210,6 +210,42 @@ int sp_probe(sp_backend *b, int kind, int64_t n, int64_t table_len, int reps, do
 /* Waits for all work queued on the backend's stream. */
 int sp_backend_sync(sp_backend *b);
 
+/* ---- serving: continuous batching of many LPs on one GPU ---------------------- */
+
+/* No counterpart in the reference (it solves one LP per call). LPs are
+ * submitted at any time; host worker threads validate and scale them; a
+ * chunk runs 64 iterations of every admitted LP (batched plain iterations,
+ * per-LP checks) and admits and retires LPs at its edges, by HBM budget and
+ * slot count. Each LP's result is bitwise the one it gets alone. Submitted
+ * params need check_every = 64; max_iterations is rounded up to a multiple
+ * of 64. Not thread-safe per server except sp_server_submit and
+ * sp_server_state, which may be called while another thread runs chunks. */
+typedef struct sp_server sp_server;
+typedef struct {
+    int64_t budget_bytes; /* device bytes the admitted LPs may hold (default 64 GiB) */
+    int max_active;       /* most LPs admitted at once (default 64) */
+    int prep_threads;     /* host threads validating and scaling (default 8) */
+    int use_graphs;       /* replay the batched steps of a chunk from a hipGraph (default 1) */
+} sp_server_params;
+/* Seconds on the server's clock (from its creation). */
+typedef struct { double submitted, admitted, completed; } sp_job_times;
+enum { SP_JOB_QUEUED = 0, SP_JOB_PREPARED = 1, SP_JOB_ACTIVE = 2, SP_JOB_DONE = 3, SP_JOB_FAILED = 4 };
+
+void sp_default_server_params(sp_server_params *p);
+int sp_server_create(sp_backend *b, const sp_server_params *params, sp_server **out);
+void sp_server_free(sp_server *s);
+/* Copies the LP; *id identifies it from then on. */
+int sp_server_submit(sp_server *s, const sp_lp_view *lp, const sp_params *params, uint64_t *id);
+/* One chunk; reports LPs admitted, waiting (submitted, not admitted) and finished so far. */
+int sp_server_run_chunk(sp_server *s, int *active, int *waiting, int *finished);
+int sp_server_state(sp_server *s, uint64_t id, int *state);
+/* For a finished LP: the Solution, x (n) and y (m) (either may be NULL) and times.
+ * A failed LP returns its error code. */
+int sp_server_result(sp_server *s, uint64_t id, sp_solution *out, double *x, double *y, sp_job_times *times);
+/* Drops a finished LP's stored result. */
+int sp_server_forget(sp_server *s, uint64_t id);
+double sp_server_clock(const sp_server *s);
+
 /* ScaledProblem::new (scaling.rs:53-103) on the host, for callers that load
  * themselves. Arrays are caller-allocated: a_values[nnz] (the scaled A' in
  * the input's sparsity), c/lv/uv/col_scale[n], lc/uc/row_scale[m]. */

@@ -73,6 +73,14 @@ class Solution(C.Structure):
                 ("mixed_fp64_iterations", _u64)]
 
 
+class ServerParams(C.Structure):
+    _fields_ = [("budget_bytes", _i64), ("max_active", C.c_int), ("prep_threads", C.c_int), ("use_graphs", C.c_int)]
+
+
+class JobTimes(C.Structure):
+    _fields_ = [("submitted", _d), ("admitted", _d), ("completed", _d)]
+
+
 # name -> (restype, argtypes): every function include/saddlepoint_hip.h declares.
 SIGNATURES = {
     "sp_backend_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
@@ -112,6 +120,15 @@ SIGNATURES = {
     "sp_backend_profile_read": (C.c_int, [_vp, C.c_int, C.c_char_p, C.c_int, _pd, C.POINTER(_i64)]),
     "sp_backend_sync": (C.c_int, [_vp]),
     "sp_probe": (C.c_int, [_vp, C.c_int, _i64, _i64, C.c_int, _pd]),
+    "sp_default_server_params": (None, [C.POINTER(ServerParams)]),
+    "sp_server_create": (C.c_int, [_vp, C.POINTER(ServerParams), C.POINTER(_vp)]),
+    "sp_server_free": (None, [_vp]),
+    "sp_server_submit": (C.c_int, [_vp, C.POINTER(LpView), C.POINTER(Params), C.POINTER(_u64)]),
+    "sp_server_run_chunk": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "sp_server_state": (C.c_int, [_vp, _u64, C.POINTER(C.c_int)]),
+    "sp_server_result": (C.c_int, [_vp, _u64, C.POINTER(Solution), _pd, _pd, C.POINTER(JobTimes)]),
+    "sp_server_forget": (C.c_int, [_vp, _u64]),
+    "sp_server_clock": (_d, [_vp]),
     "sp_scale_problem": (C.c_int, [C.POINTER(LpView), C.POINTER(Params), _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd,
                                    _pd, _pd]),
 }

@@ -122,6 +122,8 @@ void upload_csr(sp_backend* b, DevCsr& d, const int64_t* row_ptr, const int64_t*
     d.partials = dmalloc<double>((size_t)std::max(1, d.npartials));
     d.nlong = (int)plan.longs.size();
     d.longs = dupload(b->stream, plan.longs.data(), plan.longs.size());
+    d.host_desc = std::move(plan.desc);
+    d.host_longs = std::move(plan.longs);
     // The host staging vectors die at return: finish the copies first.
     hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
 }
@@ -539,6 +541,14 @@ const double* Hip::fetch(int count) {
     return b->host_slots;
 }
 
+const double* Hip::fetch_vec(const sp_vec* v, int count) {
+    if (!v || count < 0 || count > kSlots || count > v->len) throw Error(SP_ERR_INVALID, "fetch: bad slot count");
+    hip_check(hipMemcpyAsync(b->host_slots, v->d, sizeof(double) * (size_t)count, hipMemcpyDeviceToHost, b->stream),
+              "hipMemcpyAsync D2H");
+    hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    return b->host_slots;
+}
+
 // Raw sums -> the trait's structs (cpu/src/lib.rs:211-259).
 sp_primal_stats_t Hip::primal_from(const double* r) { return sp_primal_stats_t{r[0], r[1]}; }
 // dual_stats: objective = rows + cols (cpu/src/lib.rs:223).
@@ -695,6 +705,14 @@ sp_backend* sp_backend_open(int ordinal) {
         get(b->k_fin_t_primal_mixed, "sp_finish_t_primal_mixed");
         get(b->k_fin_dual_mixed, "sp_finish_dual_mixed");
         get(b->k_red_finish, "sp_red_finish");
+        get(b->k_bspmv_t64, "sp_bspmv_t_primal_halpern_f64");
+        get(b->k_bspmv_a64, "sp_bspmv_dual_halpern_f64");
+        get(b->k_bfin_t64, "sp_bfinish_t_primal_halpern_f64");
+        get(b->k_bfin_a64, "sp_bfinish_dual_halpern_f64");
+        get(b->k_bspmv_t_mixed, "sp_bspmv_t_primal_mixed");
+        get(b->k_bspmv_a_mixed, "sp_bspmv_dual_mixed");
+        get(b->k_bfin_t_mixed, "sp_bfinish_t_primal_mixed");
+        get(b->k_bfin_a_mixed, "sp_bfinish_dual_mixed");
         get(b->k_spmv_t_primal, "sp_spmv_t_primal_f64");
         get(b->k_spmv_t_primal_halpern, "sp_spmv_t_primal_halpern_f64");
         get(b->k_spmv_dual, "sp_spmv_dual_f64");

@@ -47,6 +47,8 @@ struct DevCsr {
     int npartials = 0;
     LongRow* longs = nullptr;    // the multi-chunk rows
     int nlong = 0;
+    std::vector<Desc> host_desc;       // the plan, kept for batch tables
+    std::vector<LongRow> host_longs;
     SpmvTuning tuning;
 };
 
@@ -114,6 +116,8 @@ struct sp_backend {
 
     hipFunction_t k_fin, k_fin_t_primal, k_fin_t_primal_halpern, k_fin_dual, k_fin_dual_halpern, k_fin32,
         k_fin_t_primal_mixed, k_fin_dual_mixed, k_red_finish;
+    hipFunction_t k_bspmv_t64, k_bspmv_a64, k_bfin_t64, k_bfin_a64, k_bspmv_t_mixed, k_bspmv_a_mixed,
+        k_bfin_t_mixed, k_bfin_a_mixed;
     hipFunction_t k_spmv, k_spmv_t_primal, k_spmv_t_primal_halpern, k_spmv_dual, k_spmv_dual_halpern;
     hipFunction_t k_primal_step, k_dual_step, k_halpern, k_sub, k_scale, k_fill;
     hipFunction_t k_probe_copy, k_probe_triad, k_probe_gather;
@@ -193,6 +197,9 @@ struct Hip {
     // Reductions into device result slots, read back together by fetch().
     double* slot(int i);
     const double* fetch(int count);
+    // Reads the first `count` values of a device vector into the pinned
+    // staging area (valid until the next fetch).
+    const double* fetch_vec(const sp_vec* v, int count);
     void dot_into(const sp_vec* a, const sp_vec* bv, double* dst);
     void distance_sq_into(const sp_vec* a, const sp_vec* bv, double* dst);
     void primal_stats_into(const sp_problem* p, const sp_vec* x, const sp_vec* ax, double* dst);      // 2 sums

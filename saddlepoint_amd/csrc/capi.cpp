@@ -7,6 +7,7 @@
 
 #include "backend.hpp"
 #include "saddlepoint_hip.h"
+#include "driver.hpp"
 #include "solver.hpp"
 
 namespace {
@@ -32,27 +33,6 @@ int guarded(sp_backend* b, F&& f) {
         return SP_ERR_INVALID;
     }
 }
-
-struct VecDel {
-    void operator()(sp_vec* v) const { sp::free_vec(v); }
-};
-struct ProbDel {
-    void operator()(sp_problem* p) const { sp::free_problem(p); }
-};
-struct Vec32Del {
-    void operator()(sp_vec32* v) const { sp::free_vec32(v); }
-};
-
-// The backend as the driver sees it: owning handles for vectors and problems.
-struct DriverBackend : sp::Hip {
-    using Vector = std::unique_ptr<sp_vec, VecDel>;
-    using Problem = std::unique_ptr<sp_problem, ProbDel>;
-    using Vector32 = std::unique_ptr<sp_vec32, Vec32Del>;
-    Problem load(const sp::ScaledProblem& s) { return Problem(Hip::load(s)); }
-    Vector32 zeros32(int64_t n) { return Vector32(Hip::zeros32(n)); }
-    Vector zeros(int64_t n) { return Vector(Hip::zeros(n)); }
-    Vector upload(const double* d, int64_t n) { return Vector(Hip::upload(d, n)); }
-};
 
 void check_problem(const sp_backend* b, const sp_problem* p) {
     if (!p) throw sp::Error(SP_ERR_INVALID, "null problem");
@@ -91,6 +71,8 @@ sp::ScalingParams scaling_of(const sp_params& p) {
 }
 
 }  // namespace
+
+using sp::DriverBackend;
 
 struct sp_solver {
     sp_backend* b;

@@ -1,0 +1,31 @@
+// The HIP backend as the PDHG driver (solver.hpp) sees it: the Backend
+// methods of sp::Hip plus owning handles for vectors and problems.
+#pragma once
+
+#include <memory>
+
+#include "backend.hpp"
+
+namespace sp {
+
+struct VecDel {
+    void operator()(sp_vec* v) const { free_vec(v); }
+};
+struct ProbDel {
+    void operator()(sp_problem* p) const { free_problem(p); }
+};
+struct Vec32Del {
+    void operator()(sp_vec32* v) const { free_vec32(v); }
+};
+
+struct DriverBackend : Hip {
+    using Vector = std::unique_ptr<sp_vec, VecDel>;
+    using Problem = std::unique_ptr<sp_problem, ProbDel>;
+    using Vector32 = std::unique_ptr<sp_vec32, Vec32Del>;
+    Problem load(const ScaledProblem& s) { return Problem(Hip::load(s)); }
+    Vector zeros(int64_t n) { return Vector(Hip::zeros(n)); }
+    Vector upload(const double* d, int64_t n) { return Vector(Hip::upload(d, n)); }
+    Vector32 zeros32(int64_t n) { return Vector32(Hip::zeros32(n)); }
+};
+
+}  // namespace sp

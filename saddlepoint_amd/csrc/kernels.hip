@@ -116,22 +116,21 @@ __device__ __forceinline__ void block_sum(A (&s)[K]) {
 // indices, values, row pointers and the epilogue's per-row operands, then the
 // gathers.
 //
-// Long rows (more than SP_NNZ nonzeros) are summed by a whole workgroup per
-// chunk; chunks of one row are combined in chunk order by the last workgroup
-// to finish.
+// Long rows are summed by a whole workgroup per chunk of SP_CHUNK nonzeros; a
+// row of several chunks publishes the chunk sums and the finishing kernel
+// adds them in chunk order.
 #define SP_RPT 4                // rows per thread in the epilogue
 #define SP_ROWS (SP_RPT * WG)   // most rows in one stream block
 
 template <int NNZ, class V, class X, class A, class Epi>
-__device__ __forceinline__ void spmv_body(const Desc* __restrict__ desc, const int* __restrict__ rp,
-                                          const int* __restrict__ ci, const V* __restrict__ va,
-                                          const X* __restrict__ x, A* __restrict__ partials, const Epi& epi) {
+__device__ __forceinline__ void spmv_block(const Desc d, const int* __restrict__ rp, const int* __restrict__ ci,
+                                           const V* __restrict__ va, const X* __restrict__ x,
+                                           A* __restrict__ partials, const Epi& epi) {
     __shared__ A prod[NNZ];
     __shared__ A sums[SP_ROWS];
     __shared__ int rend[SP_ROWS];
     __shared__ int crow[WG];
     __shared__ A cval[WG];
-    const Desc d = desc[blockIdx.x];
     const int tid = threadIdx.x;
     if (d.r1 >= 0) {
         const int r0 = d.r0, nr = d.r1 - d.r0, k0 = d.k0, n = d.k1 - d.k0;
@@ -249,6 +248,13 @@ __device__ __forceinline__ void spmv_body(const Desc* __restrict__ desc, const i
     }
 }
 
+template <int NNZ, class V, class X, class A, class Epi>
+__device__ __forceinline__ void spmv_body(const Desc* __restrict__ desc, const int* __restrict__ rp,
+                                          const int* __restrict__ ci, const V* __restrict__ va,
+                                          const X* __restrict__ x, A* __restrict__ partials, const Epi& epi) {
+    spmv_block<NNZ, V, X, A, Epi>(desc[blockIdx.x], rp, ci, va, x, partials, epi);
+}
+
 // The finishing pass for rows of several chunks: one wave per row adds the
 // row's chunk sums in a fixed order and applies the epilogue. (Combining in
 // the SpMV itself would need an agent-scope release fence per workgroup, and
@@ -259,14 +265,20 @@ struct LongRow {
 
 template <class A, class Epi>
 __device__ __forceinline__ void finish_body(const LongRow* __restrict__ longs, int nlong,
-                                            const A* __restrict__ partials, const Epi& epi) {
-    const int w = (int)((blockIdx.x * WG + threadIdx.x) / WAVE), lane = threadIdx.x % WAVE;
+                                            const A* __restrict__ partials, const Epi& epi, int w) {
+    const int lane = threadIdx.x % WAVE;
     if (w >= nlong) return;  // uniform per wave
     const LongRow L = longs[w];
     A s = 0;
     for (int q = lane; q < L.nch; q += WAVE) s += partials[L.pbase + q];
     s = wave_sum(s);
     if (lane == 0) epi.apply(L.row, s, epi.load(L.row));
+}
+
+template <class A, class Epi>
+__device__ __forceinline__ void finish_body(const LongRow* __restrict__ longs, int nlong,
+                                            const A* __restrict__ partials, const Epi& epi) {
+    finish_body<A>(longs, nlong, partials, epi, (int)((blockIdx.x * WG + threadIdx.x) / WAVE));
 }
 
 // Epilogues: load(row) fetches the row's operands (issued early), apply(row,
@@ -795,4 +807,134 @@ extern "C" __global__ __launch_bounds__(WG) void sp_probe_gather(const int* __re
                                                                  const double* __restrict__ table,
                                                                  double* __restrict__ out, i64 n) {
     GRID_LOOP(i, n) { out[i] = table[__builtin_nontemporal_load(idx + i)]; }
+}
+
+// ---- batched plain iterations for the serving loop --------------------------------
+//
+// One launch runs the plain iteration of every LP in the batch: the work items
+// of all LPs' own SpMV plans are concatenated, each tagged with its LP, and a
+// table holds each LP's arrays and step parameters. Every row is computed by
+// the same code, in the same order, as when its LP runs alone. The Halpern
+// weight w = (k+1)/(k+2) is formed on the device from the LP's iteration
+// counter at the chunk start plus the step index, so a captured graph of the
+// steps stays valid from chunk to chunk.
+
+struct BInst {
+    // A' (rows = m) and A'ᵀ (rows = n): row pointers, column indices, FP64 and
+    // FP32 values, long-row chunk sums.
+    const int *a_rp, *a_ci;
+    const double* a_va;
+    const float* a_va32;
+    double* a_part;
+    const int *t_rp, *t_ci;
+    const double* t_va;
+    const float* t_va32;
+    double* t_part;
+    const double *c, *lc, *uc, *lv, *uv;
+    double *x, *y, *x_bar;
+    const double *x_anchor, *y_anchor;
+    float *dx, *dy, *dx_bar;       // mixed precision displacements
+    const double *atya, *axa;      // mixed precision anchor products
+    double tau, sigma, rho;
+    unsigned long long k0;         // Halpern counter k at step 0 of the chunk
+};
+
+struct BDesc {
+    Desc d;
+    int inst, pad0, pad1, pad2;
+};
+
+struct BLong {
+    int inst, row, pbase, nch;
+};
+
+__device__ __forceinline__ double halpern_w(unsigned long long k0, int it) {
+    const unsigned long long k = k0 + (unsigned long long)it;
+    return (double)(k + 1) / (double)(k + 2);
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_bspmv_t_primal_halpern_f64(const BDesc* __restrict__ desc,
+                                                                               const BInst* __restrict__ insts,
+                                                                               int it) {
+    const BDesc bd = desc[blockIdx.x];
+    const BInst& I = insts[bd.inst];
+    spmv_block<SP_NNZ_F64, double, double, double>(
+        bd.d, I.t_rp, I.t_ci, I.t_va, (const double*)I.y, I.t_part,
+        EpiPrimalHalpern{I.x, I.c, I.lv, I.uv, I.x_anchor, I.tau, halpern_w(I.k0, it), I.rho, I.x_bar});
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_bspmv_dual_halpern_f64(const BDesc* __restrict__ desc,
+                                                                           const BInst* __restrict__ insts, int it) {
+    const BDesc bd = desc[blockIdx.x];
+    const BInst& I = insts[bd.inst];
+    spmv_block<SP_NNZ_F64, double, double, double>(
+        bd.d, I.a_rp, I.a_ci, I.a_va, (const double*)I.x_bar, I.a_part,
+        EpiDualHalpern{I.y, I.lc, I.uc, I.y_anchor, I.sigma, halpern_w(I.k0, it), I.rho});
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_bfinish_t_primal_halpern_f64(const BLong* __restrict__ longs,
+                                                                                 int nlong,
+                                                                                 const BInst* __restrict__ insts,
+                                                                                 int it) {
+    const int w = (int)((blockIdx.x * WG + threadIdx.x) / WAVE);
+    if (w >= nlong) return;
+    const BLong L = longs[w];
+    const BInst& I = insts[L.inst];
+    const LongRow lr{L.row, L.pbase, L.nch, 0};
+    finish_body<double>(&lr, 1, I.t_part,
+                        EpiPrimalHalpern{I.x, I.c, I.lv, I.uv, I.x_anchor, I.tau, halpern_w(I.k0, it), I.rho, I.x_bar},
+                        0);
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_bfinish_dual_halpern_f64(const BLong* __restrict__ longs, int nlong,
+                                                                             const BInst* __restrict__ insts, int it) {
+    const int w = (int)((blockIdx.x * WG + threadIdx.x) / WAVE);
+    if (w >= nlong) return;
+    const BLong L = longs[w];
+    const BInst& I = insts[L.inst];
+    const LongRow lr{L.row, L.pbase, L.nch, 0};
+    finish_body<double>(&lr, 1, I.a_part, EpiDualHalpern{I.y, I.lc, I.uc, I.y_anchor, I.sigma, halpern_w(I.k0, it), I.rho},
+                        0);
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_bspmv_t_primal_mixed(const BDesc* __restrict__ desc,
+                                                                         const BInst* __restrict__ insts, int it) {
+    const BDesc bd = desc[blockIdx.x];
+    const BInst& I = insts[bd.inst];
+    spmv_block<SP_NNZ_F64, float, float, float>(
+        bd.d, I.t_rp, I.t_ci, I.t_va32, (const float*)I.dy, (float*)I.t_part,
+        EpiPrimalMixed{I.dx, I.x_anchor, I.c, I.lv, I.uv, I.atya, I.tau, halpern_w(I.k0, it), I.rho, I.dx_bar});
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_bspmv_dual_mixed(const BDesc* __restrict__ desc,
+                                                                     const BInst* __restrict__ insts, int it) {
+    const BDesc bd = desc[blockIdx.x];
+    const BInst& I = insts[bd.inst];
+    spmv_block<SP_NNZ_F64, float, float, float>(
+        bd.d, I.a_rp, I.a_ci, I.a_va32, (const float*)I.dx_bar, (float*)I.a_part,
+        EpiDualMixed{I.dy, I.y_anchor, I.lc, I.uc, I.axa, I.sigma, halpern_w(I.k0, it), I.rho});
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_bfinish_t_primal_mixed(const BLong* __restrict__ longs, int nlong,
+                                                                           const BInst* __restrict__ insts, int it) {
+    const int w = (int)((blockIdx.x * WG + threadIdx.x) / WAVE);
+    if (w >= nlong) return;
+    const BLong L = longs[w];
+    const BInst& I = insts[L.inst];
+    const LongRow lr{L.row, L.pbase, L.nch, 0};
+    finish_body<float>(&lr, 1, (const float*)I.t_part,
+                       EpiPrimalMixed{I.dx, I.x_anchor, I.c, I.lv, I.uv, I.atya, I.tau, halpern_w(I.k0, it), I.rho,
+                                      I.dx_bar},
+                       0);
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_bfinish_dual_mixed(const BLong* __restrict__ longs, int nlong,
+                                                                       const BInst* __restrict__ insts, int it) {
+    const int w = (int)((blockIdx.x * WG + threadIdx.x) / WAVE);
+    if (w >= nlong) return;
+    const BLong L = longs[w];
+    const BInst& I = insts[L.inst];
+    const LongRow lr{L.row, L.pbase, L.nch, 0};
+    finish_body<float>(&lr, 1, (const float*)I.a_part,
+                       EpiDualMixed{I.dy, I.y_anchor, I.lc, I.uc, I.axa, I.sigma, halpern_w(I.k0, it), I.rho}, 0);
 }

@@ -1,0 +1,648 @@
+// Continuous-batching serving loop on one GPU.
+//
+// LPs are submitted at any time. Host worker threads validate and scale them
+// (ScaledProblem::new on the host); the loop admits prepared LPs while the
+// HBM budget and the slot count allow, keeps every admitted LP's iterates,
+// anchors, ω, step sizes and restart state on the device, and retires LPs as
+// they terminate.
+//
+// The loop runs in chunks of 64 iterations aligned to every LP's own check
+// cadence (an LP is admitted at a chunk boundary with its iteration count at
+// 0, so its checks fall on the chunk's last step). In a chunk:
+//   step 1      LPs that start a restart epoch (k = 0) take their iteration
+//               alone (it measures the anchor's fixed-point residual); the
+//               others take it together, as a batched plain iteration;
+//   steps 2-63  one batched plain iteration for all LPs per step: one
+//               launch per side, over the concatenated work items of every
+//               LP's own SpMV plan (optionally replayed from a hipGraph);
+//   step 64     each LP's check (KKT, infeasibility, restart, PID) in FP64,
+//               through the same code as a single solve.
+// Because each LP keeps its own plan and its own per-LP reductions, every
+// LP's iterates are bitwise the same alone and in any mix.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+
+#include "driver.hpp"
+#include "saddlepoint_hip.h"
+#include "solver.hpp"
+
+namespace sp {
+
+namespace {
+
+// Host mirrors of BInst / BDesc / BLong in kernels.hip (same layout).
+struct BInst {
+    const int *a_rp, *a_ci;
+    const double* a_va;
+    const float* a_va32;
+    double* a_part;
+    const int *t_rp, *t_ci;
+    const double* t_va;
+    const float* t_va32;
+    double* t_part;
+    const double *c, *lc, *uc, *lv, *uv;
+    double *x, *y, *x_bar;
+    const double *x_anchor, *y_anchor;
+    float *dx, *dy, *dx_bar;
+    const double *atya, *axa;
+    double tau, sigma, rho;
+    unsigned long long k0;
+};
+struct BDesc {
+    Desc d;
+    int inst, pad0, pad1, pad2;
+};
+struct BLong {
+    int inst, row, pbase, nch;
+};
+static_assert(sizeof(BDesc) == 32, "BDesc layout");
+static_assert(sizeof(BLong) == 16, "BLong layout");
+
+using Clock = std::chrono::steady_clock;
+
+constexpr uint64_t kChunk = 64;
+
+template <class T>
+T* dalloc(size_t n) {
+    void* p = nullptr;
+    hip_check(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)), "hipMalloc");
+    return static_cast<T*>(p);
+}
+
+// A growable device buffer.
+template <class T>
+struct DevBuf {
+    T* d = nullptr;
+    size_t cap = 0;
+    void ensure(size_t n) {
+        if (n <= cap) return;
+        if (d) (void)hipFree(d);
+        cap = std::max(n, cap * 2);
+        d = dalloc<T>(cap);
+    }
+    ~DevBuf() {
+        if (d) (void)hipFree(d);
+    }
+};
+
+}  // namespace
+
+enum JobState { kQueued = 0, kPrepared = 1, kActive = 2, kDone = 3, kFailed = 4 };
+
+struct Job {
+    uint64_t id = 0;
+    int64_t m = 0, n = 0;
+    std::vector<int64_t> rp, ci;
+    std::vector<double> va, c, lc, uc, lv, uv;
+    double c0 = 0.0;
+    bool maximize = false;
+    sp_params params{};
+    double t_submit = 0, t_admit = -1, t_done = -1;
+    int state = kQueued;
+    std::unique_ptr<ScaledProblem> scaled;
+    std::unique_ptr<Solver<DriverBackend>> solver;
+    int64_t footprint = 0;
+    sp_solution sol{};
+    std::vector<double> x, y;
+    int err = SP_OK;
+    std::string msg;
+    bool failed_now = false;  // an error in this chunk; retired at its end
+
+    LpView view() const {
+        LpView v;
+        v.m = m;
+        v.n = n;
+        v.nnz = (int64_t)va.size();
+        v.row_ptr = rp.data();
+        v.col_idx = ci.data();
+        v.values = va.data();
+        v.c = c.data();
+        v.lc = lc.data();
+        v.uc = uc.data();
+        v.lv = lv.data();
+        v.uv = uv.data();
+        v.c0 = c0;
+        v.maximize = maximize;
+        return v;
+    }
+};
+
+// Device bytes one admitted LP holds (the admission test): both matrices with
+// int32 indices and FP64 values (+ FP32 values in mixed precision), the
+// solver's 13 vectors, the problem's 7 vectors and the mixed-precision
+// displacements and anchor products.
+static int64_t footprint(int64_t m, int64_t n, int64_t nnz, bool mixed) {
+    int64_t bytes = 2 * (12 * nnz + 4 * (m + 1)) + 8 * (7 * n + 6 * m) + 8 * (4 * n + 3 * m);
+    bytes += 2 * 32 * (nnz / 1024 + 64);  // plans
+    if (mixed) bytes += 2 * 4 * nnz + 4 * (2 * n + m) + 8 * (n + m);
+    return bytes;
+}
+
+class Server {
+   public:
+    Server(sp_backend* b, const sp_server_params& sp) : b_(b), db_{{b}}, sp_(sp), t0_(Clock::now()) {
+        const int workers = std::max(1, sp_.prep_threads);
+        for (int i = 0; i < workers; ++i) workers_.emplace_back([this] { prep_loop(); });
+    }
+
+    ~Server() {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : workers_) t.join();
+        (void)hipStreamSynchronize(b_->stream);
+        for (auto& kv : jobs_) kv.second->solver.reset();
+        if (h_inst_) (void)hipHostFree(h_inst_);
+    }
+
+    double clock() const { return std::chrono::duration<double>(Clock::now() - t0_).count(); }
+
+    uint64_t submit(const sp_lp_view& lp, const sp_params& params) {
+        auto job = std::make_unique<Job>();
+        job->m = lp.a.nrows;
+        job->n = lp.a.ncols;
+        const int64_t nnz = lp.a.nnz;
+        if (job->m < 0 || job->n < 0 || nnz < 0 || !lp.a.row_ptr) throw Error(SP_ERR_INVALID, "submit: bad shape");
+        job->rp.assign(lp.a.row_ptr, lp.a.row_ptr + job->m + 1);
+        job->ci.assign(lp.a.col_idx, lp.a.col_idx + nnz);
+        job->va.assign(lp.a.values, lp.a.values + nnz);
+        job->c.assign(lp.c, lp.c + job->n);
+        job->lv.assign(lp.lv, lp.lv + job->n);
+        job->uv.assign(lp.uv, lp.uv + job->n);
+        job->lc.assign(lp.lc, lp.lc + job->m);
+        job->uc.assign(lp.uc, lp.uc + job->m);
+        job->c0 = lp.c0;
+        job->maximize = lp.maximize != 0;
+        job->params = params;
+        // The loop's chunks follow the reference's check cadence of 64; the
+        // iteration limit is rounded up to a whole number of chunks.
+        if (params.check_every != kChunk) throw Error(SP_ERR_INVALID, "serving needs check_every = 64");
+        job->params.fused = 1;
+        job->params.max_iterations = (params.max_iterations + kChunk - 1) / kChunk * kChunk;
+        job->footprint = footprint(job->m, job->n, nnz, params.precision == 1);
+        job->t_submit = clock();
+        std::lock_guard<std::mutex> g(mu_);
+        job->id = ++next_id_;
+        const uint64_t id = job->id;
+        to_prep_.push_back(job.get());
+        jobs_[id] = std::move(job);
+        cv_.notify_one();
+        return id;
+    }
+
+    // Runs one 64-iteration chunk for every admitted LP (admitting and
+    // retiring at the chunk's edges).
+    void run_chunk(int* active, int* waiting, int* done_total) {
+        admit();
+        if (!active_.empty()) chunk();
+        retire();
+        if (active) *active = (int)active_.size();
+        std::lock_guard<std::mutex> g(mu_);
+        if (waiting) *waiting = (int)(to_prep_.size() + ready_.size());
+        if (done_total) *done_total = (int)finished_;
+    }
+
+    int state(uint64_t id) {
+        std::lock_guard<std::mutex> g(mu_);
+        auto it = jobs_.find(id);
+        if (it == jobs_.end()) throw Error(SP_ERR_INVALID, "no such job");
+        return it->second->state;
+    }
+
+    void result(uint64_t id, sp_solution* out, double* x, double* y, sp_job_times* t) {
+        std::lock_guard<std::mutex> g(mu_);
+        auto it = jobs_.find(id);
+        if (it == jobs_.end()) throw Error(SP_ERR_INVALID, "no such job");
+        Job& j = *it->second;
+        if (j.state == kFailed) throw Error(j.err, j.msg);
+        if (j.state != kDone) throw Error(SP_ERR_INVALID, "job not finished");
+        if (out) *out = j.sol;
+        if (x) std::copy(j.x.begin(), j.x.end(), x);
+        if (y) std::copy(j.y.begin(), j.y.end(), y);
+        if (t) *t = sp_job_times{j.t_submit, j.t_admit, j.t_done};
+    }
+
+    void forget(uint64_t id) {
+        std::lock_guard<std::mutex> g(mu_);
+        auto it = jobs_.find(id);
+        if (it != jobs_.end() && (it->second->state == kDone || it->second->state == kFailed)) jobs_.erase(it);
+    }
+
+   private:
+    void prep_loop() {
+        for (;;) {
+            Job* j;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || !to_prep_.empty(); });
+                if (stop_) return;
+                j = to_prep_.front();
+                to_prep_.pop_front();
+            }
+            try {
+                const LpView v = j->view();
+                validate(v);
+                ScalingParams s;
+                s.geometric_mean_passes = j->params.geometric_mean_passes;
+                s.ruiz_iterations = j->params.ruiz_iterations;
+                s.has_pock_chambolle = j->params.has_pock_chambolle != 0;
+                s.pock_chambolle_alpha = j->params.pock_chambolle_alpha;
+                s.bound_objective_rescaling = j->params.bound_objective_rescaling != 0;
+                j->scaled = std::make_unique<ScaledProblem>(ScaledProblem::build(v, s));
+                std::lock_guard<std::mutex> g(mu_);
+                j->state = kPrepared;
+                ready_.push_back(j);
+            } catch (const Error& e) {
+                fail(j, e.code, e.what());
+            } catch (const std::exception& e) {
+                fail(j, SP_ERR_INVALID, e.what());
+            }
+        }
+    }
+
+    void fail(Job* j, int code, const std::string& msg) {
+        std::lock_guard<std::mutex> g(mu_);
+        j->err = code;
+        j->msg = msg;
+        j->state = kFailed;
+        j->t_done = clock();
+        ++finished_;
+    }
+
+    void admit() {
+        for (;;) {
+            Job* j = nullptr;
+            {
+                std::lock_guard<std::mutex> g(mu_);
+                if (ready_.empty()) return;
+                Job* f = ready_.front();
+                const bool fits = used_ + f->footprint <= sp_.budget_bytes || active_.empty();
+                if ((int)active_.size() >= sp_.max_active || !fits) return;
+                ready_.pop_front();
+                j = f;
+            }
+            try {
+                j->solver = std::make_unique<Solver<DriverBackend>>(db_, j->view(), j->params, std::move(*j->scaled));
+                j->scaled.reset();
+                j->solver->start();
+                j->t_admit = clock();
+                {
+                    std::lock_guard<std::mutex> g(mu_);
+                    j->state = kActive;
+                }
+                used_ += j->footprint;
+                active_.push_back(j);
+            } catch (const Error& e) {
+                j->solver.reset();
+                fail(j, e.code, e.what());
+            }
+        }
+    }
+
+    // Runs one LP's own iteration (the check path of a single solve).
+    void own_step(Job* j) {
+        try {
+            j->solver->advance(1);
+        } catch (const Error& e) {
+            j->err = e.code;
+            j->msg = e.what();
+            j->failed_now = true;
+        }
+    }
+
+    void chunk() {
+        // Step 1.
+        std::vector<Job*> batch1;
+        for (Job* j : active_) {
+            if (j->solver->plain_run() == 0) own_step(j);
+            else batch1.push_back(j);
+        }
+        if (!batch1.empty()) run_batched(batch1, 1, false);
+        // Steps 2-63: everyone that is still running and in step.
+        std::vector<Job*> all;
+        uint64_t steps = UINT64_MAX;
+        for (Job* j : active_) {
+            if (j->failed_now || j->solver->terminated()) continue;
+            all.push_back(j);
+            steps = std::min(steps, j->solver->plain_run());
+        }
+        if (!all.empty() && steps > 0 && steps != UINT64_MAX) run_batched(all, (int)steps, sp_.use_graphs != 0);
+        // Step 64 (and anything else that is due): each LP's own check.
+        for (Job* j : active_) {
+            while (!j->failed_now && !j->solver->terminated() && j->solver->plain_run() == 0) {
+                const uint64_t before = j->solver->iterations();
+                own_step(j);
+                if (j->solver->iterations() % kChunk == 0 || j->solver->iterations() == before) break;
+            }
+        }
+    }
+
+    // `steps` batched plain iterations of `group`.
+    void run_batched(const std::vector<Job*>& group, int steps, bool graph) {
+        std::vector<Job*> g64, g32;
+        for (Job* j : group) (j->solver->mixed_now() ? g32 : g64).push_back(j);
+        for (int mixed = 0; mixed < 2; ++mixed) {
+            const std::vector<Job*>& g = mixed ? g32 : g64;
+            if (g.empty()) continue;
+            Batch& bt = mixed ? batch32_ : batch64_;
+            build(bt, g, mixed != 0);
+            const bool use_graph = graph && steps > 1;
+            launch_steps(bt, steps, mixed != 0, use_graph);
+            for (Job* j : g) j->solver->plain_done((uint64_t)steps);
+        }
+    }
+
+    struct Batch {
+        std::vector<Job*> members;
+        DevBuf<BDesc> desc_t, desc_a;
+        DevBuf<BLong> long_t, long_a;
+        DevBuf<BInst> inst;
+        int nt = 0, na = 0, nlt = 0, nla = 0;
+        hipGraphExec_t exec = nullptr;
+        int graph_steps = 0;
+        ~Batch() {
+            if (exec) (void)hipGraphExecDestroy(exec);
+        }
+    };
+
+    void build(Batch& bt, const std::vector<Job*>& g, bool mixed) {
+        for (Job* j : g) j->solver->enter_plain();
+        const bool same = bt.members == g;
+        if (!same) {
+            std::vector<BDesc> dt, da;
+            std::vector<BLong> lt, la;
+            for (size_t i = 0; i < g.size(); ++i) {
+                const sp_problem* p = g[i]->solver->plain_state().problem;
+                for (const Desc& d : p->at.host_desc) dt.push_back(BDesc{d, (int)i, 0, 0, 0});
+                for (const Desc& d : p->a.host_desc) da.push_back(BDesc{d, (int)i, 0, 0, 0});
+                for (const LongRow& l : p->at.host_longs) lt.push_back(BLong{(int)i, l.row, l.pbase, l.nch});
+                for (const LongRow& l : p->a.host_longs) la.push_back(BLong{(int)i, l.row, l.pbase, l.nch});
+            }
+            auto up = [&](auto& buf, const auto& v) {
+                buf.ensure(v.size());
+                if (!v.empty())
+                    hip_check(hipMemcpyAsync(buf.d, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice,
+                                             b_->stream),
+                              "hipMemcpyAsync H2D");
+            };
+            up(bt.desc_t, dt);
+            up(bt.desc_a, da);
+            up(bt.long_t, lt);
+            up(bt.long_a, la);
+            bt.nt = (int)dt.size();
+            bt.na = (int)da.size();
+            bt.nlt = (int)lt.size();
+            bt.nla = (int)la.size();
+            bt.members = g;
+            if (bt.exec) {
+                (void)hipGraphExecDestroy(bt.exec);
+                bt.exec = nullptr;
+            }
+            hip_check(hipStreamSynchronize(b_->stream), "hipStreamSynchronize");  // host vectors die here
+        }
+        // The per-LP table: arrays and this chunk's step parameters.
+        bt.inst.ensure(g.size());
+        if (host_inst_cap_ < g.size()) {
+            if (h_inst_) (void)hipHostFree(h_inst_);
+            host_inst_cap_ = std::max<size_t>(g.size(), 2 * host_inst_cap_);
+            hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_inst_), host_inst_cap_ * sizeof(BInst),
+                                    hipHostMallocDefault),
+                      "hipHostMalloc");
+        }
+        // The pinned staging area is reused: wait for the previous upload.
+        hip_check(hipStreamSynchronize(b_->stream), "hipStreamSynchronize");
+        for (size_t i = 0; i < g.size(); ++i) {
+            const auto s = g[i]->solver->plain_state();
+            const sp_problem* p = s.problem;
+            BInst& I = h_inst_[i];
+            std::memset(&I, 0, sizeof I);
+            I.a_rp = p->a.rp;
+            I.a_ci = p->a.ci;
+            I.a_va = p->a.va;
+            I.a_va32 = p->a.va32;
+            I.a_part = p->a.partials;
+            I.t_rp = p->at.rp;
+            I.t_ci = p->at.ci;
+            I.t_va = p->at.va;
+            I.t_va32 = p->at.va32;
+            I.t_part = p->at.partials;
+            I.c = p->c;
+            I.lc = p->lc;
+            I.uc = p->uc;
+            I.lv = p->lv;
+            I.uv = p->uv;
+            I.x = s.x->d;
+            I.y = s.y->d;
+            I.x_bar = s.x_bar->d;
+            I.x_anchor = s.x_anchor->d;
+            I.y_anchor = s.y_anchor->d;
+            if (mixed) {
+                I.dx = s.dx->d;
+                I.dy = s.dy->d;
+                I.dx_bar = s.dx_bar->d;
+                I.atya = s.atya->d;
+                I.axa = s.axa->d;
+            }
+            I.tau = s.tau;
+            I.sigma = s.sigma;
+            I.rho = s.rho;
+            I.k0 = s.k;
+        }
+        hip_check(hipMemcpyAsync(bt.inst.d, h_inst_, g.size() * sizeof(BInst), hipMemcpyHostToDevice, b_->stream),
+                  "hipMemcpyAsync H2D");
+    }
+
+    template <class... Args>
+    void kl(hipFunction_t f, unsigned grid, Args... args) {
+        void* params[] = {static_cast<void*>(&args)...};
+        hip_check(hipModuleLaunchKernel(f, grid, 1, 1, 256, 1, 1, 0, b_->stream, params, nullptr),
+                  "hipModuleLaunchKernel");
+    }
+
+    void enqueue_steps(const Batch& bt, int steps, bool mixed) {
+        hipFunction_t kt = mixed ? b_->k_bspmv_t_mixed : b_->k_bspmv_t64;
+        hipFunction_t ka = mixed ? b_->k_bspmv_a_mixed : b_->k_bspmv_a64;
+        hipFunction_t ft = mixed ? b_->k_bfin_t_mixed : b_->k_bfin_t64;
+        hipFunction_t fa = mixed ? b_->k_bfin_a_mixed : b_->k_bfin_a64;
+        for (int it = 0; it < steps; ++it) {
+            if (bt.nt) kl(kt, (unsigned)bt.nt, (const BDesc*)bt.desc_t.d, (const BInst*)bt.inst.d, it);
+            if (bt.nlt) kl(ft, (unsigned)((bt.nlt + 3) / 4), (const BLong*)bt.long_t.d, bt.nlt, (const BInst*)bt.inst.d, it);
+            if (bt.na) kl(ka, (unsigned)bt.na, (const BDesc*)bt.desc_a.d, (const BInst*)bt.inst.d, it);
+            if (bt.nla) kl(fa, (unsigned)((bt.nla + 3) / 4), (const BLong*)bt.long_a.d, bt.nla, (const BInst*)bt.inst.d, it);
+        }
+    }
+
+    void launch_steps(Batch& bt, int steps, bool mixed, bool graph) {
+        if (!graph) {
+            enqueue_steps(bt, steps, mixed);
+            return;
+        }
+        if (!bt.exec || bt.graph_steps != steps) {
+            if (bt.exec) {
+                (void)hipGraphExecDestroy(bt.exec);
+                bt.exec = nullptr;
+            }
+            hipGraph_t graph_ = nullptr;
+            hip_check(hipStreamBeginCapture(b_->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            try {
+                enqueue_steps(bt, steps, mixed);
+            } catch (...) {
+                (void)hipStreamEndCapture(b_->stream, &graph_);
+                if (graph_) (void)hipGraphDestroy(graph_);
+                throw;
+            }
+            hip_check(hipStreamEndCapture(b_->stream, &graph_), "hipStreamEndCapture");
+            hipError_t e = hipGraphInstantiate(&bt.exec, graph_, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph_);
+            hip_check(e, "hipGraphInstantiate");
+            bt.graph_steps = steps;
+        }
+        hip_check(hipGraphLaunch(bt.exec, b_->stream), "hipGraphLaunch");
+    }
+
+    void retire() {
+        std::vector<Job*> keep;
+        for (Job* j : active_) {
+            if (!j->failed_now && !j->solver->terminated()) {
+                keep.push_back(j);
+                continue;
+            }
+            if (j->failed_now) {
+                j->solver.reset();
+                fail(j, j->err, j->msg);
+            } else {
+                j->x.resize((size_t)j->n);
+                j->y.resize((size_t)j->m);
+                try {
+                    j->sol = j->solver->finish(j->x.data(), j->y.data());
+                    j->sol.setup_time_s = j->t_admit - j->t_submit;
+                    j->solver.reset();
+                    std::lock_guard<std::mutex> g(mu_);
+                    j->t_done = clock();
+                    j->sol.total_time_s = j->t_done - j->t_submit;
+                    j->state = kDone;
+                    ++finished_;
+                } catch (const Error& e) {
+                    j->solver.reset();
+                    fail(j, e.code, e.what());
+                }
+            }
+            used_ -= j->footprint;
+        }
+        active_.swap(keep);
+    }
+
+    sp_backend* b_;
+    DriverBackend db_;
+    sp_server_params sp_;
+    Clock::time_point t0_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    bool stop_ = false;
+    std::deque<Job*> to_prep_, ready_;
+    std::vector<std::thread> workers_;
+    std::map<uint64_t, std::unique_ptr<Job>> jobs_;
+    uint64_t next_id_ = 0;
+    uint64_t finished_ = 0;
+    std::vector<Job*> active_;
+    int64_t used_ = 0;
+    Batch batch64_, batch32_;
+    BInst* h_inst_ = nullptr;
+    size_t host_inst_cap_ = 0;
+};
+
+}  // namespace sp
+
+struct sp_server {
+    sp_backend* b;
+    std::unique_ptr<sp::Server> s;
+};
+
+namespace {
+
+template <class F>
+int guarded_server(sp_server* sv, F&& f) {
+    if (!sv) return SP_ERR_INVALID;
+    try {
+        f();
+        sv->b->err.clear();
+        return SP_OK;
+    } catch (const sp::Error& e) {
+        sv->b->err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        sv->b->err = "host out of memory";
+        return SP_ERR_MEMORY;
+    } catch (const std::exception& e) {
+        sv->b->err = e.what();
+        return SP_ERR_INVALID;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void sp_default_server_params(sp_server_params* p) {
+    if (!p) return;
+    p->budget_bytes = (int64_t)64 << 30;
+    p->max_active = 64;
+    p->prep_threads = 8;
+    p->use_graphs = 1;
+}
+
+int sp_server_create(sp_backend* b, const sp_server_params* params, sp_server** out) {
+    if (!b || !out) return SP_ERR_INVALID;
+    *out = nullptr;
+    sp_server_params p;
+    sp_default_server_params(&p);
+    if (params) p = *params;
+    try {
+        auto* sv = new sp_server{b, std::make_unique<sp::Server>(b, p)};
+        *out = sv;
+        return SP_OK;
+    } catch (const std::exception& e) {
+        b->err = e.what();
+        return SP_ERR_INVALID;
+    }
+}
+
+void sp_server_free(sp_server* s) { delete s; }
+
+int sp_server_submit(sp_server* s, const sp_lp_view* lp, const sp_params* params, uint64_t* id) {
+    return guarded_server(s, [&] {
+        if (!lp || !params || !id) throw sp::Error(SP_ERR_INVALID, "submit: null argument");
+        *id = s->s->submit(*lp, *params);
+    });
+}
+
+int sp_server_run_chunk(sp_server* s, int* active, int* waiting, int* finished) {
+    return guarded_server(s, [&] { s->s->run_chunk(active, waiting, finished); });
+}
+
+int sp_server_state(sp_server* s, uint64_t id, int* state) {
+    return guarded_server(s, [&] {
+        if (!state) throw sp::Error(SP_ERR_INVALID, "state: null output");
+        *state = s->s->state(id);
+    });
+}
+
+int sp_server_result(sp_server* s, uint64_t id, sp_solution* out, double* x, double* y, sp_job_times* times) {
+    return guarded_server(s, [&] { s->s->result(id, out, x, y, times); });
+}
+
+int sp_server_forget(sp_server* s, uint64_t id) {
+    return guarded_server(s, [&] { s->s->forget(id); });
+}
+
+double sp_server_clock(const sp_server* s) { return s ? s->s->clock() : 0.0; }
+
+}  // extern "C"

@@ -85,6 +85,7 @@ class Solver {
         x_anchor_ = b_.zeros(n), y_anchor_ = b_.zeros(m), ax_ = b_.zeros(m), aty_ = b_.zeros(n);
         tmp_n_ = b_.zeros(n), tmp_m_ = b_.zeros(m), tmp_m2_ = b_.zeros(m);
         c_ = b_.upload(scaled_.c.data(), n);
+        slots_ = b_.zeros(kSlotCount);
         if (params.precision == 1) {
             if (!params.fused) throw Error(SP_ERR_INVALID, "mixed precision needs the fused path");
             b_.ensure_fp32(dev_.get());
@@ -122,18 +123,8 @@ class Solver {
             if (p.fused && !check && k_ != 0) {
                 // Nothing of this iteration is observed but x, y and x_bar.
                 const double w = (double)(k_ + 1) / (double)(k_ + 2);
-                if (p.precision == 1 && !fp64_fallback_) {
-                    if (!shadow_live_) {
-                        if (anchor_epoch_ != restarts_) {
-                            // The anchors moved: refresh their FP64 products.
-                            b_.spmv_t(dev_.get(), y_anchor_.get(), atya_.get());
-                            b_.spmv(dev_.get(), x_anchor_.get(), axa_.get());
-                            anchor_epoch_ = restarts_;
-                        }
-                        b_.delta_from(x_.get(), x_anchor_.get(), dx32_.get());
-                        b_.delta_from(y_.get(), y_anchor_.get(), dy32_.get());
-                        shadow_live_ = true;
-                    }
+                if (mixed_now()) {
+                    enter_plain();
                     b_.plain_iteration_mixed(dev_.get(), dx32_.get(), dy32_.get(), x_anchor_.get(), y_anchor_.get(),
                                              atya_.get(), axa_.get(), dx_bar32_.get(), tau, sigma, w, p.reflection);
                 } else {
@@ -212,6 +203,59 @@ class Solver {
     uint64_t iterations() const { return total_; }
     bool terminated() const { return terminated_; }
 
+    // ---- for a caller that runs the plain iterations itself (the serving loop) ----
+
+    // Plain iterations (no check, k != 0) that can run from here before the
+    // next iteration that is not one; 0 when that next one is due now.
+    uint64_t plain_run() const {
+        if (terminated_ || !p_.fused || k_ == 0) return 0;
+        const uint64_t ce = p_.check_every;
+        uint64_t next = (total_ / ce + 1) * ce;
+        next = std::min(next, std::max(p_.max_iterations, total_ + 1));
+        return next - 1 - total_;
+    }
+
+    // Whether plain iterations run in mixed precision now.
+    bool mixed_now() const { return p_.precision == 1 && !fp64_fallback_; }
+
+    // Brings the iterates into the form plain iterations use: in mixed
+    // precision, the FP32 displacements from the anchors (and the anchors'
+    // FP64 products after a restart).
+    void enter_plain() {
+        if (!mixed_now() || shadow_live_) return;
+        if (anchor_epoch_ != restarts_) {
+            // The anchors moved: refresh their FP64 products.
+            b_.spmv_t(dev_.get(), y_anchor_.get(), atya_.get());
+            b_.spmv(dev_.get(), x_anchor_.get(), axa_.get());
+            anchor_epoch_ = restarts_;
+        }
+        b_.delta_from(x_.get(), x_anchor_.get(), dx32_.get());
+        b_.delta_from(y_.get(), y_anchor_.get(), dy32_.get());
+        shadow_live_ = true;
+    }
+
+    // The state plain iterations read and write, for a batch table.
+    struct PlainState {
+        sp_problem* problem;
+        sp_vec *x, *y, *x_bar, *x_anchor, *y_anchor, *atya, *axa;
+        sp_vec32 *dx, *dy, *dx_bar;
+        double tau, sigma, rho;
+        uint64_t k;
+    };
+    PlainState plain_state() {
+        return PlainState{dev_.get(), x_.get(), y_.get(), x_bar_.get(), x_anchor_.get(), y_anchor_.get(),
+                          atya_ ? atya_.get() : nullptr, axa_ ? axa_.get() : nullptr,
+                          dx32_ ? dx32_.get() : nullptr, dy32_ ? dy32_.get() : nullptr,
+                          dx_bar32_ ? dx_bar32_.get() : nullptr, eta_ / omega_, eta_ * omega_, p_.reflection, k_};
+    }
+
+    // Accounts for `steps` plain iterations that the caller ran.
+    void plain_done(uint64_t steps) {
+        if (p_.precision == 1 && !mixed_now()) fp64_plain_ += steps;
+        total_ += steps;
+        k_ += steps;
+    }
+
     // solver.rs:422-437. Before termination, status is -1 and the figures are
     // those of the last check.
     sp_solution finish(double* x_out, double* y_out) {
@@ -262,9 +306,9 @@ class Solver {
         b_.sub(x_t_.get(), x_.get(), tmp_n_.get());
         b_.sub(y_t_.get(), y_.get(), tmp_m2_.get());
         b_.spmv(dev_.get(), tmp_n_.get(), tmp_m_.get());
-        b_.dot_into(tmp_n_.get(), tmp_n_.get(), b_.slot(base));
-        b_.dot_into(tmp_m2_.get(), tmp_m2_.get(), b_.slot(base + 1));
-        b_.dot_into(tmp_m2_.get(), tmp_m_.get(), b_.slot(base + 2));
+        b_.dot_into(tmp_n_.get(), tmp_n_.get(), slot(base));
+        b_.dot_into(tmp_m2_.get(), tmp_m2_.get(), slot(base + 1));
+        b_.dot_into(tmp_m2_.get(), tmp_m_.get(), slot(base + 2));
     }
 
     // fixed_point_residual's closing arithmetic (solver.rs:283-287).
@@ -296,17 +340,17 @@ class Solver {
         }
         if (check) {
             b_.spmv(dev_.get(), x_t_.get(), tmp_m_.get());
-            b_.primal_stats_into(dev_.get(), x_t_.get(), tmp_m_.get(), b_.slot(kPrimal));
+            b_.primal_stats_into(dev_.get(), x_t_.get(), tmp_m_.get(), slot(kPrimal));
             b_.spmv_t(dev_.get(), y_t_.get(), tmp_n_.get());
-            b_.dual_stats_into(dev_.get(), y_t_.get(), tmp_n_.get(), b_.slot(kDual));
+            b_.dual_stats_into(dev_.get(), y_t_.get(), tmp_n_.get(), slot(kDual));
             b_.step_dual_residual_sq_into(dev_.get(), x_.get(), x_t_.get(), aty_.get(), tmp_n_.get(), tau,
-                                          b_.slot(kStep));
+                                          slot(kStep));
             if (p.termination == 1) {
-                b_.dual_stats_into(dev_.get(), y_t_.get(), c_.get(), b_.slot(kStepRows));
-                b_.dot_into(c_.get(), x_t_.get(), b_.slot(kStepDots));
-                b_.dot_into(aty_.get(), x_t_.get(), b_.slot(kStepDots + 1));
-                b_.dot_into(x_t_.get(), x_t_.get(), b_.slot(kStepDots + 2));
-                b_.dot_into(x_.get(), x_t_.get(), b_.slot(kStepDots + 3));
+                b_.dual_stats_into(dev_.get(), y_t_.get(), c_.get(), slot(kStepRows));
+                b_.dot_into(c_.get(), x_t_.get(), slot(kStepDots));
+                b_.dot_into(aty_.get(), x_t_.get(), slot(kStepDots + 1));
+                b_.dot_into(x_t_.get(), x_t_.get(), slot(kStepDots + 2));
+                b_.dot_into(x_.get(), x_t_.get(), slot(kStepDots + 3));
             }
         }
         const double w = (double)(k_ + 1) / (double)(k_ + 2);
@@ -319,13 +363,13 @@ class Solver {
         // restart displacements (solver.rs:403-404).
         b_.sub(y_t_.get(), y_anchor_.get(), tmp_m_.get());
         b_.spmv_t(dev_.get(), tmp_m_.get(), tmp_n_.get());
-        b_.dual_ray_stats_into(dev_.get(), tmp_m_.get(), tmp_n_.get(), b_.slot(kDualRay));
+        b_.dual_ray_stats_into(dev_.get(), tmp_m_.get(), tmp_n_.get(), slot(kDualRay));
         b_.sub(x_t_.get(), x_anchor_.get(), tmp_n_.get());
         b_.spmv(dev_.get(), tmp_n_.get(), tmp_m_.get());
-        b_.primal_ray_stats_into(dev_.get(), tmp_n_.get(), tmp_m_.get(), b_.slot(kPrimalRay));
-        b_.distance_sq_into(x_t_.get(), x_anchor_.get(), b_.slot(kDistX));
-        b_.distance_sq_into(y_t_.get(), y_anchor_.get(), b_.slot(kDistY));
-        const double* r = b_.fetch(kSlotCount);
+        b_.primal_ray_stats_into(dev_.get(), tmp_n_.get(), tmp_m_.get(), slot(kPrimalRay));
+        b_.distance_sq_into(x_t_.get(), x_anchor_.get(), slot(kDistX));
+        b_.distance_sq_into(y_t_.get(), y_anchor_.get(), slot(kDistY));
+        const double* r = b_.fetch_vec(slots_.get(), kSlotCount);
 
         if (anchor_pending_) {
             anchor_residual_ = residual_of(r + kAnchorFix, eta_, anchor_omega_);
@@ -432,6 +476,8 @@ class Solver {
 
     bool anchor_pending_ = false;
     double anchor_omega_ = 1.0;
+    Vec slots_;  // this solver's result slots (device), read back by one transfer per check
+    double* slot(int i) { return slots_->d + i; }
 
    private:
     using clock = std::chrono::steady_clock;
