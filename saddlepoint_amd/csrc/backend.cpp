@@ -549,6 +549,23 @@ const double* Hip::fetch_vec(const sp_vec* v, int count) {
     return b->host_slots;
 }
 
+void Hip::fetch_vec_async(const sp_vec* v, int count, double* dst) {
+    if (!v || !dst || count < 0 || count > v->len) throw Error(SP_ERR_INVALID, "fetch: bad slot count");
+    hip_check(hipMemcpyAsync(dst, v->d, sizeof(double) * (size_t)count, hipMemcpyDeviceToHost, b->stream),
+              "hipMemcpyAsync D2H");
+}
+
+double* Hip::host_alloc(int64_t count) {
+    void* p = nullptr;
+    hip_check(hipHostMalloc(&p, sizeof(double) * (size_t)std::max<int64_t>(count, 1), hipHostMallocDefault),
+              "hipHostMalloc");
+    return static_cast<double*>(p);
+}
+
+void free_host(double* h) {
+    if (h) (void)hipHostFree(h);
+}
+
 // Raw sums -> the trait's structs (cpu/src/lib.rs:211-259).
 sp_primal_stats_t Hip::primal_from(const double* r) { return sp_primal_stats_t{r[0], r[1]}; }
 // dual_stats: objective = rows + cols (cpu/src/lib.rs:223).
@@ -713,6 +730,25 @@ sp_backend* sp_backend_open(int ordinal) {
         get(b->k_bspmv_a_mixed, "sp_bspmv_dual_mixed");
         get(b->k_bfin_t_mixed, "sp_bfinish_t_primal_mixed");
         get(b->k_bfin_a_mixed, "sp_bfinish_dual_mixed");
+        get(b->k_cspmv_store, "sp_cspmv_store_f64");
+        get(b->k_cspmv_primal, "sp_cspmv_primal_f64");
+        get(b->k_cspmv_dual, "sp_cspmv_dual_f64");
+        get(b->k_cfin_store, "sp_cfinish_store_f64");
+        get(b->k_cfin_primal, "sp_cfinish_primal_f64");
+        get(b->k_cfin_dual, "sp_cfinish_dual_f64");
+        get(b->k_csub, "sp_csub_f64");
+        get(b->k_chalpern, "sp_chalpern_f64");
+        get(b->k_cdelta_to, "sp_cdelta_to");
+        get(b->k_cdelta_from, "sp_cdelta_from");
+        get(b->k_ccopy, "sp_ccopy_f64");
+        get(b->k_cred_dot, "sp_cred_dot_f64");
+        get(b->k_cred_dist, "sp_cred_distance_sq_f64");
+        get(b->k_cred_primal, "sp_cred_primal_stats_f64");
+        get(b->k_cred_dual, "sp_cred_dual_stats_f64");
+        get(b->k_cred_step, "sp_cred_step_dual_f64");
+        get(b->k_cred_pray, "sp_cred_primal_ray_f64");
+        get(b->k_cred_dray, "sp_cred_dual_ray_f64");
+        get(b->k_cred_finish, "sp_cred_finish");
         get(b->k_spmv_t_primal, "sp_spmv_t_primal_f64");
         get(b->k_spmv_t_primal_halpern, "sp_spmv_t_primal_halpern_f64");
         get(b->k_spmv_dual, "sp_spmv_dual_f64");

@@ -118,6 +118,10 @@ struct sp_backend {
         k_fin_t_primal_mixed, k_fin_dual_mixed, k_red_finish;
     hipFunction_t k_bspmv_t64, k_bspmv_a64, k_bfin_t64, k_bfin_a64, k_bspmv_t_mixed, k_bspmv_a_mixed,
         k_bfin_t_mixed, k_bfin_a_mixed;
+    // Batched check operations (serving loop).
+    hipFunction_t k_cspmv_store, k_cspmv_primal, k_cspmv_dual, k_cfin_store, k_cfin_primal, k_cfin_dual, k_csub,
+        k_chalpern, k_cdelta_to, k_cdelta_from, k_ccopy, k_cred_dot, k_cred_dist, k_cred_primal, k_cred_dual,
+        k_cred_step, k_cred_pray, k_cred_dray, k_cred_finish;
     hipFunction_t k_spmv, k_spmv_t_primal, k_spmv_t_primal_halpern, k_spmv_dual, k_spmv_dual_halpern;
     hipFunction_t k_primal_step, k_dual_step, k_halpern, k_sub, k_scale, k_fill;
     hipFunction_t k_probe_copy, k_probe_triad, k_probe_gather;
@@ -200,6 +204,11 @@ struct Hip {
     // Reads the first `count` values of a device vector into the pinned
     // staging area (valid until the next fetch).
     const double* fetch_vec(const sp_vec* v, int count);
+    // Enqueues the copy of the first `count` values of `v` into pinned host
+    // memory `dst` (from host_alloc); valid after the next sync().
+    void fetch_vec_async(const sp_vec* v, int count, double* dst);
+    // Pinned host memory for fetch_vec_async.
+    double* host_alloc(int64_t count);
     void dot_into(const sp_vec* a, const sp_vec* bv, double* dst);
     void distance_sq_into(const sp_vec* a, const sp_vec* bv, double* dst);
     void primal_stats_into(const sp_problem* p, const sp_vec* x, const sp_vec* ax, double* dst);      // 2 sums
@@ -215,14 +224,18 @@ struct Hip {
 
     void sync();
 
-   private:
+    // Workgroups of a reduction over n elements: a fixed function of n, so a
+    // given reduction always sums in the same order.
     int reduce_grid(int64_t n) const;
+
+   private:
     const double* read_result(int k, int grid);
 };
 
 void free_problem(sp_problem* p);
 void free_vec(sp_vec* v);
 void free_vec32(sp_vec32* v);
+void free_host(double* h);
 // Folds finished event pairs into the per-kernel totals (synchronizes).
 void profile_collect(sp_backend* b);
 

@@ -18,14 +18,20 @@ struct Vec32Del {
     void operator()(sp_vec32* v) const { free_vec32(v); }
 };
 
+struct HostDel {
+    void operator()(double* h) const { free_host(h); }
+};
+
 struct DriverBackend : Hip {
     using Vector = std::unique_ptr<sp_vec, VecDel>;
     using Problem = std::unique_ptr<sp_problem, ProbDel>;
     using Vector32 = std::unique_ptr<sp_vec32, Vec32Del>;
+    using HostBuffer = std::unique_ptr<double, HostDel>;
     Problem load(const ScaledProblem& s) { return Problem(Hip::load(s)); }
     Vector zeros(int64_t n) { return Vector(Hip::zeros(n)); }
     Vector upload(const double* d, int64_t n) { return Vector(Hip::upload(d, n)); }
     Vector32 zeros32(int64_t n) { return Vector32(Hip::zeros32(n)); }
+    HostBuffer host_buffer(int64_t n) { return HostBuffer(Hip::host_alloc(n)); }
 };
 
 }  // namespace sp

@@ -609,15 +609,16 @@ extern "C" __global__ __launch_bounds__(WG) void sp_delta_to(const float* __rest
 // finish adds the per-workgroup partials in workgroup order and writes the K
 // totals to `result`. One launch and one 8·K-byte readback per method.
 
+// Workgroup b of a G-workgroup reduction over [0, n).
 template <int K, class F>
-__device__ __forceinline__ void reduce_body(i64 n, const F& f, double* __restrict__ partials,
+__device__ __forceinline__ void reduce_part(i64 n, const F& f, int b, int G, double* __restrict__ partials,
                                             double* __restrict__ result) {
     double s[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) s[k] = 0.0;
     // Tiles of 4·WG elements; a thread takes elements tid, tid+WG, ... of a tile,
     // so its four loads are independent and issue together.
-    for (i64 base = (i64)blockIdx.x * 4 * WG; base < n; base += (i64)gridDim.x * 4 * WG) {
+    for (i64 base = (i64)b * 4 * WG; base < n; base += (i64)G * 4 * WG) {
         if (base + 4 * WG <= n) {
 #pragma unroll
             for (int u = 0; u < 4; ++u) f(base + threadIdx.x + u * WG, s);
@@ -630,17 +631,23 @@ __device__ __forceinline__ void reduce_body(i64 n, const F& f, double* __restric
     }
     block_sum<K>(s);
     if (threadIdx.x == 0) {
-        double* out = gridDim.x == 1 ? result : partials + (i64)blockIdx.x * K;
+        double* out = G == 1 ? result : partials + (i64)b * K;
 #pragma unroll
         for (int k = 0; k < K; ++k) out[k] = s[k];
     }
 }
 
+template <int K, class F>
+__device__ __forceinline__ void reduce_body(i64 n, const F& f, double* __restrict__ partials,
+                                            double* __restrict__ result) {
+    reduce_part<K>(n, f, (int)blockIdx.x, (int)gridDim.x, partials, result);
+}
+
 // Adds the per-workgroup partials of a reduction (G workgroups, K sums each)
 // in workgroup order: thread t takes workgroups t, t + WG, ..., then the block
 // sums in its fixed order. One workgroup.
-extern "C" __global__ __launch_bounds__(WG) void sp_red_finish(const double* __restrict__ partials, int G, int K,
-                                                               double* __restrict__ result) {
+__device__ __forceinline__ void red_finish_body(const double* __restrict__ partials, int G, int K,
+                                                double* __restrict__ result) {
     double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (int b = threadIdx.x; b < G; b += WG)
 #pragma unroll
@@ -649,6 +656,11 @@ extern "C" __global__ __launch_bounds__(WG) void sp_red_finish(const double* __r
     block_sum<5>(s);
     if (threadIdx.x == 0)
         for (int k = 0; k < K; ++k) result[k] = s[k];
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_red_finish(const double* __restrict__ partials, int G, int K,
+                                                               double* __restrict__ result) {
+    red_finish_body(partials, G, K, result);
 }
 
 #define RED_TAIL double *__restrict__ partials, double *__restrict__ result
@@ -937,4 +949,207 @@ extern "C" __global__ __launch_bounds__(WG) void sp_bfinish_dual_mixed(const BLo
     const LongRow lr{L.row, L.pbase, L.nch, 0};
     finish_body<float>(&lr, 1, (const float*)I.a_part,
                        EpiDualMixed{I.dy, I.y_anchor, I.lc, I.uc, I.axa, I.sigma, halpern_w(I.k0, it), I.rho}, 0);
+}
+
+// ---- batched check iterations for the serving loop --------------------------------
+//
+// A check iteration (and the first iteration of a restart epoch) is a fixed
+// sequence of SpMVs, element-wise passes and reductions. The serving loop runs
+// each operation of that sequence once for all LPs that are due: a launch
+// covers every LP's work items, each tagged with the LP's entry in an operand
+// table. An LP's share of a launch is the computation of the single-LP kernel
+// with the same partition and the same order (SpMV: the LP's own plan;
+// reductions: the LP's own workgroup count, partials added by the same
+// finishing code), so its results are bitwise those of a single solve.
+
+struct COp {
+    const int *rp, *ci;      // SpMV: the matrix (A' or A'ᵀ) and its
+    const double* va;        //   long-row chunk sums
+    double* part;
+    const double* in;        // SpMV input
+    double *o0, *o1, *o2;    // outputs
+    const double *a0, *a1, *a2, *a3, *a4, *a5, *a6, *a7;  // operands
+    float* f0;               // FP32 displacement (mixed precision)
+    double s0, s1;           // scalars
+    i64 m, n;                // lengths (element-wise ops use n)
+    double* dst;             // reductions: where the K sums go
+    int G, pbase;            // reductions: workgroups, offset of its partials
+};
+
+static_assert(sizeof(COp) == 184, "COp layout is shared with the host");
+
+// A workgroup of an element-wise pass or a reduction: entry `op`, tile or
+// workgroup index `b`.
+struct CBlk {
+    int op, b;
+};
+
+#define CTILE (4 * WG)
+
+// ---- SpMVs ----
+
+extern "C" __global__ __launch_bounds__(WG) void sp_cspmv_store_f64(const BDesc* __restrict__ desc,
+                                                                    const COp* __restrict__ ops) {
+    const BDesc bd = desc[blockIdx.x];
+    const COp& o = ops[bd.inst];
+    spmv_block<SP_NNZ_F64, double, double, double>(bd.d, o.rp, o.ci, o.va, o.in, o.part, EpiStore<double>{o.o0});
+}
+
+// spmv_t + primal_step: in = y; a0..a3 = x, c, lv, uv; s0 = τ; o0..o2 = aty, x_t, x_bar.
+extern "C" __global__ __launch_bounds__(WG) void sp_cspmv_primal_f64(const BDesc* __restrict__ desc,
+                                                                     const COp* __restrict__ ops) {
+    const BDesc bd = desc[blockIdx.x];
+    const COp& o = ops[bd.inst];
+    spmv_block<SP_NNZ_F64, double, double, double>(bd.d, o.rp, o.ci, o.va, o.in, o.part,
+                                                   EpiPrimal{o.a0, o.a1, o.a2, o.a3, o.s0, o.o0, o.o1, o.o2});
+}
+
+// spmv + dual_step: in = x_bar; a0..a2 = y, lc, uc; s0 = σ; o0, o1 = ax, y_t.
+extern "C" __global__ __launch_bounds__(WG) void sp_cspmv_dual_f64(const BDesc* __restrict__ desc,
+                                                                   const COp* __restrict__ ops) {
+    const BDesc bd = desc[blockIdx.x];
+    const COp& o = ops[bd.inst];
+    spmv_block<SP_NNZ_F64, double, double, double>(bd.d, o.rp, o.ci, o.va, o.in, o.part,
+                                                   EpiDual{o.a0, o.a1, o.a2, o.s0, o.o0, o.o1});
+}
+
+#define CFIN_HEAD                                                  \
+    const int w = (int)((blockIdx.x * WG + threadIdx.x) / WAVE);  \
+    if (w >= nlong) return;                                        \
+    const BLong L = longs[w];                                      \
+    const COp& o = ops[L.inst];                                    \
+    const LongRow lr{L.row, L.pbase, L.nch, 0};
+
+extern "C" __global__ __launch_bounds__(WG) void sp_cfinish_store_f64(const BLong* __restrict__ longs, int nlong,
+                                                                      const COp* __restrict__ ops) {
+    CFIN_HEAD
+    finish_body<double>(&lr, 1, o.part, EpiStore<double>{o.o0}, 0);
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_cfinish_primal_f64(const BLong* __restrict__ longs, int nlong,
+                                                                       const COp* __restrict__ ops) {
+    CFIN_HEAD
+    finish_body<double>(&lr, 1, o.part, EpiPrimal{o.a0, o.a1, o.a2, o.a3, o.s0, o.o0, o.o1, o.o2}, 0);
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_cfinish_dual_f64(const BLong* __restrict__ longs, int nlong,
+                                                                     const COp* __restrict__ ops) {
+    CFIN_HEAD
+    finish_body<double>(&lr, 1, o.part, EpiDual{o.a0, o.a1, o.a2, o.s0, o.o0, o.o1}, 0);
+}
+
+// ---- element-wise passes: one CTILE-element tile per workgroup ----
+
+#define CELEM_HEAD                                              \
+    const CBlk k = blk[blockIdx.x];                             \
+    const COp& o = ops[k.op];                                   \
+    const i64 i0 = (i64)k.b * CTILE, i1 = min(i0 + CTILE, o.n);
+
+// o0 = a0 - a1
+extern "C" __global__ __launch_bounds__(WG) void sp_csub_f64(const CBlk* __restrict__ blk,
+                                                             const COp* __restrict__ ops) {
+    CELEM_HEAD
+    for (i64 i = i0 + threadIdx.x; i < i1; i += WG) o.o0[i] = o.a0[i] - o.a1[i];
+}
+
+// o0 = z ← s0((1+s1) a0 - s1 z) + (1-s0) a1   (z_t = a0, anchor = a1, w = s0, ρ = s1)
+extern "C" __global__ __launch_bounds__(WG) void sp_chalpern_f64(const CBlk* __restrict__ blk,
+                                                                 const COp* __restrict__ ops) {
+    CELEM_HEAD
+    const double w = o.s0, rho = o.s1;
+    for (i64 i = i0 + threadIdx.x; i < i1; i += WG)
+        o.o0[i] = w * ((1.0 + rho) * o.a0[i] - rho * o.o0[i]) + (1.0 - w) * o.a1[i];
+}
+
+// o0 = a0 + f0 (anchor + FP32 displacement)
+extern "C" __global__ __launch_bounds__(WG) void sp_cdelta_to(const CBlk* __restrict__ blk,
+                                                              const COp* __restrict__ ops) {
+    CELEM_HEAD
+    for (i64 i = i0 + threadIdx.x; i < i1; i += WG) o.o0[i] = o.a0[i] + (double)o.f0[i];
+}
+
+// f0 = (float)(a0 - a1) (iterate - anchor)
+extern "C" __global__ __launch_bounds__(WG) void sp_cdelta_from(const CBlk* __restrict__ blk,
+                                                                const COp* __restrict__ ops) {
+    CELEM_HEAD
+    for (i64 i = i0 + threadIdx.x; i < i1; i += WG) o.f0[i] = (float)(o.a0[i] - o.a1[i]);
+}
+
+// o0 = a0 (gathers every LP's result slots into one buffer for one readback,
+// and the restart copies)
+extern "C" __global__ __launch_bounds__(WG) void sp_ccopy_f64(const CBlk* __restrict__ blk,
+                                                              const COp* __restrict__ ops) {
+    CELEM_HEAD
+    for (i64 i = i0 + threadIdx.x; i < i1; i += WG) o.o0[i] = o.a0[i];
+}
+
+// ---- reductions: workgroup b of the entry's G ----
+
+#define CRED_HEAD                   \
+    const CBlk k = blk[blockIdx.x]; \
+    const COp& o = ops[k.op];
+
+extern "C" __global__ __launch_bounds__(WG) void sp_cred_dot_f64(const CBlk* __restrict__ blk,
+                                                                 const COp* __restrict__ ops,
+                                                                 double* __restrict__ partials) {
+    CRED_HEAD
+    reduce_part<1>(o.n, FDot{o.a0, o.a1}, k.b, o.G, partials + o.pbase, o.dst);
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_cred_distance_sq_f64(const CBlk* __restrict__ blk,
+                                                                         const COp* __restrict__ ops,
+                                                                         double* __restrict__ partials) {
+    CRED_HEAD
+    reduce_part<1>(o.n, FDist{o.a0, o.a1}, k.b, o.G, partials + o.pbase, o.dst);
+}
+
+// a0..a5 = ax, lc, uc, pf, c, x
+extern "C" __global__ __launch_bounds__(WG) void sp_cred_primal_stats_f64(const CBlk* __restrict__ blk,
+                                                                          const COp* __restrict__ ops,
+                                                                          double* __restrict__ partials) {
+    CRED_HEAD
+    reduce_part<2>(o.m + o.n, FPrimal{o.a0, o.a1, o.a2, o.a3, o.a4, o.a5, o.m}, k.b, o.G, partials + o.pbase, o.dst);
+}
+
+// a0..a7 = y, lc, uc, c, aty, lv, uv, df
+extern "C" __global__ __launch_bounds__(WG) void sp_cred_dual_stats_f64(const CBlk* __restrict__ blk,
+                                                                        const COp* __restrict__ ops,
+                                                                        double* __restrict__ partials) {
+    CRED_HEAD
+    reduce_part<3>(o.m + o.n, FDual{o.a0, o.a1, o.a2, o.a3, o.a4, o.a5, o.a6, o.a7, o.m}, k.b, o.G,
+                   partials + o.pbase, o.dst);
+}
+
+// a0..a4 = x, x_t, aty, aty_t, df; s0 = 1/τ
+extern "C" __global__ __launch_bounds__(WG) void sp_cred_step_dual_f64(const CBlk* __restrict__ blk,
+                                                                       const COp* __restrict__ ops,
+                                                                       double* __restrict__ partials) {
+    CRED_HEAD
+    reduce_part<1>(o.n, FStep{o.a0, o.a1, o.a2, o.a3, o.a4, o.s0}, k.b, o.G, partials + o.pbase, o.dst);
+}
+
+// a0..a6 = adx, lc, uc, dx, c, lv, uv
+extern "C" __global__ __launch_bounds__(WG) void sp_cred_primal_ray_f64(const CBlk* __restrict__ blk,
+                                                                        const COp* __restrict__ ops,
+                                                                        double* __restrict__ partials) {
+    CRED_HEAD
+    reduce_part<4>(o.m + o.n, FPrimalRay{o.a0, o.a1, o.a2, o.a3, o.a4, o.a5, o.a6, o.m}, k.b, o.G,
+                   partials + o.pbase, o.dst);
+}
+
+// a0..a5 = dy, lc, uc, atdy, lv, uv
+extern "C" __global__ __launch_bounds__(WG) void sp_cred_dual_ray_f64(const CBlk* __restrict__ blk,
+                                                                      const COp* __restrict__ ops,
+                                                                      double* __restrict__ partials) {
+    CRED_HEAD
+    reduce_part<5>(o.m + o.n, FDualRay{o.a0, o.a1, o.a2, o.a3, o.a4, o.a5, o.m}, k.b, o.G, partials + o.pbase,
+                   o.dst);
+}
+
+// One workgroup per listed entry with G > 1: its partials added in order.
+extern "C" __global__ __launch_bounds__(WG) void sp_cred_finish(const int* __restrict__ which,
+                                                                const COp* __restrict__ ops,
+                                                                const double* __restrict__ partials, int K) {
+    const COp& o = ops[which[blockIdx.x]];
+    red_finish_body(partials + o.pbase, o.G, K, o.dst);
 }

@@ -9,16 +9,19 @@
 // The loop runs in chunks of 64 iterations aligned to every LP's own check
 // cadence (an LP is admitted at a chunk boundary with its iteration count at
 // 0, so its checks fall on the chunk's last step). In a chunk:
-//   step 1      LPs that start a restart epoch (k = 0) take their iteration
-//               alone (it measures the anchor's fixed-point residual); the
-//               others take it together, as a batched plain iteration;
+//   step 1      LPs that start a restart epoch (k = 0) take the iteration
+//               that also measures the anchor's fixed-point residual, as one
+//               batched phase; the others take a batched plain iteration;
 //   steps 2-63  one batched plain iteration for all LPs per step: one
 //               launch per side, over the concatenated work items of every
 //               LP's own SpMV plan (optionally replayed from a hipGraph);
-//   step 64     each LP's check (KKT, infeasibility, restart, PID) in FP64,
-//               through the same code as a single solve.
+//   step 64     every LP's check (KKT, infeasibility, restart, PID) in FP64:
+//               the device work as one batched phase (batch_check.hpp), one
+//               readback of every LP's result slots, then each LP's
+//               decisions by the code of a single solve.
 // Because each LP keeps its own plan and its own per-LP reductions, every
-// LP's iterates are bitwise the same alone and in any mix.
+// LP's iterates are bitwise the same alone, in any mix, and in a single
+// solve.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -28,6 +31,7 @@
 #include <mutex>
 #include <thread>
 
+#include "batch_check.hpp"
 #include "driver.hpp"
 #include "saddlepoint_hip.h"
 #include "solver.hpp"
@@ -54,16 +58,6 @@ struct BInst {
     double tau, sigma, rho;
     unsigned long long k0;
 };
-struct BDesc {
-    Desc d;
-    int inst, pad0, pad1, pad2;
-};
-struct BLong {
-    int inst, row, pbase, nch;
-};
-static_assert(sizeof(BDesc) == 32, "BDesc layout");
-static_assert(sizeof(BLong) == 16, "BLong layout");
-
 using Clock = std::chrono::steady_clock;
 
 constexpr uint64_t kChunk = 64;
@@ -161,6 +155,9 @@ class Server {
         (void)hipStreamSynchronize(b_->stream);
         for (auto& kv : jobs_) kv.second->solver.reset();
         if (h_inst_) (void)hipHostFree(h_inst_);
+        if (cpart_) (void)hipFree(cpart_);
+        if (gath_d_) (void)hipFree(gath_d_);
+        if (gath_h_) (void)hipHostFree(gath_h_);
     }
 
     double clock() const { return std::chrono::duration<double>(Clock::now() - t0_).count(); }
@@ -319,13 +316,25 @@ class Server {
     }
 
     void chunk() {
+        // Everything of the previous chunk has left the stream (its last
+        // work, the restart copies, is waited for here), so the pinned
+        // staging areas can be reused.
+        hip_check(hipStreamSynchronize(b_->stream), "hipStreamSynchronize");
+        arena_.reset();
+        staged_ = 0;
+        ensure_staging(2 * active_.size());
         // Step 1.
-        std::vector<Job*> batch1;
+        std::vector<Job*> first, batch1;
         for (Job* j : active_) {
-            if (j->solver->plain_run() == 0) own_step(j);
-            else batch1.push_back(j);
+            if (j->failed_now || j->solver->terminated()) continue;
+            if (j->solver->plain_run() == 0) {
+                if (!j->solver->check_due()) first.push_back(j);
+            } else {
+                batch1.push_back(j);
+            }
         }
-        if (!batch1.empty()) run_batched(batch1, 1, false);
+        run_steps(first);
+        if (!batch1.empty()) run_batched(batch1, 1, false, first64_, first32_);
         // Steps 2-63: everyone that is still running and in step.
         std::vector<Job*> all;
         uint64_t steps = UINT64_MAX;
@@ -334,25 +343,106 @@ class Server {
             all.push_back(j);
             steps = std::min(steps, j->solver->plain_run());
         }
-        if (!all.empty() && steps > 0 && steps != UINT64_MAX) run_batched(all, (int)steps, sp_.use_graphs != 0);
-        // Step 64 (and anything else that is due): each LP's own check.
+        if (!all.empty() && steps > 0 && steps != UINT64_MAX)
+            run_batched(all, (int)steps, sp_.use_graphs != 0, batch64_, batch32_);
+        // Step 64: every LP's check.
+        std::vector<Job*> due;
         for (Job* j : active_) {
-            while (!j->failed_now && !j->solver->terminated() && j->solver->plain_run() == 0) {
-                const uint64_t before = j->solver->iterations();
-                own_step(j);
-                if (j->solver->iterations() % kChunk == 0 || j->solver->iterations() == before) break;
-            }
+            while (!j->failed_now && !j->solver->terminated() && j->solver->plain_run() == 0 &&
+                   !j->solver->check_due())
+                own_step(j);  // a restart epoch's first iteration (not reached with aligned chunks)
+            if (!j->failed_now && j->solver->check_due()) due.push_back(j);
         }
+        run_steps(due);
     }
 
-    // `steps` batched plain iterations of `group`.
-    void run_batched(const std::vector<Job*>& group, int steps, bool graph) {
+    // The next non-plain iteration of every LP in `group` (all checks, or
+    // all first iterations of a restart epoch) as one batched phase; for
+    // checks, one readback and each LP's decisions, then the restart copies.
+    void run_steps(const std::vector<Job*>& group) {
+        if (group.empty()) return;
+        std::vector<StepWork> ws;
+        std::vector<Job*> who;
+        for (Job* j : group) {
+            try {
+                ws.push_back(j->solver->begin_step());
+                who.push_back(j);
+            } catch (const Error& e) {
+                mark_failed(j, e);
+            }
+        }
+        if (ws.empty()) return;
+        const bool check = ws[0].check;
+        for (const StepWork& w : ws)
+            if (w.check != check) throw Error(SP_ERR_INVALID, "serving: mixed step kinds in one phase");
+        constexpr int K = Solver<DriverBackend>::slot_count();
+        Phase P(b_);
+        enqueue_step_work(P, db_, ws);
+        if (check) {
+            ensure_gather(ws.size() * K);
+            gather_slots(P, db_, ws, gath_d_, K);
+        }
+        P.run(arena_, cpart_, cpart_cap_);
+        if (!check) return;
+        hip_check(hipMemcpyAsync(gath_h_, gath_d_, ws.size() * K * sizeof(double), hipMemcpyDeviceToHost,
+                                 b_->stream),
+                  "hipMemcpyAsync D2H");
+        hip_check(hipStreamSynchronize(b_->stream), "hipStreamSynchronize");
+        std::vector<std::pair<sp_vec*, sp_vec*>> copies;
+        for (size_t i = 0; i < who.size(); ++i) {
+            try {
+                who[i]->solver->end_step(gath_h_ + i * K, &copies);
+            } catch (const Error& e) {
+                mark_failed(who[i], e);
+            }
+        }
+        Phase C(b_);
+        enqueue_copies(C, db_, copies);
+        C.run(arena_, cpart_, cpart_cap_);
+    }
+
+    void mark_failed(Job* j, const Error& e) {
+        j->err = e.code;
+        j->msg = e.what();
+        j->failed_now = true;
+    }
+
+    void ensure_gather(size_t n) {
+        if (gath_cap_ >= n) return;
+        hip_check(hipStreamSynchronize(b_->stream), "hipStreamSynchronize");
+        if (gath_h_) (void)hipHostFree(gath_h_);
+        if (gath_d_) (void)hipFree(gath_d_);
+        gath_h_ = nullptr;
+        gath_d_ = nullptr;
+        gath_cap_ = std::max(n, 2 * gath_cap_);
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&gath_h_), gath_cap_ * sizeof(double), hipHostMallocDefault),
+                  "hipHostMalloc");
+        hip_check(hipMalloc(reinterpret_cast<void**>(&gath_d_), gath_cap_ * sizeof(double)), "hipMalloc");
+    }
+
+    struct Batch;
+
+    // `steps` batched plain iterations of `group`, through the FP64 and the
+    // mixed-precision batch tables.
+    void run_batched(const std::vector<Job*>& group, int steps, bool graph, Batch& b64, Batch& b32) {
+        // Mixed-precision LPs entering plain iterations: anchor products and
+        // displacements, for all of them at once.
+        std::vector<PlainEntry> entries;
+        for (Job* j : group) {
+            PlainEntry e;
+            if (j->solver->begin_plain(e)) entries.push_back(e);
+        }
+        if (!entries.empty()) {
+            Phase P(b_);
+            enqueue_plain_entries(P, db_, entries);
+            P.run(arena_, cpart_, cpart_cap_);
+        }
         std::vector<Job*> g64, g32;
         for (Job* j : group) (j->solver->mixed_now() ? g32 : g64).push_back(j);
         for (int mixed = 0; mixed < 2; ++mixed) {
             const std::vector<Job*>& g = mixed ? g32 : g64;
             if (g.empty()) continue;
-            Batch& bt = mixed ? batch32_ : batch64_;
+            Batch& bt = mixed ? b32 : b64;
             build(bt, g, mixed != 0);
             const bool use_graph = graph && steps > 1;
             launch_steps(bt, steps, mixed != 0, use_graph);
@@ -361,30 +451,51 @@ class Server {
     }
 
     struct Batch {
-        std::vector<Job*> members;
+        std::vector<uint64_t> members;  // job ids (a freed job's address can come back)
+        // The concatenated work lists (host copies live until the next
+        // rebuild, so that their uploads need no synchronization).
+        std::vector<BDesc> h_dt, h_da;
+        std::vector<BLong> h_lt, h_la;
         DevBuf<BDesc> desc_t, desc_a;
         DevBuf<BLong> long_t, long_a;
         DevBuf<BInst> inst;
         int nt = 0, na = 0, nlt = 0, nla = 0;
         hipGraphExec_t exec = nullptr;
         int graph_steps = 0;
-        ~Batch() {
+        void drop_graph() {
             if (exec) (void)hipGraphExecDestroy(exec);
+            exec = nullptr;
         }
+        ~Batch() { drop_graph(); }
     };
 
+    void ensure_staging(size_t n) {
+        if (host_inst_cap_ >= n) return;
+        hip_check(hipStreamSynchronize(b_->stream), "hipStreamSynchronize");
+        if (h_inst_) (void)hipHostFree(h_inst_);
+        h_inst_ = nullptr;
+        host_inst_cap_ = std::max<size_t>(n, 2 * host_inst_cap_);
+        hip_check(
+            hipHostMalloc(reinterpret_cast<void**>(&h_inst_), host_inst_cap_ * sizeof(BInst), hipHostMallocDefault),
+            "hipHostMalloc");
+    }
+
     void build(Batch& bt, const std::vector<Job*>& g, bool mixed) {
-        for (Job* j : g) j->solver->enter_plain();
-        const bool same = bt.members == g;
-        if (!same) {
-            std::vector<BDesc> dt, da;
-            std::vector<BLong> lt, la;
+        std::vector<uint64_t> ids;
+        for (Job* j : g) ids.push_back(j->id);
+        if (bt.members != ids) {
+            // The previous uploads from these host vectors were ordered on the
+            // stream before any work of this chunk; a check synchronized since.
+            bt.h_dt.clear();
+            bt.h_da.clear();
+            bt.h_lt.clear();
+            bt.h_la.clear();
             for (size_t i = 0; i < g.size(); ++i) {
                 const sp_problem* p = g[i]->solver->plain_state().problem;
-                for (const Desc& d : p->at.host_desc) dt.push_back(BDesc{d, (int)i, 0, 0, 0});
-                for (const Desc& d : p->a.host_desc) da.push_back(BDesc{d, (int)i, 0, 0, 0});
-                for (const LongRow& l : p->at.host_longs) lt.push_back(BLong{(int)i, l.row, l.pbase, l.nch});
-                for (const LongRow& l : p->a.host_longs) la.push_back(BLong{(int)i, l.row, l.pbase, l.nch});
+                for (const Desc& d : p->at.host_desc) bt.h_dt.push_back(BDesc{d, (int)i, 0, 0, 0});
+                for (const Desc& d : p->a.host_desc) bt.h_da.push_back(BDesc{d, (int)i, 0, 0, 0});
+                for (const LongRow& l : p->at.host_longs) bt.h_lt.push_back(BLong{(int)i, l.row, l.pbase, l.nch});
+                for (const LongRow& l : p->a.host_longs) bt.h_la.push_back(BLong{(int)i, l.row, l.pbase, l.nch});
             }
             auto up = [&](auto& buf, const auto& v) {
                 buf.ensure(v.size());
@@ -393,36 +504,26 @@ class Server {
                                              b_->stream),
                               "hipMemcpyAsync H2D");
             };
-            up(bt.desc_t, dt);
-            up(bt.desc_a, da);
-            up(bt.long_t, lt);
-            up(bt.long_a, la);
-            bt.nt = (int)dt.size();
-            bt.na = (int)da.size();
-            bt.nlt = (int)lt.size();
-            bt.nla = (int)la.size();
-            bt.members = g;
-            if (bt.exec) {
-                (void)hipGraphExecDestroy(bt.exec);
-                bt.exec = nullptr;
-            }
-            hip_check(hipStreamSynchronize(b_->stream), "hipStreamSynchronize");  // host vectors die here
+            bt.drop_graph();
+            up(bt.desc_t, bt.h_dt);
+            up(bt.desc_a, bt.h_da);
+            up(bt.long_t, bt.h_lt);
+            up(bt.long_a, bt.h_la);
+            bt.nt = (int)bt.h_dt.size();
+            bt.na = (int)bt.h_da.size();
+            bt.nlt = (int)bt.h_lt.size();
+            bt.nla = (int)bt.h_la.size();
+            bt.members = std::move(ids);
+            bt.inst.ensure(g.size());
         }
-        // The per-LP table: arrays and this chunk's step parameters.
-        bt.inst.ensure(g.size());
-        if (host_inst_cap_ < g.size()) {
-            if (h_inst_) (void)hipHostFree(h_inst_);
-            host_inst_cap_ = std::max<size_t>(g.size(), 2 * host_inst_cap_);
-            hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_inst_), host_inst_cap_ * sizeof(BInst),
-                                    hipHostMallocDefault),
-                      "hipHostMalloc");
-        }
-        // The pinned staging area is reused: wait for the previous upload.
-        hip_check(hipStreamSynchronize(b_->stream), "hipStreamSynchronize");
+        // The per-LP table: arrays and this step's parameters.
+        if (staged_ + g.size() > host_inst_cap_) throw Error(SP_ERR_INVALID, "serving: staging area overflow");
+        BInst* stage = h_inst_ + staged_;
+        staged_ += g.size();
         for (size_t i = 0; i < g.size(); ++i) {
             const auto s = g[i]->solver->plain_state();
             const sp_problem* p = s.problem;
-            BInst& I = h_inst_[i];
+            BInst& I = stage[i];
             std::memset(&I, 0, sizeof I);
             I.a_rp = p->a.rp;
             I.a_ci = p->a.ci;
@@ -456,7 +557,7 @@ class Server {
             I.rho = s.rho;
             I.k0 = s.k;
         }
-        hip_check(hipMemcpyAsync(bt.inst.d, h_inst_, g.size() * sizeof(BInst), hipMemcpyHostToDevice, b_->stream),
+        hip_check(hipMemcpyAsync(bt.inst.d, stage, g.size() * sizeof(BInst), hipMemcpyHostToDevice, b_->stream),
                   "hipMemcpyAsync H2D");
     }
 
@@ -486,10 +587,7 @@ class Server {
             return;
         }
         if (!bt.exec || bt.graph_steps != steps) {
-            if (bt.exec) {
-                (void)hipGraphExecDestroy(bt.exec);
-                bt.exec = nullptr;
-            }
+            bt.drop_graph();
             hipGraph_t graph_ = nullptr;
             hip_check(hipStreamBeginCapture(b_->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
             try {
@@ -554,9 +652,19 @@ class Server {
     uint64_t finished_ = 0;
     std::vector<Job*> active_;
     int64_t used_ = 0;
-    Batch batch64_, batch32_;
+    Batch first64_, first32_;  // step 1 of a chunk (membership changes with every restart)
+    Batch batch64_, batch32_;  // steps 2-63 (membership changes with admissions and retirements)
     BInst* h_inst_ = nullptr;
     size_t host_inst_cap_ = 0;
+    size_t staged_ = 0;
+    // Batched phases: table arena, reduction partials, the result slots'
+    // gather buffer and its pinned host copy.
+    TableArena arena_{b_};
+    double* cpart_ = nullptr;
+    size_t cpart_cap_ = 0;
+    double* gath_d_ = nullptr;
+    double* gath_h_ = nullptr;
+    size_t gath_cap_ = 0;
 };
 
 }  // namespace sp

@@ -15,6 +15,7 @@
 #include <limits>
 #include <memory>
 #include <utility>
+#include <vector>
 
 #include "host.hpp"
 #include "saddlepoint_hip.h"
@@ -86,6 +87,7 @@ class Solver {
         tmp_n_ = b_.zeros(n), tmp_m_ = b_.zeros(m), tmp_m2_ = b_.zeros(m);
         c_ = b_.upload(scaled_.c.data(), n);
         slots_ = b_.zeros(kSlotCount);
+        host_slots_ = b_.host_buffer(kSlotCount);
         if (params.precision == 1) {
             if (!params.fused) throw Error(SP_ERR_INVALID, "mixed precision needs the fused path");
             b_.ensure_fp32(dev_.get());
@@ -215,6 +217,137 @@ class Solver {
         return next - 1 - total_;
     }
 
+    // Whether the next iteration is a check iteration of the fused path.
+    bool check_due() const {
+        if (terminated_ || !p_.fused) return false;
+        const uint64_t next_total = total_ + 1;
+        return next_total % p_.check_every == 0 || next_total >= p_.max_iterations;
+    }
+
+    // A check iteration in two halves, so that a caller can run many solvers'
+    // checks behind one stream synchronization: check_begin() enqueues the
+    // iteration and the readback of its results; after the caller has
+    // synchronized the backend's stream, check_end() takes the decisions and
+    // returns whether the solve has ended. Together they are advance(1) on a
+    // check iteration.
+    void check_begin() {
+        if (!check_due()) throw Error(SP_ERR_INVALID, "check_begin: no check is due");
+        if (shadow_live_) {
+            b_.delta_to(dx32_.get(), x_anchor_.get(), x_.get());
+            b_.delta_to(dy32_.get(), y_anchor_.get(), y_.get());
+            shadow_live_ = false;
+        }
+        enqueue_step(true, eta_ / omega_, eta_ * omega_, total_ + 1);
+        check_pending_ = true;
+    }
+    bool check_end() {
+        if (!check_pending_) throw Error(SP_ERR_INVALID, "check_end: no check_begin before it");
+        check_pending_ = false;
+        return decide();
+    }
+
+    // ---- for a caller that batches non-plain iterations across solvers ----
+    //
+    // begin_step() describes the device work of the next iteration that is
+    // not a plain one (a check iteration, or the first iteration of a restart
+    // epoch) and does its bookkeeping; the caller then runs exactly the
+    // operations of enqueue_step on these operands (together with other
+    // solvers' work). For a check, the caller reads the result slots back and
+    // passes them to end_step(), which takes the decisions. Restart copies are
+    // appended to `copies` (source, destination) for the caller to run.
+    struct StepWork {
+        bool check = false, k_was_zero = false, step_implied = false;
+        bool rebuild = false;  // mixed precision: x, y = anchor + displacement first
+        int fix_base = 0;      // slot of the fixed-point sums
+        sp_problem* problem = nullptr;
+        sp_vec *x, *y, *x_t, *y_t, *x_bar, *x_anchor, *y_anchor, *ax, *aty, *tmp_n, *tmp_m, *tmp_m2, *c;
+        sp_vec32 *dx = nullptr, *dy = nullptr;
+        double* slots = nullptr;  // device, slot_count() values
+        double tau = 0, sigma = 0, w = 0, rho = 0;
+    };
+    static constexpr int slot_count() { return kSlotCount; }
+    static constexpr int slot_fix() { return kFix; }
+    static constexpr int slot_primal() { return kPrimal; }
+    static constexpr int slot_dual() { return kDual; }
+    static constexpr int slot_step() { return kStep; }
+    static constexpr int slot_step_rows() { return kStepRows; }
+    static constexpr int slot_step_dots() { return kStepDots; }
+    static constexpr int slot_dual_ray() { return kDualRay; }
+    static constexpr int slot_primal_ray() { return kPrimalRay; }
+    static constexpr int slot_dist_x() { return kDistX; }
+    static constexpr int slot_dist_y() { return kDistY; }
+
+    StepWork begin_step() {
+        if (terminated_ || !p_.fused) throw Error(SP_ERR_INVALID, "begin_step: not running the fused path");
+        const bool check = check_due();
+        if (!check && k_ != 0) throw Error(SP_ERR_INVALID, "begin_step: a plain iteration is due");
+        StepWork s;
+        s.rebuild = shadow_live_;
+        shadow_live_ = false;
+        s.check = check;
+        s.k_was_zero = k_ == 0;
+        s.step_implied = p_.termination == 1;
+        s.fix_base = check ? kFix : kAnchorFix;
+        s.problem = dev_.get();
+        s.x = x_.get(), s.y = y_.get(), s.x_t = x_t_.get(), s.y_t = y_t_.get(), s.x_bar = x_bar_.get();
+        s.x_anchor = x_anchor_.get(), s.y_anchor = y_anchor_.get(), s.ax = ax_.get(), s.aty = aty_.get();
+        s.tmp_n = tmp_n_.get(), s.tmp_m = tmp_m_.get(), s.tmp_m2 = tmp_m2_.get(), s.c = c_.get();
+        s.dx = dx32_ ? dx32_.get() : nullptr;
+        s.dy = dy32_ ? dy32_.get() : nullptr;
+        s.slots = slots_->d;
+        s.tau = eta_ / omega_;
+        s.sigma = eta_ * omega_;
+        s.w = (double)(k_ + 1) / (double)(k_ + 2);
+        s.rho = p_.reflection;
+        // The bookkeeping of enqueue_step.
+        total_ += 1;
+        if (check) {
+            pending_tau_ = s.tau;
+            pending_k_was_zero_ = s.k_was_zero;
+            check_pending_ = true;
+        } else {
+            anchor_pending_ = true;
+            anchor_omega_ = omega_;
+        }
+        k_ += 1;
+        return s;
+    }
+
+    bool end_step(const double* slots, std::vector<std::pair<sp_vec*, sp_vec*>>* copies) {
+        if (!check_pending_) throw Error(SP_ERR_INVALID, "end_step: no check is pending");
+        check_pending_ = false;
+        std::copy(slots, slots + kSlotCount, host_slots_.get());
+        copy_sink_ = copies;
+        try {
+            const bool done = decide();
+            copy_sink_ = nullptr;
+            return done;
+        } catch (...) {
+            copy_sink_ = nullptr;
+            throw;
+        }
+    }
+
+    // What enter_plain() would enqueue, described for a batching caller:
+    // the anchors' FP64 products (when the anchors moved) and the FP32
+    // displacements. Returns false when there is nothing to do.
+    struct PlainEntry {
+        bool products = false;
+        sp_problem* problem = nullptr;
+        sp_vec *x, *y, *x_anchor, *y_anchor, *atya, *axa;
+        sp_vec32 *dx, *dy;
+    };
+    bool begin_plain(PlainEntry& e) {
+        if (!mixed_now() || shadow_live_) return false;
+        e.products = anchor_epoch_ != restarts_;
+        anchor_epoch_ = restarts_;
+        e.problem = dev_.get();
+        e.x = x_.get(), e.y = y_.get(), e.x_anchor = x_anchor_.get(), e.y_anchor = y_anchor_.get();
+        e.atya = atya_.get(), e.axa = axa_.get(), e.dx = dx32_.get(), e.dy = dy32_.get();
+        shadow_live_ = true;
+        return true;
+    }
+
     // Whether plain iterations run in mixed precision now.
     bool mixed_now() const { return p_.precision == 1 && !fp64_fallback_; }
 
@@ -325,6 +458,15 @@ class Solver {
     // follow are those of solver.rs:340-419, on the same values. Returns true
     // when the loop has ended.
     bool deferred_step(bool check, double tau, double sigma, uint64_t next_total) {
+        enqueue_step(check, tau, sigma, next_total);
+        if (!check) return false;
+        b_.sync();
+        return decide();
+    }
+
+    // The device work of deferred_step; on a check it ends with the
+    // asynchronous readback of the result slots into host_slots_.
+    void enqueue_step(bool check, double tau, double sigma, uint64_t next_total) {
         const sp_params& p = p_;
         b_.spmv_t_primal(dev_.get(), y_.get(), x_.get(), tau, aty_.get(), x_t_.get(), x_bar_.get());
         b_.spmv_dual(dev_.get(), x_bar_.get(), y_.get(), sigma, ax_.get(), y_t_.get());
@@ -357,7 +499,7 @@ class Solver {
         b_.halpern_step(x_.get(), x_t_.get(), x_anchor_.get(), w, p.reflection);
         b_.halpern_step(y_.get(), y_t_.get(), y_anchor_.get(), w, p.reflection);
         k_ += 1;
-        if (!check) return false;
+        if (!check) return;
 
         // Speculative: the infeasibility rays (solver.rs:291-309) and the
         // restart displacements (solver.rs:403-404).
@@ -369,8 +511,18 @@ class Solver {
         b_.primal_ray_stats_into(dev_.get(), tmp_n_.get(), tmp_m_.get(), slot(kPrimalRay));
         b_.distance_sq_into(x_t_.get(), x_anchor_.get(), slot(kDistX));
         b_.distance_sq_into(y_t_.get(), y_anchor_.get(), slot(kDistY));
-        const double* r = b_.fetch_vec(slots_.get(), kSlotCount);
+        b_.fetch_vec_async(slots_.get(), kSlotCount, host_slots_.get());
+        pending_tau_ = tau;
+        pending_k_was_zero_ = k_was_zero;
+    }
 
+    // The host decisions of a check iteration on the slots read back by
+    // enqueue_step (after the stream has been synchronized).
+    bool decide() {
+        const sp_params& p = p_;
+        const double* r = host_slots_.get();
+        const double tau = pending_tau_;
+        const bool k_was_zero = pending_k_was_zero_;
         if (anchor_pending_) {
             anchor_residual_ = residual_of(r + kAnchorFix, eta_, anchor_omega_);
             anchor_pending_ = false;
@@ -444,10 +596,17 @@ class Solver {
         // The PID balances against the step-based dual residual.
         const double rel_dual = step_dual / (1.0 + c_norm_);
         omega_ = pid_.update(p, omega_, dx, dy, rel_primal, rel_dual);
-        b_.copy(x_t_.get(), x_anchor_.get());
-        b_.copy(y_t_.get(), y_anchor_.get());
-        b_.copy(x_t_.get(), x_.get());
-        b_.copy(y_t_.get(), y_.get());
+        if (copy_sink_) {
+            copy_sink_->emplace_back(x_t_.get(), x_anchor_.get());
+            copy_sink_->emplace_back(y_t_.get(), y_anchor_.get());
+            copy_sink_->emplace_back(x_t_.get(), x_.get());
+            copy_sink_->emplace_back(y_t_.get(), y_.get());
+        } else {
+            b_.copy(x_t_.get(), x_anchor_.get());
+            b_.copy(y_t_.get(), y_anchor_.get());
+            b_.copy(x_t_.get(), x_.get());
+            b_.copy(y_t_.get(), y_.get());
+        }
         k_ = 0;
         restarts_ += 1;
         prev_residual_ = INFINITY;
@@ -477,6 +636,11 @@ class Solver {
     bool anchor_pending_ = false;
     double anchor_omega_ = 1.0;
     Vec slots_;  // this solver's result slots (device), read back by one transfer per check
+    typename B::HostBuffer host_slots_;  // their pinned host copy
+    double pending_tau_ = 0.0;
+    bool pending_k_was_zero_ = false;
+    bool check_pending_ = false;
+    std::vector<std::pair<sp_vec*, sp_vec*>>* copy_sink_ = nullptr;  // restart copies deferred to a batching caller
     double* slot(int i) { return slots_->d + i; }
 
    private:
