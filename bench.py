@@ -86,6 +86,123 @@ def cpu_baseline(problem, seconds_budget: float) -> dict:
                       f"{scale_s:.1f}s of host scaling excluded), oracle C restatement with {cores} OpenMP threads"}
 
 
+def pct(v, q):
+    return float(np.percentile(np.asarray(v, float), q)) if len(v) else None
+
+
+def fifo_latencies(arrivals, service):
+    """Completion - arrival of a one-at-a-time server taking jobs in arrival
+    order with the given service times (exact for a deterministic FIFO queue)."""
+    free, out = 0.0, []
+    for a, s in zip(arrivals, service):
+        start = max(a, free)
+        free = start + s
+        out.append(free - a)
+    return out
+
+
+def run_served(b, lps, params_of, arrivals=None, max_active=64):
+    """Serves `lps` (all at t = 0, or at the given arrival times on the
+    server's clock); returns (makespan, latencies, results)."""
+    from saddlepoint_amd.serving import Server
+
+    srv = Server(b, max_active=max_active)
+    n = len(lps)
+    ids = [None] * n
+    t_arr = [0.0] * n
+    nxt = 0
+    t0 = srv.clock()
+    if arrivals is None:
+        for i, p in enumerate(lps):
+            ids[i] = srv.submit(p, params_of(i))
+            t_arr[i] = t0
+        nxt = n
+    done, last_report = 0, time.time()
+    while done < n:
+        now = srv.clock()
+        while nxt < n and t0 + arrivals[nxt] <= now:
+            ids[nxt] = srv.submit(lps[nxt], params_of(nxt))
+            t_arr[nxt] = t0 + arrivals[nxt]
+            nxt += 1
+        active, waiting, done = srv.run_chunk()
+        if active == 0 and waiting == 0 and nxt < n and done < n:
+            time.sleep(max(0.0, t0 + arrivals[nxt] - srv.clock()))
+        if time.time() - last_report > 20:
+            log(f"  served {done}/{n} (active {active}, waiting {waiting})")
+            last_report = time.time()
+    t_end = srv.clock()
+    res = [srv.result(j, with_solution=False) for j in ids]
+    lat = [r.completed - ta for r, ta in zip(res, t_arr)]
+    srv.close()
+    return t_end - t0, lat, [r.result for r in res]
+
+
+def serving_bench(args) -> dict:
+    from saddlepoint_amd import generate
+    from saddlepoint_amd.backend import HipBackend, default_params, solve
+
+    t0 = time.time()
+    lps = generate.serving_stream(args.config, args.lps)
+    tols = [1e-4 if i % 2 == 0 else 1e-6 for i in range(len(lps))]
+    nnz = [p.a.nnz for p in lps]
+    log(f"generated {len(lps)} LPs of the config-{args.config} stream in {time.time() - t0:.1f}s; "
+        f"nnz mean {np.mean(nnz):.0f}, max {max(nnz)}")
+    b = HipBackend(0)
+    # Plain FP64 one at a time: every LP's whole solve (validate, scale, load,
+    # solve, download), back to back.
+    service, one = [], []
+    for i, (p, tol) in enumerate(zip(lps, tols)):
+        s = time.perf_counter()
+        r = solve(b, p, default_params(tolerance=tol))
+        service.append(time.perf_counter() - s)
+        one.append(r)
+        log(f"  one-at-a-time {i}: m={p.nrows} nnz={p.a.nnz} tol={tol:g} {r.status} {r.iterations} it "
+            f"{service[-1]:.3f}s")
+    total = sum(service)
+    cap = len(lps) / total
+    sat_one = fifo_latencies([0.0] * len(lps), service)
+    out = {"lps": len(lps), "config": args.config, "tolerances": "50/50 1e-4 / 1e-6",
+           "nnz_mean": float(np.mean(nnz)), "nnz_max": int(max(nnz)),
+           "one_at_a_time_fp64": {"lps_per_s": cap, "makespan_s": total, "p50_s": pct(sat_one, 50),
+                                  "p99_s": pct(sat_one, 99)}}
+    rng = np.random.default_rng(12345)
+    arrivals = list(np.cumsum(rng.exponential(1.0 / (args.rate_frac * cap), len(lps))))
+    arrivals = [a - arrivals[0] for a in arrivals]
+    poi_one = fifo_latencies(arrivals, service)
+    out["one_at_a_time_fp64"]["poisson"] = {"rate_lps_per_s": args.rate_frac * cap, "p50_s": pct(poi_one, 50),
+                                            "p99_s": pct(poi_one, 99)}
+    for prec in ("fp64", "mixed"):
+        pr = int(prec == "mixed")
+        mk, lat, res = run_served(b, lps, lambda i: default_params(tolerance=tols[i], precision=pr))
+        st = {}
+        for r in res:
+            st[r.status] = st.get(r.status, 0) + 1
+        entry = {"lps_per_s": len(lps) / mk, "makespan_s": mk, "p50_s": pct(lat, 50), "p99_s": pct(lat, 99),
+                 "statuses": st, "speedup_vs_one_at_a_time": (len(lps) / mk) / cap}
+        if pr:
+            gaps = [abs(r.objective - o.objective) / (1 + abs(o.objective)) for r, o in zip(res, one)
+                    if r.status == "Optimal" and o.status == "Optimal"]
+            kkt = [max(r.primal_residual, r.dual_residual, r.gap) for r in res if r.status == "Optimal"]
+            entry["objective_gap_vs_fp64"] = {"max": max(gaps) if gaps else None, "median": pct(gaps, 50)}
+            entry["fp64_fallback_lps"] = sum(1 for r in res if r.mixed_fp64_iterations > 0)
+            entry["per_instance"] = [{"i": i, "status": r.status, "iterations": r.iterations,
+                                      "fp64_iterations": r.iterations if o is None else o.iterations,
+                                      "objective_gap": abs(r.objective - o.objective) / (1 + abs(o.objective)),
+                                      "primal_res": r.primal_residual, "dual_res": r.dual_residual, "gap": r.gap}
+                                     for i, (r, o) in enumerate(zip(res, one))]
+        log(f"served {prec}: {entry['lps_per_s']:.3f} LPs/s (x{entry['speedup_vs_one_at_a_time']:.2f}), "
+            f"p50 {entry['p50_s']:.3f}s p99 {entry['p99_s']:.3f}s")
+        if args.poisson and pr:
+            _, plat, _ = run_served(b, lps, lambda i: default_params(tolerance=tols[i], precision=pr), arrivals)
+            entry["poisson"] = {"rate_lps_per_s": args.rate_frac * cap, "p50_s": pct(plat, 50),
+                                "p99_s": pct(plat, 99)}
+            log(f"  poisson at {args.rate_frac} x capacity: p50 {pct(plat, 50):.3f}s p99 {pct(plat, 99):.3f}s "
+                f"(one at a time: p50 {pct(poi_one, 50):.3f}s p99 {pct(poi_one, 99):.3f}s)")
+        out["served_" + prec] = entry
+    b.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +214,13 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--probe", choices=["copy", "triad", "gather"],
                     help="run only a bandwidth probe (copy: 2^27 doubles, the HBM counter calibration)")
+    ap.add_argument("--table", type=int, default=900_000, help="gather probe: table length in doubles")
+    ap.add_argument("--serving", action="store_true",
+                    help="serving stream instead: LPs/s and latency, served (FP64 and mixed) vs one at a time")
+    ap.add_argument("--config", type=int, default=3, choices=[3, 4], help="serving stream (3, or 4 with large LPs)")
+    ap.add_argument("--lps", type=int, default=64, help="serving: LPs in the sample")
+    ap.add_argument("--poisson", action="store_true", help="serving: also Poisson arrivals (mixed precision)")
+    ap.add_argument("--rate-frac", type=float, default=0.8, help="serving: arrival rate / one-at-a-time capacity")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_config1.txt"),
                     help="measured HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE summary)")
     args = ap.parse_args()
@@ -108,10 +232,22 @@ def main() -> int:
     from saddlepoint_amd import generate
     from saddlepoint_amd.backend import HipBackend, Solver, default_params
 
+    if args.serving:
+        r = serving_bench(args)
+        sm = r["served_mixed"]
+        print(json.dumps({"metric": METRIC, "value": round(sm["lps_per_s"], 4), "unit": "LPs/s", "n_gpus": 1,
+                          "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+                          "data": "synthetic (seeded serving-stream generator)",
+                          "config": {"workload": f"config{args.config} serving stream sample: {args.lps} LPs, rows "
+                                                 f"log-uniform 1k-200k, tolerance 50/50 1e-4/1e-6, all submitted "
+                                                 f"at t=0", "parallelism": "single GPU"},
+                          "serving": r}), flush=True)
+        return 0
+
     if args.probe:
         b = HipBackend(0)
         n = 1 << 27
-        table = 900_000 if args.probe == "gather" else 0
+        table = args.table if args.probe == "gather" else 0
         ms = b.probe(args.probe, n, table, reps=10)
         per = {"copy": 16, "triad": 24, "gather": 12}[args.probe]
         print(json.dumps({"probe": args.probe, "elements": n, "table": table, "ms_per_launch": ms,

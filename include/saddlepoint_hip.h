@@ -153,6 +153,8 @@ typedef struct {
                              matrix values, iterates and SpMV sums are FP32, while the Halpern
                              anchors, every check (KKT, restart residuals, PID, infeasibility)
                              and the returned solution stay FP64 */
+    int spmv_autotune;    /* 0: every matrix gets the default SpMV plan; 1: the autotuned rule
+                             (sp_spmv_tuning_for) picks each matrix's plan from its row lengths */
 } sp_params;
 
 /* Status (solver.rs:95-102). */
@@ -209,6 +211,31 @@ int sp_backend_profile_read(sp_backend *b, int index, char *name, int name_len, 
 int sp_probe(sp_backend *b, int kind, int64_t n, int64_t table_len, int reps, double *ms_per_launch);
 /* Waits for all work queued on the backend's stream. */
 int sp_backend_sync(sp_backend *b);
+
+/* ---- SpMV plans and the autotuner ------------------------------------------------ */
+
+/* No counterpart in the reference (cuSPARSE plans its own SpMV). How one
+ * matrix's SpMV is cut into workgroups: stream workgroups take consecutive
+ * rows with at most block_nnz nonzeros and max_rows rows; a row longer than
+ * long_row gets workgroups of its own (4096 nonzeros each). The plan fixes
+ * the summation order of rows that span several threads, so a matrix's plan
+ * must be the same wherever its LP runs. */
+typedef struct {
+    int block_nnz; /* 1..2048 (default 2048) */
+    int long_row;  /* 1..block_nnz (default 2048) */
+    int max_rows;  /* 1..1024 (default 1024) */
+} sp_spmv_tuning;
+void sp_spmv_tuning_default(sp_spmv_tuning *out);
+/* The plan the autotuned rule picks for a matrix with these row pointers (a
+ * deterministic function of the row lengths; the rule table is generated by
+ * saddlepoint_amd/autotune.py from timings of candidate plans). */
+void sp_spmv_tuning_for(const int64_t *row_ptr, int64_t nrows, sp_spmv_tuning *out);
+/* sp_problem_load with explicit plans for A' and A'ᵀ (NULL: the default). */
+int sp_problem_load_tuned(sp_backend *b, const sp_scaled_view *p, const sp_spmv_tuning *a,
+                          const sp_spmv_tuning *at, sp_problem **out);
+/* Average milliseconds of one spmv (transpose = 0) or spmv_t (1) of a loaded
+ * problem over `reps` launches, timed with HIP events on the backend's stream. */
+int sp_spmv_time(sp_backend *b, const sp_problem *p, int transpose, int reps, double *ms);
 
 /* ---- serving: continuous batching of many LPs on one GPU ---------------------- */
 

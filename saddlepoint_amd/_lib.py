@@ -63,7 +63,7 @@ class Params(C.Structure):
                 ("power_iterations", _i64), ("geometric_mean_passes", _i64), ("ruiz_iterations", _i64),
                 ("has_pock_chambolle", C.c_int), ("pock_chambolle_alpha", _d),
                 ("bound_objective_rescaling", C.c_int), ("termination", C.c_int), ("fused", C.c_int),
-                ("log_every_checks", C.c_int), ("precision", C.c_int)]
+                ("log_every_checks", C.c_int), ("precision", C.c_int), ("spmv_autotune", C.c_int)]
 
 
 class Solution(C.Structure):
@@ -79,6 +79,10 @@ class ServerParams(C.Structure):
 
 class JobTimes(C.Structure):
     _fields_ = [("submitted", _d), ("admitted", _d), ("completed", _d)]
+
+
+class SpmvTuning(C.Structure):
+    _fields_ = [("block_nnz", C.c_int), ("long_row", C.c_int), ("max_rows", C.c_int)]
 
 
 # name -> (restype, argtypes): every function include/saddlepoint_hip.h declares.
@@ -120,6 +124,10 @@ SIGNATURES = {
     "sp_backend_profile_read": (C.c_int, [_vp, C.c_int, C.c_char_p, C.c_int, _pd, C.POINTER(_i64)]),
     "sp_backend_sync": (C.c_int, [_vp]),
     "sp_probe": (C.c_int, [_vp, C.c_int, _i64, _i64, C.c_int, _pd]),
+    "sp_spmv_tuning_default": (None, [C.c_void_p]),
+    "sp_spmv_tuning_for": (None, [C.POINTER(_i64), _i64, C.c_void_p]),
+    "sp_problem_load_tuned": (C.c_int, [_vp, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(_vp)]),
+    "sp_spmv_time": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _pd]),
     "sp_default_server_params": (None, [C.POINTER(ServerParams)]),
     "sp_server_create": (C.c_int, [_vp, C.POINTER(ServerParams), C.POINTER(_vp)]),
     "sp_server_free": (None, [_vp]),

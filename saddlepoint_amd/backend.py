@@ -14,7 +14,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from ._lib import (SP_OK, BackendError, CsrView, DualStats, LpView, Params, PrimalStats, RayStats, ScaledView,
-                   Solution, lib)
+                   Solution, SpmvTuning, lib)
 from .problem import CsrMatrix, LpProblem
 
 STATUS = {-1: "Running", 0: "Optimal", 1: "PrimalInfeasible", 2: "DualInfeasible", 3: "IterationLimit", 4: "TimeLimit"}
@@ -137,10 +137,13 @@ class Problem:
         self.h = handle
         self.m, self.n = m, n
 
-    def __del__(self):
+    def free(self):
         if getattr(self, "h", None):
             lib().sp_problem_free(self.h)
             self.h = None
+
+    def __del__(self):
+        self.free()
 
 
 class HipBackend:
@@ -168,14 +171,27 @@ class HipBackend:
     def name(self) -> str:
         return lib().sp_backend_name(self.h).decode()
 
-    def load(self, s: ScaledProblem) -> Problem:
+    def load(self, s: ScaledProblem, tuning_a=None, tuning_at=None) -> Problem:
+        """``Backend::load``; optionally with explicit SpMV plans for A' and
+        A'ᵀ, each a (block_nnz, long_row, max_rows) tuple."""
         keep = _Keep()
         v = ScaledView(keep.csr(s.a), keep.csr(s.at), keep.f(s.c), keep.f(s.lc), keep.f(s.uc), keep.f(s.lv),
                        keep.f(s.uv), keep.f(s.row_scale), keep.f(s.col_scale), float(s.bound_scale),
                        float(s.obj_scale))
         h = C.c_void_p()
-        self._ok(lib().sp_problem_load(self.h, C.byref(v), C.byref(h)))
+        if tuning_a is None and tuning_at is None:
+            self._ok(lib().sp_problem_load(self.h, C.byref(v), C.byref(h)))
+        else:
+            ta = C.byref(SpmvTuning(*tuning_a)) if tuning_a is not None else None
+            tt = C.byref(SpmvTuning(*tuning_at)) if tuning_at is not None else None
+            self._ok(lib().sp_problem_load_tuned(self.h, C.byref(v), ta, tt, C.byref(h)))
         return Problem(self, h, s.a.nrows, s.a.ncols)
+
+    def spmv_time(self, p: Problem, transpose: bool = False, reps: int = 20) -> float:
+        """Average milliseconds of one spmv (or spmv_t) of a loaded problem."""
+        ms = C.c_double()
+        self._ok(lib().sp_spmv_time(self.h, p.h, int(bool(transpose)), int(reps), C.byref(ms)))
+        return ms.value
 
     def upload(self, data) -> Vector:
         a = _f64(data)

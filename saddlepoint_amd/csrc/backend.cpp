@@ -6,6 +6,7 @@
 #include <dlfcn.h>
 
 #include <algorithm>
+#include <memory>
 #include <climits>
 #include <cstring>
 
@@ -103,7 +104,7 @@ void need_len(const sp_vec* v, int64_t len, const char* what) {
 
 // Uploads one CSR matrix with int32 indices and builds its SpMV plan.
 void upload_csr(sp_backend* b, DevCsr& d, const int64_t* row_ptr, const int64_t* col_idx, const double* values,
-                int64_t nrows, int64_t ncols, int64_t nnz) {
+                int64_t nrows, int64_t ncols, int64_t nnz, const SpmvTuning& tuning) {
     if (nrows > INT_MAX - 1 || ncols > INT_MAX - 1 || nnz > INT_MAX)
         throw Error(SP_ERR_TOO_LARGE, "problem too large for 32-bit sparse indices");
     d.nrows = (int)nrows;
@@ -115,6 +116,7 @@ void upload_csr(sp_backend* b, DevCsr& d, const int64_t* row_ptr, const int64_t*
     d.rp = dupload(b->stream, rp.data(), rp.size());
     d.ci = dupload(b->stream, ci.data(), ci.size());
     d.va = dupload(b->stream, values, (size_t)nnz);
+    d.tuning = tuning;
     SpmvPlan plan = plan_spmv(row_ptr, nrows, d.tuning);
     d.nblocks = (int)plan.desc.size();
     d.desc = dupload(b->stream, plan.desc.data(), plan.desc.size());
@@ -212,7 +214,7 @@ void free_vec(sp_vec* v) {
     delete v;
 }
 
-sp_problem* Hip::load(const ScaledProblem& s) {
+sp_problem* Hip::load(const ScaledProblem& s, bool autotune) {
     sp_scaled_view v{};
     v.a = {s.a.nrows, s.a.ncols, s.a.nnz(), s.a.row_ptr.data(), s.a.col_idx.data(), s.a.values.data()};
     v.at = {s.at.nrows, s.at.ncols, s.at.nnz(), s.at.row_ptr.data(), s.at.col_idx.data(), s.at.values.data()};
@@ -225,10 +227,13 @@ sp_problem* Hip::load(const ScaledProblem& s) {
     v.col_scale = s.col_scale.data();
     v.bound_scale = s.bound_scale;
     v.obj_scale = s.obj_scale;
-    return load_view(v);
+    if (!autotune) return load_view(v);
+    const SpmvTuning ta = autotuned_plan(s.a.row_ptr.data(), s.a.nrows);
+    const SpmvTuning tat = autotuned_plan(s.at.row_ptr.data(), s.at.nrows);
+    return load_view(v, &ta, &tat);
 }
 
-sp_problem* Hip::load_view(const sp_scaled_view& v) {
+sp_problem* Hip::load_view(const sp_scaled_view& v, const SpmvTuning* ta, const SpmvTuning* tat) {
     const int64_t m = v.a.nrows, n = v.a.ncols;
     if (v.at.nrows != n || v.at.ncols != m || v.at.nnz != v.a.nnz)
         throw Error(SP_ERR_SHAPE, "load: A'ᵀ does not match A'");
@@ -237,8 +242,8 @@ sp_problem* Hip::load_view(const sp_scaled_view& v) {
     p->m = m;
     p->n = n;
     try {
-        upload_csr(b, p->a, v.a.row_ptr, v.a.col_idx, v.a.values, m, n, v.a.nnz);
-        upload_csr(b, p->at, v.at.row_ptr, v.at.col_idx, v.at.values, n, m, v.at.nnz);
+        upload_csr(b, p->a, v.a.row_ptr, v.a.col_idx, v.a.values, m, n, v.a.nnz, ta ? *ta : SpmvTuning{});
+        upload_csr(b, p->at, v.at.row_ptr, v.at.col_idx, v.at.values, n, m, v.at.nnz, tat ? *tat : SpmvTuning{});
         // Residual factors are formed on the host with the reference's
         // arithmetic (scaling.rs:124-131) and uploaded, not recomputed.
         std::vector<double> pf((size_t)m), df((size_t)n);
@@ -422,6 +427,27 @@ double Hip::probe(int kind, int64_t n, int64_t table_len, int reps, uint64_t see
 }
 
 void Hip::sync() { hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize"); }
+
+double Hip::spmv_time(const sp_problem* p, bool transpose, int reps) {
+    if (!p || reps < 1) throw Error(SP_ERR_INVALID, "spmv_time: bad argument");
+    const DevCsr& M = transpose ? p->at : p->a;
+    std::unique_ptr<sp_vec, void (*)(sp_vec*)> x(zeros(M.ncols), free_vec), out(zeros(M.nrows), free_vec);
+    if (M.ncols) launch(b, b->k_fill, ew_grid(M.ncols), x->d, 1.0, (long long)M.ncols);
+    spmv_launch(b, b->k_spmv, b->k_fin, M, M.va, x->d, out->d);  // warm-up
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms = 0.0f;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    hipError_t err = hipEventRecord(e0, b->stream);
+    for (int r = 0; r < reps && err == hipSuccess; ++r) spmv_launch(b, b->k_spmv, b->k_fin, M, M.va, x->d, out->d);
+    if (err == hipSuccess) err = hipEventRecord(e1, b->stream);
+    if (err == hipSuccess) err = hipEventSynchronize(e1);
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    hip_check(err, "spmv_time events");
+    return (double)ms / reps;
+}
 
 // ---- SpMV --------------------------------------------------------------------
 

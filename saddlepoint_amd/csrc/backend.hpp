@@ -59,6 +59,9 @@ struct SpmvPlan {
     int npartials = 0;
 };
 SpmvPlan plan_spmv(const int64_t* row_ptr, int64_t nrows, const SpmvTuning& t);
+// The autotuned rule: a plan chosen from the matrix's row-length features by
+// the table that the tuning sweep generated (autotune.cpp).
+SpmvTuning autotuned_plan(const int64_t* row_ptr, int64_t nrows);
 
 }  // namespace sp
 
@@ -143,8 +146,10 @@ struct Hip {
     static constexpr int kSlots = 64;
 
     std::string name() const { return b->name; }
-    sp_problem* load(const ScaledProblem& s);
-    sp_problem* load_view(const sp_scaled_view& v);
+    sp_problem* load(const ScaledProblem& s, bool autotune = false);
+    sp_problem* load_view(const sp_scaled_view& v, const SpmvTuning* ta = nullptr, const SpmvTuning* tat = nullptr);
+    // Average ms of one SpMV (A' or, with transpose, A'ᵀ) over reps launches.
+    double spmv_time(const sp_problem* p, bool transpose, int reps);
     sp_vec* upload(const double* data, int64_t len);
     sp_vec* zeros(int64_t len);
     void download(const sp_vec* v, double* out, int64_t len);

@@ -281,6 +281,7 @@ void sp_default_params(sp_params* p) {
     p->fused = 1;
     p->log_every_checks = 0;
     p->precision = 0;
+    p->spmv_autotune = 0;
 }
 
 int sp_solve(sp_backend* b, const sp_lp_view* lp, const sp_params* params, sp_solution* out, double* x_out,
@@ -376,6 +377,47 @@ int sp_probe(sp_backend* b, int kind, int64_t n, int64_t table_len, int reps, do
 
 int sp_backend_sync(sp_backend* b) {
     return guarded(b, [&] { sp::Hip{b}.sync(); });
+}
+
+namespace {
+sp_spmv_tuning to_c(const sp::SpmvTuning& t) { return sp_spmv_tuning{t.block_nnz, t.long_row, t.max_rows}; }
+sp::SpmvTuning from_c(const sp_spmv_tuning& t) {
+    if (t.block_nnz < 1 || t.block_nnz > 2048 || t.long_row < 1 || t.long_row > t.block_nnz || t.max_rows < 1 ||
+        t.max_rows > 1024)
+        throw sp::Error(SP_ERR_INVALID, "spmv tuning out of range");
+    sp::SpmvTuning r;
+    r.block_nnz = t.block_nnz;
+    r.long_row = t.long_row;
+    r.max_rows = t.max_rows;
+    return r;
+}
+}  // namespace
+
+void sp_spmv_tuning_default(sp_spmv_tuning* out) {
+    if (out) *out = to_c(sp::SpmvTuning{});
+}
+
+void sp_spmv_tuning_for(const int64_t* row_ptr, int64_t nrows, sp_spmv_tuning* out) {
+    if (!out) return;
+    *out = (row_ptr && nrows >= 0) ? to_c(sp::autotuned_plan(row_ptr, nrows)) : to_c(sp::SpmvTuning{});
+}
+
+int sp_problem_load_tuned(sp_backend* b, const sp_scaled_view* p, const sp_spmv_tuning* a, const sp_spmv_tuning* at,
+                          sp_problem** out) {
+    return guarded(b, [&] {
+        if (!p || !out) throw sp::Error(SP_ERR_INVALID, "load: null argument");
+        *out = nullptr;
+        const sp::SpmvTuning ta = a ? from_c(*a) : sp::SpmvTuning{};
+        const sp::SpmvTuning tat = at ? from_c(*at) : sp::SpmvTuning{};
+        *out = sp::Hip{b}.load_view(*p, &ta, &tat);
+    });
+}
+
+int sp_spmv_time(sp_backend* b, const sp_problem* p, int transpose, int reps, double* ms) {
+    return guarded(b, [&] {
+        if (!p || !ms) throw sp::Error(SP_ERR_INVALID, "spmv_time: null argument");
+        *ms = sp::Hip{b}.spmv_time(p, transpose != 0, reps);
+    });
 }
 
 int sp_scale_problem(const sp_lp_view* lp, const sp_params* params, double* a_values, double* c, double* lc,

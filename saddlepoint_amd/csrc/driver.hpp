@@ -27,7 +27,7 @@ struct DriverBackend : Hip {
     using Problem = std::unique_ptr<sp_problem, ProbDel>;
     using Vector32 = std::unique_ptr<sp_vec32, Vec32Del>;
     using HostBuffer = std::unique_ptr<double, HostDel>;
-    Problem load(const ScaledProblem& s) { return Problem(Hip::load(s)); }
+    Problem load(const ScaledProblem& s, bool autotune = false) { return Problem(Hip::load(s, autotune)); }
     Vector zeros(int64_t n) { return Vector(Hip::zeros(n)); }
     Vector upload(const double* d, int64_t n) { return Vector(Hip::upload(d, n)); }
     Vector32 zeros32(int64_t n) { return Vector32(Hip::zeros32(n)); }

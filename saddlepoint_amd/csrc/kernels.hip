@@ -815,10 +815,28 @@ extern "C" __global__ __launch_bounds__(WG) void sp_probe_triad(const double* __
 }
 
 // out[i] = table[idx[i]]: streams idx and out, gathers from table.
+// Eight independent gathers in flight per thread, as in the SpMV's stream
+// blocks, so the rate measured is the memory system's, not one load's latency.
 extern "C" __global__ __launch_bounds__(WG) void sp_probe_gather(const int* __restrict__ idx,
                                                                  const double* __restrict__ table,
                                                                  double* __restrict__ out, i64 n) {
-    GRID_LOOP(i, n) { out[i] = table[__builtin_nontemporal_load(idx + i)]; }
+    const i64 stride = (i64)gridDim.x * WG;
+    for (i64 base = (i64)blockIdx.x * WG + threadIdx.x; base < n; base += 8 * stride) {
+        int c[8];
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const i64 i = base + u * stride;
+            c[u] = i < n ? __builtin_nontemporal_load(idx + i) : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = table[c[u]];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const i64 i = base + u * stride;
+            if (i < n) out[i] = v[u];
+        }
+    }
 }
 
 // ---- batched plain iterations for the serving loop --------------------------------

@@ -80,7 +80,7 @@ class Solver {
           maximize_(problem.maximize) {
         b_norm_ = finite_bound_norm(problem.lc, problem.uc, problem.m);
         c_norm_ = norm2(problem.c, problem.n);
-        dev_ = b_.load(scaled_);
+        dev_ = b_.load(scaled_, params.spmv_autotune != 0);
         const int64_t m = m_, n = n_;
         x_ = b_.zeros(n), y_ = b_.zeros(m), x_t_ = b_.zeros(n), y_t_ = b_.zeros(m), x_bar_ = b_.zeros(n);
         x_anchor_ = b_.zeros(n), y_anchor_ = b_.zeros(m), ax_ = b_.zeros(m), aty_ = b_.zeros(n);

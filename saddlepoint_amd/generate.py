@@ -157,6 +157,21 @@ def serving_lp(seed: int, m_lo: int = 1000, m_hi: int = 200_000, zipf_rows: floa
     return _finish(rng, a, eq_frac=0.6, name=f"serving-{seed}")
 
 
+def serving_stream(config: int, count: int, start: int = 0):
+    """Instances start .. start+count-1 of a serving stream: config 3 draws
+    every LP from ``serving_lp``; config 4 makes 10% of them (a seeded draw per
+    instance) large, with 200k-1M rows and Zipf(1.5) row lengths."""
+    out = []
+    for i in range(start, start + count):
+        seed = seed_of(config, i)
+        large = config == 4 and np.random.default_rng(seed ^ 0x5EED).random() < 0.1
+        if large:
+            out.append(serving_lp(seed, 200_000, 1_000_000, zipf_rows=1.5))
+        else:
+            out.append(serving_lp(seed))
+    return out
+
+
 def kat_lp(seed: int, m: int = 300, n: int = 500, per_col: int = 4):
     """An LP with a closed-form optimum; returns (problem, optimal objective).
 

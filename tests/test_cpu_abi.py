@@ -148,3 +148,27 @@ def test_from_triplets_sums_duplicates_and_drops_zeros():
 def test_netlib_sizes(name, m, n, nnz):
     p = netlib(name)
     assert (p.nrows, p.ncols, p.a.nnz) == (m, n, nnz)
+
+
+def test_autotuned_plan_rule_is_deterministic_and_in_range():
+    import ctypes as C
+
+    from saddlepoint_amd._lib import SpmvTuning, lib
+
+    def plan(rp):
+        t = SpmvTuning()
+        rp = np.ascontiguousarray(rp, dtype=np.int64)
+        lib().sp_spmv_tuning_for(rp.ctypes.data_as(C.POINTER(C.c_int64)), len(rp) - 1, C.byref(t))
+        return t.block_nnz, t.long_row, t.max_rows
+
+    d = SpmvTuning()
+    lib().sp_spmv_tuning_default(C.byref(d))
+    assert (d.block_nnz, d.long_row, d.max_rows) == (2048, 2048, 1024)
+    rng = np.random.default_rng(3)
+    for rows in (1, 10, 2000, 50_000):
+        for mean in (2, 17, 60, 300):
+            rp = np.concatenate([[0], np.cumsum(rng.integers(0, 2 * mean, rows))])
+            a = plan(rp)
+            assert a == plan(rp.copy())
+            assert 1 <= a[0] <= 2048 and 1 <= a[1] <= a[0] and 1 <= a[2] <= 1024
+    assert plan(np.zeros(5, np.int64)) == (2048, 2048, 1024)  # no nonzeros: the default

@@ -55,11 +55,11 @@ def scaled_pair(p):
     return so, sg
 
 
-def compare(name, p, gpu):
-    print(f"ops parity: {name} m={p.nrows} n={p.ncols} nnz={p.a.nnz}", flush=True)
+def compare(name, p, gpu, tuning=None):
+    print(f"ops parity: {name} m={p.nrows} n={p.ncols} nnz={p.a.nnz} plan={tuning}", flush=True)
     cpu, sp = scaled_pair(p)
     m, n = p.nrows, p.ncols
-    pg = gpu.load(sp)
+    pg = gpu.load(sp) if tuning is None else gpu.load(sp, tuning_a=tuning, tuning_at=tuning)
     up, down = gpu.upload, gpu.download
     x, y, xa, ya = vector(n, 1), vector(m, 2), vector(n, 3), vector(m, 4)
     gx, gy, gxa, gya = up(x), up(y), up(xa), up(ya)
@@ -159,6 +159,14 @@ def test_backend_ops_match_cpu_very_long_rows(gpu):
     p = generate._finish(rng, a, eq_frac=0.6)
     assert np.diff(p.a.row_ptr).max() > 4096
     compare("very-long-rows", p, gpu)
+
+
+@pytest.mark.parametrize("tuning", [(256, 256, 256), (512, 64, 1024), (1024, 1024, 37), (2048, 100, 1024)])
+def test_backend_ops_match_cpu_under_other_plans(gpu, tuning):
+    # Every SpMV plan the autotuner may pick (smaller stream blocks, rows
+    # split off as long rows, few rows per block) computes the same products.
+    compare("israel", netlib("israel"), gpu, tuning)
+    compare("config1@0.05", generate.config1(scale=0.05), gpu, tuning)
 
 
 def test_spmv_is_deterministic(gpu):

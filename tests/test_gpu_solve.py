@@ -28,7 +28,8 @@ def iterations_match(a: int, b: int) -> bool:
 
 def run_pair(gpu, name, p, **kw):
     params = default_params(tolerance=TOL, **kw)
-    c, _, _ = O.solve(p, O.default_params(tolerance=TOL, **{k: v for k, v in kw.items() if k != "fused"}))
+    c, _, _ = O.solve(p, O.default_params(tolerance=TOL, **{k: v for k, v in kw.items()
+                                                             if k not in ("fused", "spmv_autotune")}))
     g = solve(gpu, p, params)
     print(f"{name:>12}: cpu {O.STATUS[c.status]} {c.iterations} it; gpu {g.status} {g.iterations} it "
           f"{g.solve_time:.3f}s", flush=True)
@@ -130,3 +131,10 @@ def test_mixed_precision_needs_fused(gpu):
 
     with pytest.raises(BackendError):
         solve(gpu, netlib("afiro"), default_params(precision=1, fused=0))
+
+
+def test_autotuned_plans_solve_like_the_oracle(gpu):
+    """spmv_autotune = 1: each matrix gets the rule's plan (config0's A' takes
+    a 1024-nonzero plan); the solve still matches the oracle."""
+    run_pair(gpu, "config0", generate.config0(), spmv_autotune=1)
+    run_pair(gpu, "israel", netlib("israel"), spmv_autotune=1)
