@@ -155,6 +155,9 @@ typedef struct {
                              and the returned solution stay FP64 */
     int spmv_autotune;    /* 0: every matrix gets the default SpMV plan; 1: the autotuned rule
                              (sp_spmv_tuning_for) picks each matrix's plan from its row lengths */
+    int gpu_scaling;      /* 0: ScaledProblem::new on the host; 1: A and its transpose are uploaded
+                             unscaled and the scaling passes run on the device (bitwise the host's
+                             result where the sums add in the host's order) */
 } sp_params;
 
 /* Status (solver.rs:95-102). */
@@ -211,6 +214,21 @@ int sp_backend_profile_read(sp_backend *b, int index, char *name, int name_len, 
 int sp_probe(sp_backend *b, int kind, int64_t n, int64_t table_len, int reps, double *ms_per_launch);
 /* Waits for all work queued on the backend's stream. */
 int sp_backend_sync(sp_backend *b);
+
+/* ---- the KKT checker ---------------------------------------------------------------- */
+
+/* KktReport (crates/saddlepoint-check/src/lib.rs:10-25). */
+typedef struct {
+    double primal_objective, dual_objective;
+    double constraint_violation, bound_violation, dual_violation;
+    double relative_primal, relative_dual, relative_gap;
+} sp_kkt_report;
+/* check() (check/src/lib.rs:62-118) on the device: recomputes Ax and Aᵀy of the
+ * original (unscaled) LP, which it uploads itself, and the violations, the
+ * dual objective and the relative measures of a claimed (x[n], y[m]), with y
+ * in the solver's sign convention. Shares nothing with a solve but the LP.
+ * Sums are in a fixed tree order (not the reference's serial order). */
+int sp_check(sp_backend *b, const sp_lp_view *lp, const double *x, const double *y, sp_kkt_report *out);
 
 /* ---- SpMV plans and the autotuner ------------------------------------------------ */
 
@@ -279,6 +297,12 @@ double sp_server_clock(const sp_server *s);
 int sp_scale_problem(const sp_lp_view *lp, const sp_params *params, double *a_values, double *c, double *lc,
                      double *uc, double *lv, double *uv, double *row_scale, double *col_scale,
                      double *bound_scale, double *obj_scale);
+/* The same on the device (the scaling passes of sp_params.gpu_scaling = 1),
+ * read back into the same caller-allocated arrays; also at_values[nnz], the
+ * scaled A'ᵀ in the transpose's sparsity. */
+int sp_scale_problem_gpu(sp_backend *b, const sp_lp_view *lp, const sp_params *params, double *a_values,
+                         double *at_values, double *c, double *lc, double *uc, double *lv, double *uv,
+                         double *row_scale, double *col_scale, double *bound_scale, double *obj_scale);
 
 #ifdef __cplusplus
 }

@@ -63,7 +63,8 @@ class Params(C.Structure):
                 ("power_iterations", _i64), ("geometric_mean_passes", _i64), ("ruiz_iterations", _i64),
                 ("has_pock_chambolle", C.c_int), ("pock_chambolle_alpha", _d),
                 ("bound_objective_rescaling", C.c_int), ("termination", C.c_int), ("fused", C.c_int),
-                ("log_every_checks", C.c_int), ("precision", C.c_int), ("spmv_autotune", C.c_int)]
+                ("log_every_checks", C.c_int), ("precision", C.c_int), ("spmv_autotune", C.c_int),
+                ("gpu_scaling", C.c_int)]
 
 
 class Solution(C.Structure):
@@ -83,6 +84,16 @@ class JobTimes(C.Structure):
 
 class SpmvTuning(C.Structure):
     _fields_ = [("block_nnz", C.c_int), ("long_row", C.c_int), ("max_rows", C.c_int)]
+
+
+class KktReport(C.Structure):
+    """sp_kkt_report (check/src/lib.rs:10-25)."""
+    _fields_ = [("primal_objective", _d), ("dual_objective", _d), ("constraint_violation", _d),
+                ("bound_violation", _d), ("dual_violation", _d), ("relative_primal", _d), ("relative_dual", _d),
+                ("relative_gap", _d)]
+
+    def max_relative(self) -> float:
+        return max(self.relative_primal, self.relative_dual, self.relative_gap)
 
 
 # name -> (restype, argtypes): every function include/saddlepoint_hip.h declares.
@@ -139,6 +150,9 @@ SIGNATURES = {
     "sp_server_clock": (_d, [_vp]),
     "sp_scale_problem": (C.c_int, [C.POINTER(LpView), C.POINTER(Params), _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd,
                                    _pd, _pd]),
+    "sp_check": (C.c_int, [_vp, C.POINTER(LpView), _pd, _pd, C.c_void_p]),
+    "sp_scale_problem_gpu": (C.c_int, [_vp, C.POINTER(LpView), C.POINTER(Params), _pd, _pd, _pd, _pd, _pd, _pd, _pd,
+                                       _pd, _pd, _pd, _pd]),
 }
 
 _lib = None

@@ -13,8 +13,8 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import (SP_OK, BackendError, CsrView, DualStats, LpView, Params, PrimalStats, RayStats, ScaledView,
-                   Solution, SpmvTuning, lib)
+from ._lib import (SP_OK, BackendError, CsrView, DualStats, KktReport, LpView, Params, PrimalStats, RayStats,
+                   ScaledView, Solution, SpmvTuning, lib)
 from .problem import CsrMatrix, LpProblem
 
 STATUS = {-1: "Running", 0: "Optimal", 1: "PrimalInfeasible", 2: "DualInfeasible", 3: "IterationLimit", 4: "TimeLimit"}
@@ -96,6 +96,27 @@ class ScaledProblem:
             raise BackendError(rc, lib().sp_last_error(None).decode())
         a = CsrMatrix(m, n, p.a.row_ptr.copy(), p.a.col_idx.copy(), vals)
         return ScaledProblem(a=a, at=a.transpose(), bound_scale=bs.value, obj_scale=os_.value, **out)
+
+    @staticmethod
+    def new_on_device(backend: "HipBackend", p: LpProblem, params: Params | None = None) -> "ScaledProblem":
+        """The same scaled problem, computed by the device scaling passes
+        (``sp_scale_problem_gpu``)."""
+        params = params or default_params()
+        v, keep = lp_view(p)
+        m, n = p.nrows, p.ncols
+        out = {k: np.zeros(n) for k in ("c", "lv", "uv", "col_scale")}
+        out.update({k: np.zeros(m) for k in ("lc", "uc", "row_scale")})
+        vals, tvals = np.zeros(p.a.nnz), np.zeros(p.a.nnz)
+        bs, os_ = C.c_double(), C.c_double()
+        ptr = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+        backend._ok(lib().sp_scale_problem_gpu(backend.h, C.byref(v), C.byref(params), ptr(vals), ptr(tvals),
+                                               ptr(out["c"]), ptr(out["lc"]), ptr(out["uc"]), ptr(out["lv"]),
+                                               ptr(out["uv"]), ptr(out["row_scale"]), ptr(out["col_scale"]),
+                                               C.byref(bs), C.byref(os_)))
+        a = CsrMatrix(m, n, p.a.row_ptr.copy(), p.a.col_idx.copy(), vals)
+        at = a.transpose()
+        at.values = tvals
+        return ScaledProblem(a=a, at=at, bound_scale=bs.value, obj_scale=os_.value, **out)
 
     def primal_residual_factors(self):  # scaling.rs:124-126
         return 1.0 / (self.row_scale * self.bound_scale)
@@ -319,6 +340,19 @@ class SolveResult:
     setup_time: float
     final_omega: float
     mixed_fp64_iterations: int = 0
+
+
+def check(backend: HipBackend, p: LpProblem, x, y) -> KktReport:
+    """``saddlepoint_check::check`` (check/src/lib.rs:62-118) on the GPU."""
+    v, keep = lp_view(p)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    if x.size != p.ncols or y.size != p.nrows:
+        raise ValueError("check: x and y must match the LP's columns and rows")
+    r = KktReport()
+    backend._ok(lib().sp_check(backend.h, C.byref(v), x.ctypes.data_as(C.POINTER(C.c_double)),
+                               y.ctypes.data_as(C.POINTER(C.c_double)), C.byref(r)))
+    return r
 
 
 def solve(backend: HipBackend, p: LpProblem, params: Params | None = None, **kw) -> SolveResult:

@@ -6,6 +6,7 @@
 #include <dlfcn.h>
 
 #include <algorithm>
+#include <cmath>
 #include <memory>
 #include <climits>
 #include <cstring>
@@ -231,6 +232,174 @@ sp_problem* Hip::load(const ScaledProblem& s, bool autotune) {
     const SpmvTuning ta = autotuned_plan(s.a.row_ptr.data(), s.a.nrows);
     const SpmvTuning tat = autotuned_plan(s.at.row_ptr.data(), s.at.nrows);
     return load_view(v, &ta, &tat);
+}
+
+sp_problem* Hip::load_scaled(const LpView& p, const ScalingParams& sp, bool autotune, ScaledProblem& s) {
+    const int64_t m = p.m, n = p.n;
+    Csr t = transpose_of(m, n, p.row_ptr, p.col_idx, p.values);
+    SpmvTuning ta, tat;
+    if (autotune) {
+        ta = autotuned_plan(p.row_ptr, m);
+        tat = autotuned_plan(t.row_ptr.data(), n);
+    }
+    auto* prob = new sp_problem;
+    prob->b = b;
+    prob->m = m;
+    prob->n = n;
+    std::vector<double*> scratch;
+    auto tmp = [&](int64_t len) {
+        scratch.push_back(dmalloc<double>((size_t)std::max<int64_t>(len, 1)));
+        return scratch.back();
+    };
+    try {
+        upload_csr(b, prob->a, p.row_ptr, p.col_idx, p.values, m, n, p.nnz, ta);
+        upload_csr(b, prob->at, t.row_ptr.data(), t.col_idx.data(), t.values.data(), n, m, p.nnz, tat);
+        t = Csr{};
+        DevCsr& A = prob->a;
+        DevCsr& T = prob->at;
+        double *rs = tmp(m), *cs = tmp(n), *r1 = tmp(m), *r2 = tmp(m), *c1 = tmp(n), *c2 = tmp(n), *rf = tmp(m),
+               *cf = tmp(n);
+        const double* none = nullptr;
+        if (m) launch(b, b->k_fill, ew_grid(m), rs, 1.0, (long long)m);
+        if (n) launch(b, b->k_fill, ew_grid(n), cs, 1.0, (long long)n);
+        auto wave_grid = [](int64_t rows) { return (unsigned)std::max<int64_t>(1, (rows * 64 + kWG - 1) / kWG); };
+        auto minmax = [&](const DevCsr& M, double* lo, double* hi) {
+            if (M.nrows) launch(b, b->k_sc_minmax, wave_grid(M.nrows), (const int*)M.rp, (const double*)M.va, M.nrows, lo, hi);
+        };
+        auto powsum = [&](const DevCsr& M, double e, double* out) {
+            if (M.nrows)
+                launch(b, b->k_sc_powsum, (unsigned)((M.nrows + kWG - 1) / kWG), (const int*)M.rp, (const double*)M.va,
+                       M.nrows, e, out);
+        };
+        auto factor = [&](const double* a, const double* bb, double* f, int64_t len) {
+            if (len) launch(b, b->k_sc_factor, ew_grid(len), a, bb, f, (long long)len);
+        };
+        // v ← v·(rowf·colf) on A and, with the roles swapped, on A'ᵀ.
+        auto apply = [&](const double* r, const double* c) {
+            if (A.nrows) launch(b, b->k_sc_apply, wave_grid(A.nrows), (const int*)A.rp, (const int*)A.ci, A.va, A.nrows, r, c);
+            if (T.nrows) launch(b, b->k_sc_apply, wave_grid(T.nrows), (const int*)T.rp, (const int*)T.ci, T.va, T.nrows, c, r);
+        };
+        auto mul = [&](double* v, const double* f, int64_t len) {
+            if (len) launch(b, b->k_sc_mul, ew_grid(len), v, f, (long long)len);
+        };
+        // scaling.rs:59-70: geometric-mean passes, rows then columns.
+        for (int64_t pass = 0; pass < sp.geometric_mean_passes; ++pass) {
+            minmax(A, r1, r2);
+            factor(r1, r2, rf, m);
+            apply(rf, none);
+            mul(rs, rf, m);
+            minmax(T, c1, c2);
+            factor(c1, c2, cf, n);
+            apply(none, cf);
+            mul(cs, cf, n);
+        }
+        // scaling.rs:71-76: Ruiz by the square root of the line maxima.
+        for (int64_t it = 0; it < sp.ruiz_iterations; ++it) {
+            minmax(A, nullptr, r2);
+            minmax(T, nullptr, c2);
+            factor(r2, none, rf, m);
+            factor(c2, none, cf, n);
+            apply(rf, cf);
+            mul(rs, rf, m);
+            mul(cs, cf, n);
+        }
+        // scaling.rs:77-84: Pock-Chambolle.
+        if (sp.has_pock_chambolle) {
+            powsum(A, sp.pock_chambolle_alpha, r1);
+            factor(r1, none, rf, m);
+            powsum(T, 2.0 - sp.pock_chambolle_alpha, c1);
+            factor(c1, none, cf, n);
+            apply(rf, cf);
+            mul(rs, rf, m);
+            mul(cs, cf, n);
+        }
+        s.row_scale.resize((size_t)m);
+        s.col_scale.resize((size_t)n);
+        if (m) hip_check(hipMemcpyAsync(s.row_scale.data(), rs, 8 * (size_t)m, hipMemcpyDeviceToHost, b->stream), "D2H");
+        if (n) hip_check(hipMemcpyAsync(s.col_scale.data(), cs, 8 * (size_t)n, hipMemcpyDeviceToHost, b->stream), "D2H");
+        hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        for (double* q : scratch) dfree(q);
+        scratch.clear();
+        // The vector part on the host, as ScaledProblem::new does it.
+        s.a = Csr{};
+        s.at = Csr{};
+        s.a.nrows = m, s.a.ncols = n, s.at.nrows = n, s.at.ncols = m;
+        finish_scaling(p, sp, s);
+        std::vector<double> pf((size_t)m), df((size_t)n);
+        for (int64_t i = 0; i < m; ++i) pf[(size_t)i] = 1.0 / (s.row_scale[(size_t)i] * s.bound_scale);
+        for (int64_t j = 0; j < n; ++j) df[(size_t)j] = 1.0 / (s.col_scale[(size_t)j] * s.obj_scale);
+        prob->c = dupload(b->stream, s.c.data(), (size_t)n);
+        prob->lv = dupload(b->stream, s.lv.data(), (size_t)n);
+        prob->uv = dupload(b->stream, s.uv.data(), (size_t)n);
+        prob->lc = dupload(b->stream, s.lc.data(), (size_t)m);
+        prob->uc = dupload(b->stream, s.uc.data(), (size_t)m);
+        prob->pf = dupload(b->stream, pf.data(), (size_t)m);
+        prob->df = dupload(b->stream, df.data(), (size_t)n);
+        hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    } catch (...) {
+        (void)hipStreamSynchronize(b->stream);
+        for (double* q : scratch) dfree(q);
+        free_problem(prob);
+        throw;
+    }
+    return prob;
+}
+
+sp_kkt_report Hip::check(const LpView& p, const double* x, const double* y) {
+    const int64_t m = p.m, n = p.n;
+    if (!x || !y) throw Error(SP_ERR_INVALID, "check: null vector");
+    Csr t = transpose_of(m, n, p.row_ptr, p.col_idx, p.values);
+    DevCsr A, T;
+    std::vector<double*> vecs;
+    auto up = [&](const double* h, int64_t len) {
+        vecs.push_back(dupload(b->stream, h, (size_t)std::max<int64_t>(len, 0)));
+        return vecs.back();
+    };
+    double r[7] = {0, 0, 0, 0, 0, 0, 0};
+    try {
+        upload_csr(b, A, p.row_ptr, p.col_idx, p.values, m, n, p.nnz, SpmvTuning{});
+        upload_csr(b, T, t.row_ptr.data(), t.col_idx.data(), t.values.data(), n, m, p.nnz, SpmvTuning{});
+        t = Csr{};
+        double *dx = up(x, n), *dy = up(y, m), *c = up(p.c, n), *lc = up(p.lc, m), *uc = up(p.uc, m),
+               *lv = up(p.lv, n), *uv = up(p.uv, n);
+        vecs.push_back(dmalloc<double>((size_t)std::max<int64_t>(m, 1)));
+        double* ax = vecs.back();
+        vecs.push_back(dmalloc<double>((size_t)std::max<int64_t>(n, 1)));
+        double* aty = vecs.back();
+        spmv_launch(b, b->k_spmv, b->k_fin, A, A.va, dx, ax);
+        spmv_launch(b, b->k_spmv, b->k_fin, T, T.va, dy, aty);
+        const int grid = reduce_grid(m + n);
+        launch(b, b->k_red_check, (unsigned)grid, (const double*)ax, (const double*)dy, (const double*)lc,
+               (const double*)uc, (long long)m, (const double*)dx, (const double*)aty, (const double*)c,
+               (const double*)lv, (const double*)uv, (long long)n, b->red_partials, b->red_result);
+        if (grid > 1) launch(b, b->k_red_finish7, 1u, (const double*)b->red_partials, grid, b->red_result);
+        hip_check(hipMemcpyAsync(r, b->red_result, sizeof r, hipMemcpyDeviceToHost, b->stream), "D2H");
+        hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    } catch (...) {
+        (void)hipStreamSynchronize(b->stream);
+        free_csr(A);
+        free_csr(T);
+        for (double* v : vecs) dfree(v);
+        throw;
+    }
+    free_csr(A);
+    free_csr(T);
+    for (double* v : vecs) dfree(v);
+    // check/src/lib.rs:76-117 on the sums.
+    sp_kkt_report k;
+    k.primal_objective = r[0] + p.c0;
+    k.dual_objective = r[3] + p.c0;
+    k.constraint_violation = std::sqrt(r[1]);
+    k.bound_violation = std::sqrt(r[2]);
+    k.dual_violation = std::sqrt(r[4]);
+    const double b_norm = std::sqrt(r[5]), c_norm = std::sqrt(r[6]);
+    const double primal = std::sqrt(k.constraint_violation * k.constraint_violation +
+                                    k.bound_violation * k.bound_violation);
+    k.relative_primal = primal / (1.0 + b_norm);
+    k.relative_dual = k.dual_violation / (1.0 + c_norm);
+    k.relative_gap = std::fabs(k.primal_objective - k.dual_objective) /
+                     (1.0 + std::fabs(k.primal_objective) + std::fabs(k.dual_objective));
+    return k;
 }
 
 sp_problem* Hip::load_view(const sp_scaled_view& v, const SpmvTuning* ta, const SpmvTuning* tat) {
@@ -775,6 +944,13 @@ sp_backend* sp_backend_open(int ordinal) {
         get(b->k_cred_pray, "sp_cred_primal_ray_f64");
         get(b->k_cred_dray, "sp_cred_dual_ray_f64");
         get(b->k_cred_finish, "sp_cred_finish");
+        get(b->k_sc_minmax, "sp_sc_minmax");
+        get(b->k_sc_powsum, "sp_sc_powsum");
+        get(b->k_sc_factor, "sp_sc_factor");
+        get(b->k_sc_apply, "sp_sc_apply");
+        get(b->k_sc_mul, "sp_sc_mul");
+        get(b->k_red_check, "sp_red_check_f64");
+        get(b->k_red_finish7, "sp_red_finish7");
         get(b->k_spmv_t_primal, "sp_spmv_t_primal_f64");
         get(b->k_spmv_t_primal_halpern, "sp_spmv_t_primal_halpern_f64");
         get(b->k_spmv_dual, "sp_spmv_dual_f64");

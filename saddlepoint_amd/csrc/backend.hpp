@@ -125,6 +125,7 @@ struct sp_backend {
     hipFunction_t k_cspmv_store, k_cspmv_primal, k_cspmv_dual, k_cfin_store, k_cfin_primal, k_cfin_dual, k_csub,
         k_chalpern, k_cdelta_to, k_cdelta_from, k_ccopy, k_cred_dot, k_cred_dist, k_cred_primal, k_cred_dual,
         k_cred_step, k_cred_pray, k_cred_dray, k_cred_finish;
+    hipFunction_t k_sc_minmax, k_sc_powsum, k_sc_factor, k_sc_apply, k_sc_mul, k_red_check, k_red_finish7;
     hipFunction_t k_spmv, k_spmv_t_primal, k_spmv_t_primal_halpern, k_spmv_dual, k_spmv_dual_halpern;
     hipFunction_t k_primal_step, k_dual_step, k_halpern, k_sub, k_scale, k_fill;
     hipFunction_t k_probe_copy, k_probe_triad, k_probe_gather;
@@ -148,6 +149,12 @@ struct Hip {
     std::string name() const { return b->name; }
     sp_problem* load(const ScaledProblem& s, bool autotune = false);
     sp_problem* load_view(const sp_scaled_view& v, const SpmvTuning* ta = nullptr, const SpmvTuning* tat = nullptr);
+    // ScaledProblem::new on the device (scaling.rs:53-103): uploads A and its
+    // transpose unscaled, runs the scaling passes there, and fills `s` with
+    // the factors and scaled vectors (its matrices stay empty).
+    sp_problem* load_scaled(const LpView& p, const ScalingParams& sp, bool autotune, ScaledProblem& s);
+    // check() of crates/saddlepoint-check on the device, from the original LP.
+    sp_kkt_report check(const LpView& p, const double* x, const double* y);
     // Average ms of one SpMV (A' or, with transpose, A'ᵀ) over reps launches.
     double spmv_time(const sp_problem* p, bool transpose, int reps);
     sp_vec* upload(const double* data, int64_t len);

@@ -60,15 +60,7 @@ sp::LpView view_of(const sp_lp_view* lp) {
     return v;
 }
 
-sp::ScalingParams scaling_of(const sp_params& p) {
-    sp::ScalingParams s;
-    s.geometric_mean_passes = p.geometric_mean_passes;
-    s.ruiz_iterations = p.ruiz_iterations;
-    s.has_pock_chambolle = p.has_pock_chambolle != 0;
-    s.pock_chambolle_alpha = p.pock_chambolle_alpha;
-    s.bound_objective_rescaling = p.bound_objective_rescaling != 0;
-    return s;
-}
+sp::ScalingParams scaling_of(const sp_params& p) { return sp::scaling_params_of(p); }
 
 }  // namespace
 
@@ -282,6 +274,7 @@ void sp_default_params(sp_params* p) {
     p->log_every_checks = 0;
     p->precision = 0;
     p->spmv_autotune = 0;
+    p->gpu_scaling = 0;
 }
 
 int sp_solve(sp_backend* b, const sp_lp_view* lp, const sp_params* params, sp_solution* out, double* x_out,
@@ -292,7 +285,8 @@ int sp_solve(sp_backend* b, const sp_lp_view* lp, const sp_params* params, sp_so
         const auto t0 = std::chrono::steady_clock::now();
         const sp::LpView v = view_of(lp);
         sp::validate(v);
-        sp::ScaledProblem scaled = sp::ScaledProblem::build(v, scaling_of(*params));
+        sp::ScaledProblem scaled =
+            params->gpu_scaling ? sp::ScaledProblem{} : sp::ScaledProblem::build(v, scaling_of(*params));
         DriverBackend db{{b}};
         sp::Solver<DriverBackend> solver(db, v, *params, std::move(scaled));
         const double setup = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -310,9 +304,10 @@ int sp_solver_create(sp_backend* b, const sp_lp_view* lp, const sp_params* param
         const auto t0 = std::chrono::steady_clock::now();
         const sp::LpView v = view_of(lp);
         sp::validate(v);
-        sp::ScaledProblem scaled = sp::ScaledProblem::build(v, scaling_of(*params));
-        const double setup = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        auto* s = new sp_solver(b, v, *params, std::move(scaled), setup);
+        sp::ScaledProblem scaled =
+            params->gpu_scaling ? sp::ScaledProblem{} : sp::ScaledProblem::build(v, scaling_of(*params));
+        auto* s = new sp_solver(b, v, *params, std::move(scaled),
+                                std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
         try {
             s->solver.start();
         } catch (...) {
@@ -446,6 +441,43 @@ int sp_scale_problem(const sp_lp_view* lp, const sp_params* params, double* a_va
         g_create_error = e.what();
         return SP_ERR_INVALID;
     }
+}
+
+int sp_check(sp_backend* b, const sp_lp_view* lp, const double* x, const double* y, sp_kkt_report* out) {
+    return guarded(b, [&] {
+        if (!out) throw sp::Error(SP_ERR_INVALID, "check: null output");
+        const sp::LpView v = view_of(lp);
+        sp::validate(v);
+        *out = sp::Hip{b}.check(v, x, y);
+    });
+}
+
+int sp_scale_problem_gpu(sp_backend* b, const sp_lp_view* lp, const sp_params* params, double* a_values,
+                         double* at_values, double* c, double* lc, double* uc, double* lv, double* uv,
+                         double* row_scale, double* col_scale, double* bound_scale, double* obj_scale) {
+    return guarded(b, [&] {
+        if (!params || !a_values || !at_values || !c || !lc || !uc || !lv || !uv || !row_scale || !col_scale ||
+            !bound_scale || !obj_scale)
+            throw sp::Error(SP_ERR_INVALID, "scale_problem_gpu: null argument");
+        const sp::LpView v = view_of(lp);
+        sp::validate(v);
+        sp::ScaledProblem s;
+        std::unique_ptr<sp_problem, void (*)(sp_problem*)> p(
+            sp::Hip{b}.load_scaled(v, scaling_of(*params), params->spmv_autotune != 0, s), sp::free_problem);
+        if (v.nnz) {
+            sp::hip_check(hipMemcpy(a_values, p->a.va, 8 * (size_t)v.nnz, hipMemcpyDeviceToHost), "D2H");
+            sp::hip_check(hipMemcpy(at_values, p->at.va, 8 * (size_t)v.nnz, hipMemcpyDeviceToHost), "D2H");
+        }
+        std::copy(s.c.begin(), s.c.end(), c);
+        std::copy(s.lc.begin(), s.lc.end(), lc);
+        std::copy(s.uc.begin(), s.uc.end(), uc);
+        std::copy(s.lv.begin(), s.lv.end(), lv);
+        std::copy(s.uv.begin(), s.uv.end(), uv);
+        std::copy(s.row_scale.begin(), s.row_scale.end(), row_scale);
+        std::copy(s.col_scale.begin(), s.col_scale.end(), col_scale);
+        *bound_scale = s.bound_scale;
+        *obj_scale = s.obj_scale;
+    });
 }
 
 }  // extern "C"

@@ -61,6 +61,13 @@ struct ScaledProblem {
     double objective_factor() const { return 1.0 / (bound_scale * obj_scale); }  // :134-136
 };
 
+// The explicit transpose of a CSR matrix given by its arrays (problem.rs:88-109).
+Csr transpose_of(int64_t nrows, int64_t ncols, const int64_t* row_ptr, const int64_t* col_idx, const double* values);
+
+// The vector part of ScaledProblem::new once the matrix scaling is known
+// (scaling.rs:80-103): c, bounds and the bound/objective rescaling.
+void finish_scaling(const LpView& p, const ScalingParams& params, ScaledProblem& s);
+
 double norm2(const double* v, int64_t n);                                   // scaling.rs:186-188
 double finite_bound_norm(const double* lo, const double* hi, int64_t m);    // scaling.rs:191-201
 void validate(const LpView& p);                                             // problem.rs:166-194

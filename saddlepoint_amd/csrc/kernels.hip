@@ -798,6 +798,149 @@ extern "C" __global__ __launch_bounds__(WG) void sp_red_dual_ray_f64(
     reduce_body<5>(m + n, FDualRay{dy, lc, uc, atdy, lv, uv, m}, partials, result);
 }
 
+// ---- preconditioning passes (scaling.rs:53-103, 139-184) ---------------------------
+//
+// The scaling loop runs on the device over A and its transpose A'ᵀ (whose rows
+// are A's columns, in A's row order). Every pass reproduces the host's
+// arithmetic: row and column extrema are exact whatever the order; the
+// Pock-Chambolle sums add a line's terms left to right as the host does (for
+// a column, in row order, which is A'ᵀ's order); a factor is applied as
+// v·(r_i·c_j) to both copies (IEEE products commute), so both stay equal to
+// the host's scaled matrix and its transpose.
+
+__device__ __forceinline__ double inv_sqrt_or_one(double v) {
+    return (v > 0.0 && __builtin_isfinite(v)) ? 1.0 / sqrt(v) : 1.0;
+}
+
+// Per row (one wave per row): lo = min |a|, hi = max |a| (lo = 0 for a row
+// without nonzero magnitude, scaling.rs:159-178). lo may be null (max only).
+extern "C" __global__ __launch_bounds__(WG) void sp_sc_minmax(const int* __restrict__ rp,
+                                                              const double* __restrict__ va, int nrows,
+                                                              double* __restrict__ lo, double* __restrict__ hi) {
+    const int row = (int)((blockIdx.x * WG + threadIdx.x) / WAVE), lane = threadIdx.x % WAVE;
+    if (row >= nrows) return;  // uniform per wave
+    double mn = __builtin_inf(), mx = 0.0;
+    for (int k = rp[row] + lane; k < rp[row + 1]; k += WAVE) {
+        const double v = fabs(va[k]);
+        mn = v < mn ? v : mn;
+        mx = v > mx ? v : mx;
+    }
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        const double a = __shfl_xor(mn, o, WAVE), b = __shfl_xor(mx, o, WAVE);
+        mn = a < mn ? a : mn;
+        mx = b > mx ? b : mx;
+    }
+    if (lane == 0) {
+        hi[row] = mx;
+        if (lo) lo[row] = mx == 0.0 ? 0.0 : mn;
+    }
+}
+
+// Per row (one thread per row, left to right as the host): Σ |a|^e.
+extern "C" __global__ __launch_bounds__(WG) void sp_sc_powsum(const int* __restrict__ rp,
+                                                              const double* __restrict__ va, int nrows, double e,
+                                                              double* __restrict__ out) {
+    const int row = blockIdx.x * WG + threadIdx.x;
+    if (row >= nrows) return;
+    double s = 0.0;
+    const int k0 = rp[row], k1 = rp[row + 1];
+    if (e == 1.0) {
+        for (int k = k0; k < k1; ++k) s = s + fabs(va[k]);
+    } else {
+        for (int k = k0; k < k1; ++k) s = s + pow(fabs(va[k]), e);
+    }
+    out[row] = s;
+}
+
+// f = inv_sqrt(a·b) (geometric mean, b non-null) or inv_sqrt(a).
+extern "C" __global__ __launch_bounds__(WG) void sp_sc_factor(const double* __restrict__ a,
+                                                              const double* __restrict__ b,
+                                                              double* __restrict__ f, i64 n) {
+    GRID_LOOP(i, n) { f[i] = inv_sqrt_or_one(b ? a[i] * b[i] : a[i]); }
+}
+
+// v_k ← v_k·(rf[row]·cf[col_k]) over a CSR matrix (one wave per row); a null
+// factor vector reads as ones.
+extern "C" __global__ __launch_bounds__(WG) void sp_sc_apply(const int* __restrict__ rp, const int* __restrict__ ci,
+                                                             double* __restrict__ va, int nrows,
+                                                             const double* __restrict__ rf,
+                                                             const double* __restrict__ cf) {
+    const int row = (int)((blockIdx.x * WG + threadIdx.x) / WAVE), lane = threadIdx.x % WAVE;
+    if (row >= nrows) return;
+    const double r = rf ? rf[row] : 1.0;
+    for (int k = rp[row] + lane; k < rp[row + 1]; k += WAVE) va[k] = va[k] * (r * (cf ? cf[ci[k]] : 1.0));
+}
+
+// v ← v·f element-wise (the accumulated row and column scales).
+extern "C" __global__ __launch_bounds__(WG) void sp_sc_mul(double* __restrict__ v, const double* __restrict__ f,
+                                                           i64 n) {
+    GRID_LOOP(i, n) { v[i] *= f[i]; }
+}
+
+// ---- the KKT checker (crates/saddlepoint-check/src/lib.rs:62-118) ---------------------
+//
+// One pass over rows then columns of a claimed (x, y) with Ax and Aᵀy of the
+// original matrix: seven sums, [Σ c·x, Σ outside(Ax, lc, uc)², Σ outside(x,
+// lv, uv)², dual objective terms, Σ dual-sign violations², ‖b‖², ‖c‖²].
+
+__device__ __forceinline__ double outside(double v, double lo, double hi) {
+    return v < lo ? lo - v : (v > hi ? v - hi : 0.0);
+}
+
+// check/src/lib.rs:84-90: a positive multiplier needs a finite lower bound,
+// a negative one a finite upper bound; otherwise it is a violation.
+__device__ __forceinline__ void dual_term(double v, double lo, double hi, double& obj, double& viol) {
+    if (v > 0.0) {
+        if (__builtin_isfinite(lo)) obj += v * lo;
+        else viol += v * v;
+    } else if (v < 0.0) {
+        if (__builtin_isfinite(hi)) obj += v * hi;
+        else viol += v * v;
+    }
+}
+
+struct FCheck {
+    const double *ax, *y, *lc, *uc, *x, *aty, *c, *lv, *uv;
+    i64 m;
+    __device__ void operator()(i64 t, double (&s)[7]) const {
+        if (t < m) {
+            const double o = outside(ax[t], lc[t], uc[t]);
+            s[1] += o * o;
+            dual_term(y[t], lc[t], uc[t], s[3], s[4]);
+            const double l = __builtin_isfinite(lc[t]) ? lc[t] : 0.0;
+            const double u = (__builtin_isfinite(uc[t]) && uc[t] != lc[t]) ? uc[t] : 0.0;
+            s[5] += l * l + u * u;
+        } else {
+            const i64 j = t - m;
+            s[0] += c[j] * x[j];
+            const double o = outside(x[j], lv[j], uv[j]);
+            s[2] += o * o;
+            dual_term(c[j] - aty[j], lv[j], uv[j], s[3], s[4]);
+            s[6] += c[j] * c[j];
+        }
+    }
+};
+
+extern "C" __global__ __launch_bounds__(WG) void sp_red_check_f64(
+    const double* __restrict__ ax, const double* __restrict__ y, const double* __restrict__ lc,
+    const double* __restrict__ uc, i64 m, const double* __restrict__ x, const double* __restrict__ aty,
+    const double* __restrict__ c, const double* __restrict__ lv, const double* __restrict__ uv, i64 n, RED_TAIL) {
+    reduce_body<7>(m + n, FCheck{ax, y, lc, uc, x, aty, c, lv, uv, m}, partials, result);
+}
+
+// The per-workgroup partials of sp_red_check_f64, in workgroup order.
+extern "C" __global__ __launch_bounds__(WG) void sp_red_finish7(const double* __restrict__ partials, int G,
+                                                                double* __restrict__ result) {
+    double s[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < G; b += WG)
+#pragma unroll
+        for (int k = 0; k < 7; ++k) s[k] += partials[(i64)b * 7 + k];
+    block_sum<7>(s);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 7; ++k) result[k] = s[k];
+}
+
 // ---- bandwidth probes (the counterpart of the reference's sp-bandwidth) ------------
 //
 // Achievable HBM copy and triad rates, and the rate of random 8-byte gathers

@@ -9,24 +9,27 @@
 
 namespace sp {
 
-Csr Csr::transpose() const {
+Csr transpose_of(int64_t nrows, int64_t ncols, const int64_t* row_ptr, const int64_t* col_idx, const double* values) {
     Csr t;
     t.nrows = ncols;
     t.ncols = nrows;
+    const int64_t nnz = row_ptr[nrows];
     t.row_ptr.assign((size_t)ncols + 1, 0);
-    for (int64_t c : col_idx) t.row_ptr[(size_t)c + 1] += 1;
+    for (int64_t k = 0; k < nnz; ++k) t.row_ptr[(size_t)col_idx[k] + 1] += 1;
     for (int64_t j = 0; j < ncols; ++j) t.row_ptr[(size_t)j + 1] += t.row_ptr[(size_t)j];
     std::vector<int64_t> next(t.row_ptr.begin(), t.row_ptr.end() - 1);
-    t.col_idx.resize(values.size());
-    t.values.resize(values.size());
+    t.col_idx.resize((size_t)nnz);
+    t.values.resize((size_t)nnz);
     for (int64_t i = 0; i < nrows; ++i)
-        for (int64_t k = row_ptr[(size_t)i]; k < row_ptr[(size_t)i + 1]; ++k) {
-            const int64_t q = next[(size_t)col_idx[(size_t)k]]++;
+        for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
+            const int64_t q = next[(size_t)col_idx[k]]++;
             t.col_idx[(size_t)q] = i;
-            t.values[(size_t)q] = values[(size_t)k];
+            t.values[(size_t)q] = values[k];
         }
     return t;
 }
+
+Csr Csr::transpose() const { return transpose_of(nrows, ncols, row_ptr.data(), col_idx.data(), values.data()); }
 
 void Csr::scale(const std::vector<double>& row, const std::vector<double>& col) {
     for (int64_t i = 0; i < nrows; ++i) {
@@ -150,6 +153,13 @@ ScaledProblem ScaledProblem::build(const LpView& p, const ScalingParams& params)
         apply(s.a, s.row_scale, s.col_scale, r, c);
     }
 
+    finish_scaling(p, params, s);
+    s.at = s.a.transpose();
+    return s;
+}
+
+void finish_scaling(const LpView& p, const ScalingParams& params, ScaledProblem& s) {
+    const int64_t m = p.m, n = p.n;
     s.c.resize((size_t)n);
     s.lv.resize((size_t)n);
     s.uv.resize((size_t)n);
@@ -171,8 +181,6 @@ ScaledProblem ScaledProblem::build(const LpView& p, const ScalingParams& params)
             for (double& e : *v) e *= s.bound_scale;
         for (double& e : s.c) e *= s.obj_scale;
     }
-    s.at = s.a.transpose();
-    return s;
 }
 
 std::vector<double> ScaledProblem::unscale_primal(const std::vector<double>& xs) const {

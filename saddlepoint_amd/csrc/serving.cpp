@@ -253,7 +253,9 @@ class Server {
                 s.has_pock_chambolle = j->params.has_pock_chambolle != 0;
                 s.pock_chambolle_alpha = j->params.pock_chambolle_alpha;
                 s.bound_objective_rescaling = j->params.bound_objective_rescaling != 0;
-                j->scaled = std::make_unique<ScaledProblem>(ScaledProblem::build(v, s));
+                // With device scaling, admission scales the LP on the GPU.
+                j->scaled = std::make_unique<ScaledProblem>(j->params.gpu_scaling ? ScaledProblem{}
+                                                                                   : ScaledProblem::build(v, s));
                 std::lock_guard<std::mutex> g(mu_);
                 j->state = kPrepared;
                 ready_.push_back(j);

@@ -22,6 +22,17 @@
 
 namespace sp {
 
+// ScalingParams from the C ABI's parameters (scaling.rs:25-34).
+inline ScalingParams scaling_params_of(const sp_params& p) {
+    ScalingParams s;
+    s.geometric_mean_passes = p.geometric_mean_passes;
+    s.ruiz_iterations = p.ruiz_iterations;
+    s.has_pock_chambolle = p.has_pock_chambolle != 0;
+    s.pock_chambolle_alpha = p.pock_chambolle_alpha;
+    s.bound_objective_rescaling = p.bound_objective_rescaling != 0;
+    return s;
+}
+
 // PidState (solver.rs:445-480).
 struct PidState {
     double integral = 0.0;
@@ -75,12 +86,20 @@ class Solver {
     // Solver::new (solver.rs:185-212): load, then thirteen vectors, zero except
     // c. The LP itself is not kept: only m, n, c0, the sense and the two
     // norms that run() needs.
+    // With params.gpu_scaling, `scaled` is ignored: the backend uploads the LP
+    // unscaled and scales it on the device, and scaled_ receives the factors.
     Solver(B& b, const LpView& problem, const sp_params& params, ScaledProblem scaled)
-        : b_(b), p_(params), scaled_(std::move(scaled)), m_(problem.m), n_(problem.n), c0_(problem.c0),
-          maximize_(problem.maximize) {
+        : b_(b), p_(params), scaled_(std::move(scaled)), m_(problem.m), n_(problem.n), nnz_(problem.nnz),
+          c0_(problem.c0), maximize_(problem.maximize) {
         b_norm_ = finite_bound_norm(problem.lc, problem.uc, problem.m);
         c_norm_ = norm2(problem.c, problem.n);
-        dev_ = b_.load(scaled_, params.spmv_autotune != 0);
+        if (params.gpu_scaling)
+            dev_ = b_.load_scaled(problem, scaling_params_of(params), params.spmv_autotune != 0, scaled_);
+        else
+            dev_ = b_.load(scaled_, params.spmv_autotune != 0);
+        // The scaled matrices live on the device now; the host keeps the factors.
+        scaled_.a = Csr{};
+        scaled_.at = Csr{};
         const int64_t m = m_, n = n_;
         x_ = b_.zeros(n), y_ = b_.zeros(m), x_t_ = b_.zeros(n), y_t_ = b_.zeros(m), x_bar_ = b_.zeros(n);
         x_anchor_ = b_.zeros(n), y_anchor_ = b_.zeros(m), ax_ = b_.zeros(m), aty_ = b_.zeros(n);
@@ -657,7 +676,7 @@ class Solver {
     // Largest singular value of A' by power iteration on A'ᵀA' (solver.rs:215-242).
     double estimate_norm() {
         const int64_t n = n_;
-        if (n == 0 || scaled_.a.nnz() == 0) return 1.0;
+        if (n == 0 || nnz_ == 0) return 1.0;
         std::vector<double> start((size_t)n);
         for (int64_t j = 0; j < n; ++j) start[(size_t)j] = 1.0 + 0.01 * (double)(j % 7);
         Vec v = b_.upload(start.data(), n);
@@ -759,8 +778,8 @@ class Solver {
 
     B& b_;
     const sp_params p_;
-    const ScaledProblem scaled_;
-    int64_t m_, n_;
+    ScaledProblem scaled_;
+    int64_t m_, n_, nnz_;
     double c0_;
     bool maximize_;
     double b_norm_ = 0.0, c_norm_ = 0.0;
