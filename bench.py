@@ -203,6 +203,51 @@ def serving_bench(args) -> dict:
     return out
 
 
+def large_bench(args) -> dict:
+    """The large single instance (config 2), plain FP64 against the mixed
+    precision mode at 1e-4 and 1e-6: iterations, solve time, and each mixed
+    result's objective gap and KKT report (device checker) beside FP64's."""
+    from saddlepoint_amd import generate
+    from saddlepoint_amd.backend import HipBackend, check, default_params, solve
+
+    t0 = time.time()
+    p = generate.config2(scale=args.scale)
+    log(f"generated config2 m={p.nrows} n={p.ncols} nnz={p.a.nnz} in {time.time() - t0:.1f}s")
+    b = HipBackend(0)
+    out = {"m": p.nrows, "n": p.ncols, "nnz": p.a.nnz, "scale": args.scale, "runs": []}
+    fp64 = {}
+    for tol in (1e-4, 1e-6):
+        for prec in ("fp64", "mixed"):
+            prm = default_params(tolerance=tol, precision=int(prec == "mixed"), gpu_scaling=1,
+                                 max_iterations=args.max_iterations, time_limit_s=args.time_limit)
+            t = time.perf_counter()
+            r = solve(b, p, prm)
+            wall = time.perf_counter() - t
+            k = check(b, p, r.x, r.y)
+            e = {"tolerance": tol, "precision": prec, "status": r.status, "iterations": r.iterations,
+                 "restarts": r.restarts, "setup_s": r.setup_time, "solve_s": r.solve_time, "wall_s": wall,
+                 "iters_per_s": r.iterations / max(r.solve_time, 1e-9), "objective": r.objective,
+                 "kkt_relative_primal": k.relative_primal, "kkt_relative_dual": k.relative_dual,
+                 "kkt_relative_gap": k.relative_gap, "fp64_iterations_in_mixed": r.mixed_fp64_iterations}
+            if prec == "fp64":
+                fp64[tol] = r
+            else:
+                f = fp64[tol]
+                e["objective_gap_vs_fp64"] = abs(r.objective - f.objective) / (1 + abs(f.objective))
+                e["x_rel_diff_vs_fp64"] = float(np.max(np.abs(r.x - f.x)) / (1 + np.max(np.abs(f.x))))
+            log(f"  tol {tol:g} {prec}: {r.status} {r.iterations} it, solve {r.solve_time:.2f}s "
+                f"({e['iters_per_s']:.0f} it/s), setup {r.setup_time:.2f}s, kkt max {k.max_relative():.2e}")
+            out["runs"].append(e)
+    # Host scaling of the same LP, for the setup comparison.
+    t = time.perf_counter()
+    from saddlepoint_amd.backend import ScaledProblem
+    ScaledProblem.new(p)
+    out["host_scaling_s"] = time.perf_counter() - t
+    log(f"  host ScaledProblem::new: {out['host_scaling_s']:.2f}s")
+    b.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +260,10 @@ def main() -> int:
     ap.add_argument("--probe", choices=["copy", "triad", "gather"],
                     help="run only a bandwidth probe (copy: 2^27 doubles, the HBM counter calibration)")
     ap.add_argument("--table", type=int, default=900_000, help="gather probe: table length in doubles")
+    ap.add_argument("--large", action="store_true",
+                    help="config 2 instead: FP64 vs mixed precision at 1e-4 and 1e-6 (use --scale to shrink)")
+    ap.add_argument("--max-iterations", type=int, default=200_000)
+    ap.add_argument("--time-limit", type=float, default=150.0, help="--large: seconds per solve")
     ap.add_argument("--serving", action="store_true",
                     help="serving stream instead: LPs/s and latency, served (FP64 and mixed) vs one at a time")
     ap.add_argument("--config", type=int, default=3, choices=[3, 4], help="serving stream (3, or 4 with large LPs)")
@@ -231,6 +280,11 @@ def main() -> int:
 
     from saddlepoint_amd import generate
     from saddlepoint_amd.backend import HipBackend, Solver, default_params
+
+    if args.large:
+        r = large_bench(args)
+        print(json.dumps({"metric": METRIC, "large": r}), flush=True)
+        return 0
 
     if args.serving:
         r = serving_bench(args)
