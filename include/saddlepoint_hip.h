@@ -158,6 +158,10 @@ typedef struct {
     int gpu_scaling;      /* 0: ScaledProblem::new on the host; 1: A and its transpose are uploaded
                              unscaled and the scaling passes run on the device (bitwise the host's
                              result where the sums add in the host's order) */
+    int reorder_columns;  /* 1 (default): the driver relabels columns by decreasing nonzero count
+                             before loading, so the most gathered entries share cache lines; each
+                             row keeps its nonzero order and the solution comes back in the
+                             original order. 0: the LP's own column order */
 } sp_params;
 
 /* Status (solver.rs:95-102). */

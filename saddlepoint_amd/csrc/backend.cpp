@@ -234,7 +234,8 @@ sp_problem* Hip::load(const ScaledProblem& s, bool autotune) {
     return load_view(v, &ta, &tat);
 }
 
-sp_problem* Hip::load_scaled(const LpView& p, const ScalingParams& sp, bool autotune, ScaledProblem& s) {
+sp_problem* Hip::load_scaled(const LpView& p, const ScalingParams& sp, bool autotune, ScaledProblem& s,
+                             const int64_t* rank) {
     const int64_t m = p.m, n = p.n;
     Csr t = transpose_of(m, n, p.row_ptr, p.col_idx, p.values);
     SpmvTuning ta, tat;
@@ -324,7 +325,7 @@ sp_problem* Hip::load_scaled(const LpView& p, const ScalingParams& sp, bool auto
         s.a = Csr{};
         s.at = Csr{};
         s.a.nrows = m, s.a.ncols = n, s.at.nrows = n, s.at.ncols = m;
-        finish_scaling(p, sp, s);
+        finish_scaling(p, sp, s, rank);
         std::vector<double> pf((size_t)m), df((size_t)n);
         for (int64_t i = 0; i < m; ++i) pf[(size_t)i] = 1.0 / (s.row_scale[(size_t)i] * s.bound_scale);
         for (int64_t j = 0; j < n; ++j) df[(size_t)j] = 1.0 / (s.col_scale[(size_t)j] * s.obj_scale);

@@ -152,7 +152,8 @@ struct Hip {
     // ScaledProblem::new on the device (scaling.rs:53-103): uploads A and its
     // transpose unscaled, runs the scaling passes there, and fills `s` with
     // the factors and scaled vectors (its matrices stay empty).
-    sp_problem* load_scaled(const LpView& p, const ScalingParams& sp, bool autotune, ScaledProblem& s);
+    sp_problem* load_scaled(const LpView& p, const ScalingParams& sp, bool autotune, ScaledProblem& s,
+                            const int64_t* rank = nullptr);
     // check() of crates/saddlepoint-check on the device, from the original LP.
     sp_kkt_report check(const LpView& p, const double* x, const double* y);
     // Average ms of one SpMV (A' or, with transpose, A'ᵀ) over reps launches.

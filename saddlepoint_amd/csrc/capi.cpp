@@ -275,6 +275,7 @@ void sp_default_params(sp_params* p) {
     p->precision = 0;
     p->spmv_autotune = 0;
     p->gpu_scaling = 0;
+    p->reorder_columns = 1;
 }
 
 int sp_solve(sp_backend* b, const sp_lp_view* lp, const sp_params* params, sp_solution* out, double* x_out,

@@ -28,8 +28,9 @@ struct DriverBackend : Hip {
     using Vector32 = std::unique_ptr<sp_vec32, Vec32Del>;
     using HostBuffer = std::unique_ptr<double, HostDel>;
     Problem load(const ScaledProblem& s, bool autotune = false) { return Problem(Hip::load(s, autotune)); }
-    Problem load_scaled(const LpView& p, const ScalingParams& sp, bool autotune, ScaledProblem& s) {
-        return Problem(Hip::load_scaled(p, sp, autotune, s));
+    Problem load_scaled(const LpView& p, const ScalingParams& sp, bool autotune, ScaledProblem& s,
+                        const int64_t* rank = nullptr) {
+        return Problem(Hip::load_scaled(p, sp, autotune, s, rank));
     }
     Vector zeros(int64_t n) { return Vector(Hip::zeros(n)); }
     Vector upload(const double* d, int64_t n) { return Vector(Hip::upload(d, n)); }

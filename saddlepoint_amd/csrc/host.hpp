@@ -65,8 +65,30 @@ struct ScaledProblem {
 Csr transpose_of(int64_t nrows, int64_t ncols, const int64_t* row_ptr, const int64_t* col_idx, const double* values);
 
 // The vector part of ScaledProblem::new once the matrix scaling is known
-// (scaling.rs:80-103): c, bounds and the bound/objective rescaling.
-void finish_scaling(const LpView& p, const ScalingParams& params, ScaledProblem& s);
+// (scaling.rs:80-103): c, bounds and the bound/objective rescaling. With
+// `rank` (the position of original column j), ‖c‖ is summed in the original
+// column order, so a relabeled LP scales to exactly the relabeled result.
+void finish_scaling(const LpView& p, const ScalingParams& params, ScaledProblem& s, const int64_t* rank = nullptr);
+
+// A relabeling of the columns by decreasing nonzero count (ties by index), so
+// that the most gathered entries of the column-indexed vectors share cache
+// lines. Each row keeps its nonzeros in their order: products and row sums
+// are unchanged, only where column-indexed values are stored.
+struct ColumnOrder {
+    std::vector<int64_t> order;  // new position -> original column
+    std::vector<int64_t> rank;   // original column -> new position
+    bool empty() const { return order.empty(); }
+};
+ColumnOrder popularity_order(const LpView& p);
+// The LP with relabeled columns; `v` points into the owned arrays.
+struct RelabeledLp {
+    LpView v;
+    std::vector<int64_t> col_idx;
+    std::vector<double> c, lv, uv;
+};
+RelabeledLp relabel(const LpView& p, const ColumnOrder& o);
+// The same relabeling applied to a scaled problem built from the original LP.
+void relabel(ScaledProblem& s, const ColumnOrder& o);
 
 double norm2(const double* v, int64_t n);                                   // scaling.rs:186-188
 double finite_bound_norm(const double* lo, const double* hi, int64_t m);    // scaling.rs:191-201

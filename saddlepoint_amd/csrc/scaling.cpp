@@ -1,8 +1,10 @@
 // Host preconditioning: restates crates/saddlepoint-core/src/scaling.rs:53-201
 // and the CsrMatrix helpers of problem.rs:88-126, operation for operation, so
 // the scaled problem is bitwise the one the reference builds.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <numeric>
 
 #include "host.hpp"
 #include "saddlepoint_hip.h"
@@ -158,7 +160,7 @@ ScaledProblem ScaledProblem::build(const LpView& p, const ScalingParams& params)
     return s;
 }
 
-void finish_scaling(const LpView& p, const ScalingParams& params, ScaledProblem& s) {
+void finish_scaling(const LpView& p, const ScalingParams& params, ScaledProblem& s, const int64_t* rank) {
     const int64_t m = p.m, n = p.n;
     s.c.resize((size_t)n);
     s.lv.resize((size_t)n);
@@ -176,7 +178,18 @@ void finish_scaling(const LpView& p, const ScalingParams& params, ScaledProblem&
     }
     if (params.bound_objective_rescaling) {
         s.bound_scale = 1.0 / (1.0 + finite_bound_norm(s.lc.data(), s.uc.data(), m));
-        s.obj_scale = 1.0 / (1.0 + norm2(s.c.data(), n));
+        double cn;
+        if (rank) {
+            double acc = 0.0;
+            for (int64_t j = 0; j < n; ++j) {
+                const double v = s.c[(size_t)rank[j]];
+                acc += v * v;
+            }
+            cn = std::sqrt(acc);
+        } else {
+            cn = norm2(s.c.data(), n);
+        }
+        s.obj_scale = 1.0 / (1.0 + cn);
         for (auto* v : {&s.lc, &s.uc, &s.lv, &s.uv})
             for (double& e : *v) e *= s.bound_scale;
         for (double& e : s.c) e *= s.obj_scale;
@@ -205,6 +218,71 @@ std::vector<double> ScaledProblem::dual_residual_factors() const {
     std::vector<double> f(col_scale.size());
     for (size_t j = 0; j < f.size(); ++j) f[j] = 1.0 / (col_scale[j] * obj_scale);
     return f;
+}
+
+ColumnOrder popularity_order(const LpView& p) {
+    ColumnOrder o;
+    std::vector<int64_t> count((size_t)p.n, 0);
+    for (int64_t k = 0; k < p.nnz; ++k) count[(size_t)p.col_idx[k]] += 1;
+    o.order.resize((size_t)p.n);
+    std::iota(o.order.begin(), o.order.end(), int64_t(0));
+    std::stable_sort(o.order.begin(), o.order.end(),
+                     [&](int64_t a, int64_t b) { return count[(size_t)a] > count[(size_t)b]; });
+    o.rank.resize((size_t)p.n);
+    for (int64_t q = 0; q < p.n; ++q) o.rank[(size_t)o.order[(size_t)q]] = q;
+    return o;
+}
+
+RelabeledLp relabel(const LpView& p, const ColumnOrder& o) {
+    RelabeledLp r;
+    r.v = p;
+    r.col_idx.resize((size_t)p.nnz);
+    for (int64_t k = 0; k < p.nnz; ++k) r.col_idx[(size_t)k] = o.rank[(size_t)p.col_idx[k]];
+    r.c.resize((size_t)p.n);
+    r.lv.resize((size_t)p.n);
+    r.uv.resize((size_t)p.n);
+    for (int64_t q = 0; q < p.n; ++q) {
+        const int64_t j = o.order[(size_t)q];
+        r.c[(size_t)q] = p.c[j];
+        r.lv[(size_t)q] = p.lv[j];
+        r.uv[(size_t)q] = p.uv[j];
+    }
+    r.v.col_idx = r.col_idx.data();
+    r.v.c = r.c.data();
+    r.v.lv = r.lv.data();
+    r.v.uv = r.uv.data();
+    return r;
+}
+
+void relabel(ScaledProblem& s, const ColumnOrder& o) {
+    const int64_t n = s.a.ncols;
+    for (auto& j : s.a.col_idx) j = o.rank[(size_t)j];
+    // A'ᵀ: row q of the relabeled transpose is row order[q] of the old one.
+    Csr t;
+    t.nrows = s.at.nrows;
+    t.ncols = s.at.ncols;
+    t.row_ptr.resize((size_t)n + 1);
+    t.col_idx.resize(s.at.col_idx.size());
+    t.values.resize(s.at.values.size());
+    t.row_ptr[0] = 0;
+    for (int64_t q = 0; q < n; ++q) {
+        const int64_t j = o.order[(size_t)q];
+        const int64_t b = s.at.row_ptr[(size_t)j], e = s.at.row_ptr[(size_t)j + 1];
+        const int64_t at = t.row_ptr[(size_t)q];
+        std::copy(s.at.col_idx.begin() + b, s.at.col_idx.begin() + e, t.col_idx.begin() + at);
+        std::copy(s.at.values.begin() + b, s.at.values.begin() + e, t.values.begin() + at);
+        t.row_ptr[(size_t)q + 1] = at + (e - b);
+    }
+    s.at = std::move(t);
+    auto permute = [&](std::vector<double>& v) {
+        std::vector<double> w(v.size());
+        for (int64_t q = 0; q < n; ++q) w[(size_t)q] = v[(size_t)o.order[(size_t)q]];
+        v.swap(w);
+    };
+    permute(s.c);
+    permute(s.lv);
+    permute(s.uv);
+    permute(s.col_scale);
 }
 
 void validate(const LpView& p) {

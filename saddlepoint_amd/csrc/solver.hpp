@@ -93,10 +93,21 @@ class Solver {
           c0_(problem.c0), maximize_(problem.maximize) {
         b_norm_ = finite_bound_norm(problem.lc, problem.uc, problem.m);
         c_norm_ = norm2(problem.c, problem.n);
-        if (params.gpu_scaling)
-            dev_ = b_.load_scaled(problem, scaling_params_of(params), params.spmv_autotune != 0, scaled_);
-        else
+        // Columns relabeled by popularity (the solution is returned in the
+        // original order; see ColumnOrder).
+        if (params.reorder_columns) cols_ = popularity_order(problem);
+        if (params.gpu_scaling) {
+            if (cols_.empty()) {
+                dev_ = b_.load_scaled(problem, scaling_params_of(params), params.spmv_autotune != 0, scaled_);
+            } else {
+                const RelabeledLp r = relabel(problem, cols_);
+                dev_ = b_.load_scaled(r.v, scaling_params_of(params), params.spmv_autotune != 0, scaled_,
+                                      cols_.rank.data());
+            }
+        } else {
+            if (!cols_.empty()) relabel(scaled_, cols_);
             dev_ = b_.load(scaled_, params.spmv_autotune != 0);
+        }
         // The scaled matrices live on the device now; the host keeps the factors.
         scaled_.a = Csr{};
         scaled_.at = Csr{};
@@ -417,7 +428,9 @@ class Solver {
             b_.download(y_t_.get(), ys.data(), m_);
             const std::vector<double> x = scaled_.unscale_primal(xs);
             const std::vector<double> y = scaled_.unscale_dual(ys);
-            if (x_out) std::copy(x.begin(), x.end(), x_out);
+            if (x_out && cols_.empty()) std::copy(x.begin(), x.end(), x_out);
+            if (x_out && !cols_.empty())
+                for (int64_t q = 0; q < n_; ++q) x_out[cols_.order[(size_t)q]] = x[(size_t)q];
             if (y_out) std::copy(y.begin(), y.end(), y_out);
         }
         const double sign = maximize_ ? -1.0 : 1.0;
@@ -678,7 +691,10 @@ class Solver {
         const int64_t n = n_;
         if (n == 0 || nnz_ == 0) return 1.0;
         std::vector<double> start((size_t)n);
-        for (int64_t j = 0; j < n; ++j) start[(size_t)j] = 1.0 + 0.01 * (double)(j % 7);
+        for (int64_t j = 0; j < n; ++j) {
+            const int64_t orig = cols_.empty() ? j : cols_.order[(size_t)j];
+            start[(size_t)j] = 1.0 + 0.01 * (double)(orig % 7);
+        }
         Vec v = b_.upload(start.data(), n);
         const double s = 1.0 / std::sqrt(b_.norm_sq(v.get()));
         b_.scale(v.get(), s);
@@ -779,6 +795,7 @@ class Solver {
     B& b_;
     const sp_params p_;
     ScaledProblem scaled_;
+    ColumnOrder cols_;  // empty unless params.reorder_columns
     int64_t m_, n_, nnz_;
     double c0_;
     bool maximize_;

@@ -138,3 +138,19 @@ def test_autotuned_plans_solve_like_the_oracle(gpu):
     a 1024-nonzero plan); the solve still matches the oracle."""
     run_pair(gpu, "config0", generate.config0(), spmv_autotune=1)
     run_pair(gpu, "israel", netlib("israel"), spmv_autotune=1)
+
+
+@pytest.mark.parametrize("name", ["israel", "shell", "woodinfe", "config0"])
+def test_column_relabeling_changes_storage_not_results(gpu, name):
+    """reorder_columns (on by default) stores column-indexed vectors by
+    decreasing column popularity; the answer comes back in the LP's order and
+    agrees with the unrelabeled solve (reductions add in another order, so
+    iteration counts agree within 1% rather than bitwise)."""
+    p = generate.config0() if name == "config0" else netlib(name)
+    a = solve(gpu, p, default_params(reorder_columns=0))
+    b = solve(gpu, p, default_params(reorder_columns=1))
+    print(f"{name}: plain order {a.status} {a.iterations} it; relabeled {b.status} {b.iterations} it", flush=True)
+    assert a.status == b.status and iterations_match(a.iterations, b.iterations)
+    if a.status == "Optimal":
+        O.check(p, b.x, b.y).verify(2 * TOL)
+        assert abs(a.objective - b.objective) / (1 + abs(a.objective)) <= 2 * TOL
