@@ -219,6 +219,22 @@ int sp_probe(sp_backend *b, int kind, int64_t n, int64_t table_len, int reps, do
 /* Waits for all work queued on the backend's stream. */
 int sp_backend_sync(sp_backend *b);
 
+/* ---- reading LPs ------------------------------------------------------------------- */
+
+/* An LP owned by the library, as read from a file. */
+typedef struct sp_lp sp_lp;
+/* read_mps_file (crates/saddlepoint-io/src/lib.rs:50-309): free-format MPS
+ * (NAME, OBJSENSE, ROWS, COLUMNS with integer markers ignored, RHS, RANGES,
+ * BOUNDS; first RHS/RANGES/BOUNDS set), assembled as from_triplets does
+ * (problem.rs:33-77). SP_ERR_PROBLEM with "line N: ..." (sp_last_error(NULL))
+ * on malformed input, SP_ERR_INVALID when the file cannot be read. */
+int sp_lp_read_mps(const char *path, sp_lp **out);
+/* A view of the LP's arrays (valid until sp_lp_free), ready for sp_solve,
+ * sp_check or sp_server_submit. */
+int sp_lp_view_of(const sp_lp *lp, sp_lp_view *out);
+const char *sp_lp_name(const sp_lp *lp);
+void sp_lp_free(sp_lp *lp);
+
 /* ---- the KKT checker ---------------------------------------------------------------- */
 
 /* KktReport (crates/saddlepoint-check/src/lib.rs:10-25). */

@@ -2,6 +2,7 @@
 // messages around the HIP backend (backend.cpp) and the driver (solver.hpp).
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 
@@ -443,6 +444,60 @@ int sp_scale_problem(const sp_lp_view* lp, const sp_params* params, double* a_va
         return SP_ERR_INVALID;
     }
 }
+
+int sp_lp_read_mps(const char* path, sp_lp** out) {
+    try {
+        if (!path || !out) throw sp::Error(SP_ERR_INVALID, "read_mps: null argument");
+        *out = nullptr;
+        std::FILE* fp = std::fopen(path, "rb");
+        if (!fp) throw sp::Error(SP_ERR_INVALID, std::string(path) + ": cannot open");
+        std::string text;
+        char buf[1 << 16];
+        size_t got;
+        while ((got = std::fread(buf, 1, sizeof buf, fp)) > 0) text.append(buf, got);
+        const bool bad = std::ferror(fp) != 0;
+        std::fclose(fp);
+        if (bad) throw sp::Error(SP_ERR_INVALID, std::string(path) + ": read error");
+        *out = sp::read_mps_text(text.data(), text.size());
+        if ((*out)->name.empty()) {
+            std::string stem(path);
+            const size_t slash = stem.find_last_of('/');
+            if (slash != std::string::npos) stem = stem.substr(slash + 1);
+            const size_t dot = stem.find('.');
+            if (dot != std::string::npos) stem = stem.substr(0, dot);
+            (*out)->name = stem;
+        }
+        g_create_error.clear();
+        return SP_OK;
+    } catch (const sp::Error& e) {
+        g_create_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_create_error = "host out of memory";
+        return SP_ERR_MEMORY;
+    } catch (const std::exception& e) {
+        g_create_error = e.what();
+        return SP_ERR_INVALID;
+    }
+}
+
+int sp_lp_view_of(const sp_lp* lp, sp_lp_view* out) {
+    if (!lp || !out) return SP_ERR_INVALID;
+    out->a = sp_csr_view{lp->m, lp->n, (int64_t)lp->values.size(), lp->row_ptr.data(), lp->col_idx.data(),
+                         lp->values.data()};
+    out->c = lp->c.data();
+    out->lc = lp->lc.data();
+    out->uc = lp->uc.data();
+    out->lv = lp->lv.data();
+    out->uv = lp->uv.data();
+    out->c0 = lp->c0;
+    out->maximize = lp->maximize ? 1 : 0;
+    return SP_OK;
+}
+
+const char* sp_lp_name(const sp_lp* lp) { return lp ? lp->name.c_str() : ""; }
+
+void sp_lp_free(sp_lp* lp) { delete lp; }
 
 int sp_check(sp_backend* b, const sp_lp_view* lp, const double* x, const double* y, sp_kkt_report* out) {
     return guarded(b, [&] {

@@ -90,6 +90,24 @@ RelabeledLp relabel(const LpView& p, const ColumnOrder& o);
 // The same relabeling applied to a scaled problem built from the original LP.
 void relabel(ScaledProblem& s, const ColumnOrder& o);
 
+}  // namespace sp
+
+// An LP read from a file (the C ABI's sp_lp): LpProblem (problem.rs:131-145)
+// with owned arrays.
+struct sp_lp {
+    std::string name;
+    int64_t m = 0, n = 0;
+    std::vector<int64_t> row_ptr, col_idx;
+    std::vector<double> values, c, lc, uc, lv, uv;
+    double c0 = 0.0;
+    bool maximize = false;
+};
+
+namespace sp {
+
+// MPS text to an LP (crates/saddlepoint-io/src/lib.rs:59-309); throws Error.
+sp_lp* read_mps_text(const char* data, size_t size);
+
 double norm2(const double* v, int64_t n);                                   // scaling.rs:186-188
 double finite_bound_norm(const double* lo, const double* hi, int64_t m);    // scaling.rs:191-201
 void validate(const LpView& p);                                             // problem.rs:166-194

@@ -49,6 +49,39 @@ def read_mps_file(path) -> LpProblem:
     return p
 
 
+def read_mps_file_native(path) -> LpProblem:
+    """The same LP through the library's C++ reader (``sp_lp_read_mps``), for
+    files at MIPLIB scale; row and column names are not returned."""
+    import ctypes as C
+
+    from ._lib import SP_OK, LpView, lib
+
+    h = C.c_void_p()
+    rc = lib().sp_lp_read_mps(str(path).encode(), C.byref(h))
+    if rc != SP_OK:
+        msg = lib().sp_last_error(None).decode()
+        if msg.startswith("line "):
+            raise MpsError(int(msg.split(":", 1)[0][5:]), msg.split(": ", 1)[1])
+        raise OSError(msg)
+    try:
+        v = LpView()
+        lib().sp_lp_view_of(h, C.byref(v))
+        m, n, nnz = v.a.nrows, v.a.ncols, v.a.nnz
+
+        def arr(ptr, count, dtype):
+            if count == 0:
+                return np.zeros(0, dtype)
+            return np.ctypeslib.as_array(ptr, shape=(count,)).astype(dtype, copy=True)
+
+        a = CsrMatrix(m, n, arr(v.a.row_ptr, m + 1, np.int64), arr(v.a.col_idx, nnz, np.int64),
+                      arr(v.a.values, nnz, np.float64))
+        return LpProblem(a=a, c=arr(v.c, n, np.float64), lc=arr(v.lc, m, np.float64), uc=arr(v.uc, m, np.float64),
+                         lv=arr(v.lv, n, np.float64), uv=arr(v.uv, n, np.float64), c0=float(v.c0),
+                         maximize=bool(v.maximize), name=lib().sp_lp_name(h).decode())
+    finally:
+        lib().sp_lp_free(h)
+
+
 def read_mps(text: str) -> LpProblem:  # io/src/lib.rs:59-309
     name = ""
     maximize = False

@@ -7,7 +7,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-from conftest import netlib
+from conftest import NETLIB, netlib
 from oracle import oracle as O
 from saddlepoint_amd import _lib, generate, mps
 from saddlepoint_amd.backend import ScaledProblem, default_params
@@ -172,3 +172,30 @@ def test_autotuned_plan_rule_is_deterministic_and_in_range():
             assert a == plan(rp.copy())
             assert 1 <= a[0] <= 2048 and 1 <= a[1] <= a[0] and 1 <= a[2] <= 1024
     assert plan(np.zeros(5, np.int64)) == (2048, 2048, 1024)  # no nonzeros: the default
+
+
+def _same_lp(a, b):
+    assert (a.nrows, a.ncols, a.a.nnz) == (b.nrows, b.ncols, b.a.nnz)
+    for f in ("row_ptr", "col_idx", "values"):
+        assert np.array_equal(getattr(a.a, f), getattr(b.a, f)), f
+    for f in ("c", "lc", "uc", "lv", "uv"):
+        assert np.array_equal(np.asarray(getattr(a, f)), np.asarray(getattr(b, f))), f
+    assert a.c0 == b.c0 and bool(a.maximize) == bool(b.maximize) and a.name == b.name
+
+
+@pytest.mark.parametrize("path", sorted(NETLIB.glob("*.mps")), ids=lambda p: p.stem)
+def test_native_mps_reader_is_the_python_reader(path):
+    _same_lp(mps.read_mps_file_native(path), mps.read_mps_file(path))
+
+
+def test_native_mps_reader_sections_and_errors(tmp_path):
+    f = tmp_path / "sample.mps"
+    f.write_text(SAMPLE)
+    _same_lp(mps.read_mps_file_native(f), mps.read_mps_file(f))
+    bad = tmp_path / "bad.mps"
+    bad.write_text("NAME X\nROWS\n N obj\nCOLUMNS\n    x  nope  1\nENDATA\n")
+    with pytest.raises(mps.MpsError) as e:
+        mps.read_mps_file_native(bad)
+    assert e.value.line == 5
+    with pytest.raises(OSError):
+        mps.read_mps_file_native(tmp_path / "missing.mps")
