@@ -357,6 +357,16 @@ def main() -> int:
     roofline = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "bytes_per_launch": kb[dom]}
+    # The kernel's practical ceiling: one random gather per nonzero from the
+    # gathered vector (x-bar on the A side, y on the A-transpose side); the
+    # probe measures the rate of random 8-byte gathers from a table that size.
+    gathered = n if "dual" in dom or dom == "sp_spmv_f64" else m
+    probe_ms = b.probe("gather", 1 << 26, gathered, reps=10)
+    probe_rate = (1 << 26) / (probe_ms * 1e-3)
+    kernel_rate = nnz / (per_kernel[dom]["avg_us"] * 1e-6)
+    roofline["gather"] = {"gathers_per_launch": nnz, "table_bytes": 8 * gathered,
+                          "achieved_per_s": round(kernel_rate / 1e9, 1), "probe_per_s": round(probe_rate / 1e9, 1),
+                          "unit": "G gathers/s", "frac": round(kernel_rate / probe_rate, 3)}
 
     out = {
         "metric": METRIC,

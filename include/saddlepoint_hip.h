@@ -162,6 +162,8 @@ typedef struct {
                              before loading, so the most gathered entries share cache lines; each
                              row keeps its nonzero order and the solution comes back in the
                              original order. 0: the LP's own column order */
+    int64_t mixed_stall_checks; /* precision = 1: checks without halving the relative KKT error after
+                                   which the solve continues in FP64 (0: 64) */
 } sp_params;
 
 /* Status (solver.rs:95-102). */

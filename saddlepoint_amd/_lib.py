@@ -64,7 +64,7 @@ class Params(C.Structure):
                 ("has_pock_chambolle", C.c_int), ("pock_chambolle_alpha", _d),
                 ("bound_objective_rescaling", C.c_int), ("termination", C.c_int), ("fused", C.c_int),
                 ("log_every_checks", C.c_int), ("precision", C.c_int), ("spmv_autotune", C.c_int),
-                ("gpu_scaling", C.c_int), ("reorder_columns", C.c_int)]
+                ("gpu_scaling", C.c_int), ("reorder_columns", C.c_int), ("mixed_stall_checks", _i64)]
 
 
 class Solution(C.Structure):

@@ -779,7 +779,7 @@ class Solver {
 
     // Mixed precision only: FP64 takes over when the relative KKT error has not
     // improved on its best for kStallChecks checks in a row.
-    static constexpr uint64_t kStallChecks = 64;
+    static constexpr uint64_t kStallChecks = 64;  // when params.mixed_stall_checks is 0
     void watch_progress(const Kkt& k) {
         if (p_.precision != 1 || fp64_fallback_) return;
         const double err = std::max({k.primal_res / (1.0 + b_norm_), k.dual_res / (1.0 + c_norm_),
@@ -787,7 +787,7 @@ class Solver {
         if (err < 0.5 * best_err_) {
             best_err_ = err;
             stalled_checks_ = 0;
-        } else if (++stalled_checks_ >= kStallChecks) {
+        } else if (++stalled_checks_ >= (p_.mixed_stall_checks > 0 ? (uint64_t)p_.mixed_stall_checks : kStallChecks)) {
             fp64_fallback_ = true;
         }
     }
