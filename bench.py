@@ -361,6 +361,8 @@ def main() -> int:
     # gathered vector (x-bar on the A side, y on the A-transpose side); the
     # probe measures the rate of random 8-byte gathers from a table that size.
     gathered = n if "dual" in dom or dom == "sp_spmv_f64" else m
+    if "mixed" in dom:
+        gathered = max(1, gathered // 2)  # FP32 values: the table of a double probe half as long
     probe_ms = b.probe("gather", 1 << 26, gathered, reps=10)
     probe_rate = (1 << 26) / (probe_ms * 1e-3)
     kernel_rate = nnz / (per_kernel[dom]["avg_us"] * 1e-6)

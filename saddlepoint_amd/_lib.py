@@ -17,8 +17,13 @@ SP_OK = 0
 ERRORS = {1: "UNAVAILABLE", 2: "HIP", 3: "SHAPE", 4: "TOO_LARGE", 5: "INVALID", 6: "PROBLEM", 7: "NUMERICAL",
           8: "MEMORY"}
 SP_ERR_UNAVAILABLE = 1
+SP_ERR_HIP = 2
 SP_ERR_SHAPE = 3
+SP_ERR_TOO_LARGE = 4
+SP_ERR_INVALID = 5
 SP_ERR_PROBLEM = 6
+SP_ERR_NUMERICAL = 7
+SP_ERR_MEMORY = 8
 
 _d = C.c_double
 _i64 = C.c_int64

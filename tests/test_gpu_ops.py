@@ -214,3 +214,21 @@ def test_shape_errors_are_reported(gpu):
     with pytest.raises(BackendError) as e:
         gpu.dot(gpu.zeros(3), gpu.zeros(4))
     assert e.value.code == SP_ERR_SHAPE
+
+
+def test_int32_limit_is_reported(gpu):
+    """More rows or nonzeros than 32-bit device indices hold: SP_ERR_TOO_LARGE
+    (the reference's CudaError::TooLarge, sparse.rs:60-62), before any array
+    is touched."""
+    import ctypes as C
+
+    from saddlepoint_amd._lib import SP_ERR_TOO_LARGE, CsrView, ScaledView, lib
+
+    big = 1 << 31
+    v = ScaledView()
+    v.a = CsrView(big, 1, 1, None, None, None)
+    v.at = CsrView(1, big, 1, None, None, None)
+    h = C.c_void_p()
+    rc = lib().sp_problem_load(gpu.h, C.byref(v), C.byref(h))
+    assert rc == SP_ERR_TOO_LARGE and not h.value
+    assert "32-bit" in lib().sp_last_error(gpu.h).decode()

@@ -154,3 +154,54 @@ def test_column_relabeling_changes_storage_not_results(gpu, name):
     if a.status == "Optimal":
         O.check(p, b.x, b.y).verify(2 * TOL)
         assert abs(a.objective - b.objective) / (1 + abs(a.objective)) <= 2 * TOL
+
+
+def edge_cases():
+    import math
+
+    from saddlepoint_amd.problem import CsrMatrix, LpProblem
+
+    inf = math.inf
+    return {
+        # No constraint rows: the bounds alone (optimum -3 at x = (0, 3)).
+        "no_rows": LpProblem(a=CsrMatrix.from_triplets(0, 2, [], [], []), c=[1.0, -1.0], lc=[], uc=[],
+                             lv=[0.0, -1.0], uv=[2.0, 3.0]),
+        # Rows without a nonzero.
+        "no_nonzeros": LpProblem(a=CsrMatrix.from_triplets(1, 2, [], [], []), c=[1.0, 2.0], lc=[0.0], uc=[1.0],
+                                 lv=[0.0, 0.0], uv=[1.0, 1.0]),
+        # An empty row and an empty column among ordinary ones.
+        "ragged": LpProblem(a=CsrMatrix.from_triplets(3, 3, [0, 0, 2], [0, 1, 0], [1.0, 1.0, 2.0]),
+                            c=[-1.0, -2.0, 0.5], lc=[-inf, -inf, 0.0], uc=[4.0, 5.0, 6.0], lv=[0.0, 0.0, 0.0],
+                            uv=[inf, 3.0, 1.0]),
+        "one_variable": LpProblem(a=CsrMatrix.from_triplets(1, 1, [0], [0], [2.0]), c=[-1.0], lc=[-inf],
+                                  uc=[3.0], lv=[0.0], uv=[inf]),
+    }
+
+
+@pytest.mark.parametrize("name", list(edge_cases()))
+def test_edge_case_lps_match_the_oracle(gpu, name):
+    p = edge_cases()[name]
+    c, cx, _ = O.solve(p, O.default_params(tolerance=TOL))
+    for kw in (dict(), dict(precision=1), dict(gpu_scaling=1)):
+        g = solve(gpu, p, default_params(tolerance=TOL, **kw))
+        print(f"{name} {kw}: oracle {O.STATUS[c.status]} {c.iterations} it {c.objective:.9f}; "
+              f"gpu {g.status} {g.iterations} it {g.objective:.9f}", flush=True)
+        assert g.status == O.STATUS[c.status] == "Optimal"
+        assert abs(g.objective - c.objective) <= 2 * TOL * (1 + abs(c.objective))
+        if not kw:
+            assert g.iterations == c.iterations
+
+
+def test_edge_case_lps_served_as_alone(gpu):
+    from saddlepoint_amd.serving import Server
+
+    items = list(edge_cases().items()) + [("afiro", netlib("afiro"))]
+    s = Server(gpu)
+    ids = [s.submit(p) for _, p in items]
+    s.drain(ids)
+    for (name, p), i in zip(items, ids):
+        r = s.result(i).result
+        g = solve(gpu, p)
+        assert r.status == g.status and r.iterations == g.iterations, name
+        assert np.array_equal(r.x, g.x) and np.array_equal(r.y, g.y), name
+    s.close()

@@ -20,7 +20,7 @@ def test_spbench_rows(tmp_path, capsys):
     rows = list(csv.DictReader(open(out)))
     assert list(rows[0].keys()) == spbench.HEADER.split(",")
     assert len(rows) == 7  # one header despite two runs
-    by = {(r["instance"], r["solver"]): r for r in rows}
+    by = {(r["instance"].lower(), r["solver"]): r for r in rows}  # the instance is the MPS NAME
     assert by[("afiro", "gpu")]["status"] == "Optimal" and by[("afiro", "gpu+step")]["status"] == "Optimal"
     assert by[("woodinfe", "gpu")]["status"] == "PrimalInfeasible"
     for key in (("afiro", "gpu"), ("israel", "gpu"), ("israel", "gpu-mixed")):
