@@ -102,7 +102,7 @@ def fifo_latencies(arrivals, service):
     return out
 
 
-def run_served(b, lps, params_of, arrivals=None, max_active=64):
+def run_served(b, lps, params_of, arrivals=None, max_active=64, with_solution=False):
     """Serves `lps` (all at t = 0, or at the given arrival times on the
     server's clock); returns (makespan, latencies, results)."""
     from saddlepoint_amd.serving import Server
@@ -132,7 +132,7 @@ def run_served(b, lps, params_of, arrivals=None, max_active=64):
             log(f"  served {done}/{n} (active {active}, waiting {waiting})")
             last_report = time.time()
     t_end = srv.clock()
-    res = [srv.result(j, with_solution=False) for j in ids]
+    res = [srv.result(j, with_solution=with_solution) for j in ids]
     lat = [r.completed - ta for r, ta in zip(res, t_arr)]
     srv.close()
     return t_end - t0, lat, [r.result for r in res]
@@ -149,12 +149,15 @@ def serving_bench(args) -> dict:
     log(f"generated {len(lps)} LPs of the config-{args.config} stream in {time.time() - t0:.1f}s; "
         f"nnz mean {np.mean(nnz):.0f}, max {max(nnz)}")
     b = HipBackend(0)
+    limit = args.lp_time_limit if args.lp_time_limit > 0 else -1.0
+    it_cap = args.lp_max_iterations
+    autotune = int(args.config == 4)  # config 4: each LP's SpMV plans from the autotuned rule
     # Plain FP64 one at a time: every LP's whole solve (validate, scale, load,
     # solve, download), back to back.
     service, one = [], []
     for i, (p, tol) in enumerate(zip(lps, tols)):
         s = time.perf_counter()
-        r = solve(b, p, default_params(tolerance=tol))
+        r = solve(b, p, default_params(tolerance=tol, time_limit_s=limit, max_iterations=it_cap))
         service.append(time.perf_counter() - s)
         one.append(r)
         log(f"  one-at-a-time {i}: m={p.nrows} nnz={p.a.nnz} tol={tol:g} {r.status} {r.iterations} it "
@@ -174,12 +177,23 @@ def serving_bench(args) -> dict:
                                             "p99_s": pct(poi_one, 99)}
     for prec in ("fp64", "mixed"):
         pr = int(prec == "mixed")
-        mk, lat, res = run_served(b, lps, lambda i: default_params(tolerance=tols[i], precision=pr))
+        pm = (lambda i: default_params(tolerance=tols[i], precision=pr, time_limit_s=limit, max_iterations=it_cap,
+                                       spmv_autotune=autotune * pr))
+        mk, lat, res = run_served(b, lps, pm, with_solution=args.check)
         st = {}
         for r in res:
             st[r.status] = st.get(r.status, 0) + 1
         entry = {"lps_per_s": len(lps) / mk, "makespan_s": mk, "p50_s": pct(lat, 50), "p99_s": pct(lat, 99),
                  "statuses": st, "speedup_vs_one_at_a_time": (len(lps) / mk) / cap}
+        if args.check:
+            # The KKT checker on every served solution (sp_check, on the device).
+            from saddlepoint_amd.backend import check
+            kk = [check(b, p, r.x, r.y).max_relative() for p, r in zip(lps, res) if r.status == "Optimal"]
+            ok = sum(1 for p, r, t in zip(lps, res, tols)
+                     if r.status == "Optimal" and check(b, p, r.x, r.y).max_relative() <= 2 * t)
+            entry["checked"] = {"optimal": len(kk), "pass_at_2tol": ok, "max_relative_kkt": max(kk) if kk else None}
+            for r in res:
+                r.x = r.y = None
         if pr:
             gaps = [abs(r.objective - o.objective) / (1 + abs(o.objective)) for r, o in zip(res, one)
                     if r.status == "Optimal" and o.status == "Optimal"]
@@ -194,7 +208,7 @@ def serving_bench(args) -> dict:
         log(f"served {prec}: {entry['lps_per_s']:.3f} LPs/s (x{entry['speedup_vs_one_at_a_time']:.2f}), "
             f"p50 {entry['p50_s']:.3f}s p99 {entry['p99_s']:.3f}s")
         if args.poisson and pr:
-            _, plat, _ = run_served(b, lps, lambda i: default_params(tolerance=tols[i], precision=pr), arrivals)
+            _, plat, _ = run_served(b, lps, pm, arrivals)
             entry["poisson"] = {"rate_lps_per_s": args.rate_frac * cap, "p50_s": pct(plat, 50),
                                 "p99_s": pct(plat, 99)}
             log(f"  poisson at {args.rate_frac} x capacity: p50 {pct(plat, 50):.3f}s p99 {pct(plat, 99):.3f}s "
@@ -271,6 +285,10 @@ def main() -> int:
     ap.add_argument("--lps", type=int, default=64, help="serving: LPs in the sample")
     ap.add_argument("--poisson", action="store_true", help="serving: also Poisson arrivals (mixed precision)")
     ap.add_argument("--rate-frac", type=float, default=0.8, help="serving: arrival rate / one-at-a-time capacity")
+    ap.add_argument("--lp-time-limit", type=float, default=0.0, help="serving: seconds per LP (0: none)")
+    ap.add_argument("--lp-max-iterations", type=int, default=1_000_000,
+                    help="serving: iteration cap per LP, the same in every mode (a multiple of 64)")
+    ap.add_argument("--check", action="store_true", help="serving: KKT-check every served solution on the device")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_config1.txt"),
                     help="measured HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE summary)")
     args = ap.parse_args()
@@ -294,8 +312,10 @@ def main() -> int:
                           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
                           "data": "synthetic (seeded serving-stream generator)",
                           "config": {"workload": f"config{args.config} serving stream sample: {args.lps} LPs, rows "
-                                                 f"log-uniform 1k-200k, tolerance 50/50 1e-4/1e-6, all submitted "
-                                                 f"at t=0", "parallelism": "single GPU"},
+                                                 f"log-uniform 1k-200k" + (" plus 10% large (200k-1M rows, Zipf(1.5) "
+                                                 "row lengths)" if args.config == 4 else "") +
+                                                 ", tolerance 50/50 1e-4/1e-6, all submitted at t=0",
+                                     "parallelism": "single GPU"},
                           "serving": r}), flush=True)
         return 0
 
