@@ -237,6 +237,14 @@ int sp_lp_view_of(const sp_lp *lp, sp_lp_view *out);
 const char *sp_lp_name(const sp_lp *lp);
 void sp_lp_free(sp_lp *lp);
 
+/* CsrMatrix::transpose (problem.rs:88-109) on the device: a stable radix sort
+ * of the nonzeros by column, the result the host's counting sort gives. The
+ * input is an m×n CSR matrix; the caller allocates t_row_ptr[n+1],
+ * t_col_idx[nnz], t_values[nnz]. (The device preconditioning and the checker
+ * transpose this way.) */
+int sp_transpose_gpu(sp_backend *b, int64_t nrows, int64_t ncols, const int64_t *row_ptr, const int64_t *col_idx,
+                     const double *values, int64_t *t_row_ptr, int64_t *t_col_idx, double *t_values);
+
 /* ---- the KKT checker ---------------------------------------------------------------- */
 
 /* KktReport (crates/saddlepoint-check/src/lib.rs:10-25). */

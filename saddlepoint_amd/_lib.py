@@ -156,6 +156,7 @@ SIGNATURES = {
     "sp_scale_problem": (C.c_int, [C.POINTER(LpView), C.POINTER(Params), _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd,
                                    _pd, _pd]),
     "sp_check": (C.c_int, [_vp, C.POINTER(LpView), _pd, _pd, C.c_void_p]),
+    "sp_transpose_gpu": (C.c_int, [_vp, _i64, _i64, _pi64, _pi64, _pd, _pi64, _pi64, _pd]),
     "sp_lp_read_mps": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),
     "sp_lp_view_of": (C.c_int, [_vp, C.POINTER(LpView)]),
     "sp_lp_name": (C.c_char_p, [_vp]),

@@ -342,6 +342,22 @@ class SolveResult:
     mixed_fp64_iterations: int = 0
 
 
+def transpose_on_device(backend: HipBackend, a: CsrMatrix) -> CsrMatrix:
+    """``CsrMatrix::transpose`` (problem.rs:88-109) by the device's radix sort."""
+    rp = np.ascontiguousarray(a.row_ptr, dtype=np.int64)
+    ci = np.ascontiguousarray(a.col_idx, dtype=np.int64)
+    va = np.ascontiguousarray(a.values, dtype=np.float64)
+    nnz = int(rp[-1])
+    trp = np.zeros(a.ncols + 1, np.int64)
+    tci = np.zeros(max(nnz, 1), np.int64)
+    tva = np.zeros(max(nnz, 1), np.float64)
+    i64p = lambda v: v.ctypes.data_as(C.POINTER(C.c_int64))  # noqa: E731
+    f64p = lambda v: v.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    backend._ok(lib().sp_transpose_gpu(backend.h, a.nrows, a.ncols, i64p(rp), i64p(ci), f64p(va), i64p(trp),
+                                       i64p(tci), f64p(tva)))
+    return CsrMatrix(a.ncols, a.nrows, trp, tci[:nnz], tva[:nnz])
+
+
 def check(backend: HipBackend, p: LpProblem, x, y) -> KktReport:
     """``saddlepoint_check::check`` (check/src/lib.rs:62-118) on the GPU."""
     v, keep = lp_view(p)

@@ -103,6 +103,8 @@ void need_len(const sp_vec* v, int64_t len, const char* what) {
                                       std::to_string(len));
 }
 
+void plan_csr(sp_backend* b, DevCsr& d, const int64_t* row_ptr, const SpmvTuning& tuning);
+
 // Uploads one CSR matrix with int32 indices and builds its SpMV plan.
 void upload_csr(sp_backend* b, DevCsr& d, const int64_t* row_ptr, const int64_t* col_idx, const double* values,
                 int64_t nrows, int64_t ncols, int64_t nnz, const SpmvTuning& tuning) {
@@ -117,6 +119,12 @@ void upload_csr(sp_backend* b, DevCsr& d, const int64_t* row_ptr, const int64_t*
     d.rp = dupload(b->stream, rp.data(), rp.size());
     d.ci = dupload(b->stream, ci.data(), ci.size());
     d.va = dupload(b->stream, values, (size_t)nnz);
+    plan_csr(b, d, row_ptr, tuning);
+}
+
+// The SpMV plan of a matrix whose arrays are on the device.
+void plan_csr(sp_backend* b, DevCsr& d, const int64_t* row_ptr, const SpmvTuning& tuning) {
+    const int64_t nrows = d.nrows;
     d.tuning = tuning;
     SpmvPlan plan = plan_spmv(row_ptr, nrows, d.tuning);
     d.nblocks = (int)plan.desc.size();
@@ -203,6 +211,8 @@ void free_problem(sp_problem* p) {
     delete p;
 }
 
+void free_csr_public(DevCsr& d) { free_csr(d); }
+
 void free_vec32(sp_vec32* v) {
     if (!v) return;
     dfree(v->d);
@@ -237,12 +247,7 @@ sp_problem* Hip::load(const ScaledProblem& s, bool autotune) {
 sp_problem* Hip::load_scaled(const LpView& p, const ScalingParams& sp, bool autotune, ScaledProblem& s,
                              const int64_t* rank) {
     const int64_t m = p.m, n = p.n;
-    Csr t = transpose_of(m, n, p.row_ptr, p.col_idx, p.values);
-    SpmvTuning ta, tat;
-    if (autotune) {
-        ta = autotuned_plan(p.row_ptr, m);
-        tat = autotuned_plan(t.row_ptr.data(), n);
-    }
+    const SpmvTuning ta = autotune ? autotuned_plan(p.row_ptr, m) : SpmvTuning{};
     auto* prob = new sp_problem;
     prob->b = b;
     prob->m = m;
@@ -254,8 +259,8 @@ sp_problem* Hip::load_scaled(const LpView& p, const ScalingParams& sp, bool auto
     };
     try {
         upload_csr(b, prob->a, p.row_ptr, p.col_idx, p.values, m, n, p.nnz, ta);
-        upload_csr(b, prob->at, t.row_ptr.data(), t.col_idx.data(), t.values.data(), n, m, p.nnz, tat);
-        t = Csr{};
+        std::vector<int64_t> trp;
+        transpose(prob->a, prob->at, autotune, trp);
         DevCsr& A = prob->a;
         DevCsr& T = prob->at;
         double *rs = tmp(m), *cs = tmp(n), *r1 = tmp(m), *r2 = tmp(m), *c1 = tmp(n), *c2 = tmp(n), *rf = tmp(m),
@@ -346,10 +351,89 @@ sp_problem* Hip::load_scaled(const LpView& p, const ScalingParams& sp, bool auto
     return prob;
 }
 
+void Hip::upload(DevCsr& d, const int64_t* row_ptr, const int64_t* col_idx, const double* values, int64_t nrows,
+                 int64_t ncols, int64_t nnz, const SpmvTuning& tuning) {
+    upload_csr(b, d, row_ptr, col_idx, values, nrows, ncols, nnz, tuning);
+}
+
+namespace {
+
+// In-place exclusive scan of `len` ints on the device (multi-level).
+void scan_ints(sp_backend* b, int* a, int64_t len, std::vector<int*>& scratch) {
+    const int64_t per = 4 * kWG;
+    const int64_t nb = (len + per - 1) / per;
+    if (nb == 0) return;
+    scratch.push_back(dmalloc<int>((size_t)nb));
+    int* sums = scratch.back();
+    launch(b, b->k_scan_tiles, (unsigned)nb, a, (long long)len, sums);
+    if (nb > 1) {
+        scan_ints(b, sums, nb, scratch);
+        launch(b, b->k_scan_add, (unsigned)nb, a, (long long)len, (const int*)sums);
+    }
+}
+
+constexpr int kTrTile = 256 * 16;  // TR_TILE in kernels.hip
+
+}  // namespace
+
+void Hip::transpose(const DevCsr& A, DevCsr& T, bool autotune, std::vector<int64_t>& host_rp) {
+    const int nnz = (int)A.nnz, ncols = A.ncols, nrows = A.nrows;
+    T = DevCsr{};
+    T.nrows = ncols;
+    T.ncols = nrows;
+    T.nnz = A.nnz;
+    std::vector<int*> scratch;
+    auto ialloc = [&](int64_t len) {
+        scratch.push_back(dmalloc<int>((size_t)std::max<int64_t>(len, 1)));
+        return scratch.back();
+    };
+    try {
+        T.rp = dmalloc<int>((size_t)ncols + 1);
+        T.ci = dmalloc<int>((size_t)std::max(nnz, 1));
+        T.va = dmalloc<double>((size_t)std::max(nnz, 1));
+        int* keys = ialloc(nnz);
+        if (nnz > 0) {
+            int *rows = ialloc(nnz), *vals = ialloc(nnz), *keys2 = ialloc(nnz), *vals2 = ialloc(nnz);
+            const int ntiles = (nnz + kTrTile - 1) / kTrTile;
+            int* hist = ialloc((int64_t)256 * ntiles);
+            launch(b, b->k_tr_rows, (unsigned)std::max<int64_t>(1, ((int64_t)nrows * 64 + kWG - 1) / kWG),
+                   (const int*)A.rp, nrows, rows);
+            hip_check(hipMemcpyAsync(keys, A.ci, sizeof(int) * (size_t)nnz, hipMemcpyDeviceToDevice, b->stream),
+                      "hipMemcpyAsync D2D");
+            launch(b, b->k_tr_iota, ew_grid(nnz), vals, (long long)nnz);
+            int bits = 1;
+            while (bits < 31 && (int64_t(1) << bits) < ncols) ++bits;
+            for (int shift = 0; shift < bits; shift += 8) {
+                launch(b, b->k_tr_hist, (unsigned)ntiles, (const int*)keys, nnz, shift, hist, ntiles);
+                scan_ints(b, hist, (int64_t)256 * ntiles, scratch);
+                launch(b, b->k_tr_scatter, (unsigned)ntiles, (const int*)keys, (const int*)vals, nnz, shift,
+                       (const int*)hist, ntiles, keys2, vals2);
+                std::swap(keys, keys2);
+                std::swap(vals, vals2);
+            }
+            launch(b, b->k_tr_gather, ew_grid(nnz), (const int*)vals, (const int*)rows, (const double*)A.va,
+                   (long long)nnz, T.ci, T.va);
+        }
+        launch(b, b->k_tr_ptr, (unsigned)((int64_t(nnz) + 1 + kWG - 1) / kWG), (const int*)keys, nnz, ncols, T.rp);
+        std::vector<int> rp((size_t)ncols + 1);
+        hip_check(hipMemcpyAsync(rp.data(), T.rp, sizeof(int) * rp.size(), hipMemcpyDeviceToHost, b->stream),
+                  "hipMemcpyAsync D2H");
+        hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        for (int* q : scratch) dfree(q);
+        scratch.clear();
+        host_rp.assign(rp.begin(), rp.end());
+        plan_csr(b, T, host_rp.data(), autotune ? autotuned_plan(host_rp.data(), ncols) : SpmvTuning{});
+    } catch (...) {
+        (void)hipStreamSynchronize(b->stream);
+        for (int* q : scratch) dfree(q);
+        free_csr(T);
+        throw;
+    }
+}
+
 sp_kkt_report Hip::check(const LpView& p, const double* x, const double* y) {
     const int64_t m = p.m, n = p.n;
     if (!x || !y) throw Error(SP_ERR_INVALID, "check: null vector");
-    Csr t = transpose_of(m, n, p.row_ptr, p.col_idx, p.values);
     DevCsr A, T;
     std::vector<double*> vecs;
     auto up = [&](const double* h, int64_t len) {
@@ -359,8 +443,8 @@ sp_kkt_report Hip::check(const LpView& p, const double* x, const double* y) {
     double r[7] = {0, 0, 0, 0, 0, 0, 0};
     try {
         upload_csr(b, A, p.row_ptr, p.col_idx, p.values, m, n, p.nnz, SpmvTuning{});
-        upload_csr(b, T, t.row_ptr.data(), t.col_idx.data(), t.values.data(), n, m, p.nnz, SpmvTuning{});
-        t = Csr{};
+        std::vector<int64_t> trp;
+        transpose(A, T, false, trp);
         double *dx = up(x, n), *dy = up(y, m), *c = up(p.c, n), *lc = up(p.lc, m), *uc = up(p.uc, m),
                *lv = up(p.lv, n), *uv = up(p.uv, n);
         vecs.push_back(dmalloc<double>((size_t)std::max<int64_t>(m, 1)));
@@ -950,6 +1034,14 @@ sp_backend* sp_backend_open(int ordinal) {
         get(b->k_sc_factor, "sp_sc_factor");
         get(b->k_sc_apply, "sp_sc_apply");
         get(b->k_sc_mul, "sp_sc_mul");
+        get(b->k_tr_rows, "sp_tr_rows");
+        get(b->k_tr_hist, "sp_tr_hist");
+        get(b->k_scan_tiles, "sp_scan_tiles");
+        get(b->k_scan_add, "sp_scan_add");
+        get(b->k_tr_scatter, "sp_tr_scatter");
+        get(b->k_tr_ptr, "sp_tr_ptr");
+        get(b->k_tr_gather, "sp_tr_gather");
+        get(b->k_tr_iota, "sp_tr_iota");
         get(b->k_red_check, "sp_red_check_f64");
         get(b->k_red_finish7, "sp_red_finish7");
         get(b->k_spmv_t_primal, "sp_spmv_t_primal_f64");

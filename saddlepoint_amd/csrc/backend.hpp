@@ -126,6 +126,7 @@ struct sp_backend {
         k_chalpern, k_cdelta_to, k_cdelta_from, k_ccopy, k_cred_dot, k_cred_dist, k_cred_primal, k_cred_dual,
         k_cred_step, k_cred_pray, k_cred_dray, k_cred_finish;
     hipFunction_t k_sc_minmax, k_sc_powsum, k_sc_factor, k_sc_apply, k_sc_mul, k_red_check, k_red_finish7;
+    hipFunction_t k_tr_rows, k_tr_hist, k_scan_tiles, k_scan_add, k_tr_scatter, k_tr_ptr, k_tr_gather, k_tr_iota;
     hipFunction_t k_spmv, k_spmv_t_primal, k_spmv_t_primal_halpern, k_spmv_dual, k_spmv_dual_halpern;
     hipFunction_t k_primal_step, k_dual_step, k_halpern, k_sub, k_scale, k_fill;
     hipFunction_t k_probe_copy, k_probe_triad, k_probe_gather;
@@ -154,6 +155,14 @@ struct Hip {
     // the factors and scaled vectors (its matrices stay empty).
     sp_problem* load_scaled(const LpView& p, const ScalingParams& sp, bool autotune, ScaledProblem& s,
                             const int64_t* rank = nullptr);
+    // The transpose of a device CSR matrix on the device (problem.rs:88-109):
+    // T gets row pointers, column indices, values and its SpMV plan (the
+    // autotuned rule's or the default); its row pointers are also returned
+    // in `host_rp`.
+    void transpose(const DevCsr& A, DevCsr& T, bool autotune, std::vector<int64_t>& host_rp);
+    // Uploads a CSR matrix (int32 indices) and plans it, as load does.
+    void upload(DevCsr& d, const int64_t* row_ptr, const int64_t* col_idx, const double* values, int64_t nrows,
+                int64_t ncols, int64_t nnz, const SpmvTuning& tuning);
     // check() of crates/saddlepoint-check on the device, from the original LP.
     sp_kkt_report check(const LpView& p, const double* x, const double* y);
     // Average ms of one SpMV (A' or, with transpose, A'ᵀ) over reps launches.
@@ -248,6 +257,7 @@ struct Hip {
 void free_problem(sp_problem* p);
 void free_vec(sp_vec* v);
 void free_vec32(sp_vec32* v);
+void free_csr_public(DevCsr& d);  // releases a DevCsr's device arrays
 void free_host(double* h);
 // Folds finished event pairs into the per-kernel totals (synchronizes).
 void profile_collect(sp_backend* b);

@@ -499,6 +499,38 @@ const char* sp_lp_name(const sp_lp* lp) { return lp ? lp->name.c_str() : ""; }
 
 void sp_lp_free(sp_lp* lp) { delete lp; }
 
+int sp_transpose_gpu(sp_backend* b, int64_t nrows, int64_t ncols, const int64_t* row_ptr, const int64_t* col_idx,
+                     const double* values, int64_t* t_row_ptr, int64_t* t_col_idx, double* t_values) {
+    return guarded(b, [&] {
+        if (nrows < 0 || ncols < 0 || !row_ptr || !t_row_ptr) throw sp::Error(SP_ERR_INVALID, "transpose: bad argument");
+        const int64_t nnz = row_ptr[nrows];
+        if (nnz > 0 && (!col_idx || !values || !t_col_idx || !t_values))
+            throw sp::Error(SP_ERR_INVALID, "transpose: null array");
+        for (int64_t k = 0; k < nnz; ++k)
+            if (col_idx[k] < 0 || col_idx[k] >= ncols) throw sp::Error(SP_ERR_INVALID, "transpose: column out of range");
+        sp::DevCsr A, T;
+        sp::Hip h{b};
+        std::vector<int64_t> trp;
+        try {
+            h.upload(A, row_ptr, col_idx, values, nrows, ncols, nnz, sp::SpmvTuning{});
+            h.transpose(A, T, false, trp);
+            std::copy(trp.begin(), trp.end(), t_row_ptr);
+            if (nnz) {
+                std::vector<int> ci((size_t)nnz);
+                sp::hip_check(hipMemcpy(ci.data(), T.ci, 4 * (size_t)nnz, hipMemcpyDeviceToHost), "D2H");
+                sp::hip_check(hipMemcpy(t_values, T.va, 8 * (size_t)nnz, hipMemcpyDeviceToHost), "D2H");
+                for (int64_t k = 0; k < nnz; ++k) t_col_idx[k] = ci[(size_t)k];
+            }
+        } catch (...) {
+            sp::free_csr_public(A);
+            sp::free_csr_public(T);
+            throw;
+        }
+        sp::free_csr_public(A);
+        sp::free_csr_public(T);
+    });
+}
+
 int sp_check(sp_backend* b, const sp_lp_view* lp, const double* x, const double* y, sp_kkt_report* out) {
     return guarded(b, [&] {
         if (!out) throw sp::Error(SP_ERR_INVALID, "check: null output");

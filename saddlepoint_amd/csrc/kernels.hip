@@ -878,6 +878,158 @@ extern "C" __global__ __launch_bounds__(WG) void sp_sc_mul(double* __restrict__ 
     GRID_LOOP(i, n) { v[i] *= f[i]; }
 }
 
+// ---- transpose on the device (problem.rs:88-109) ---------------------------------------
+//
+// A'ᵀ is A's nonzeros ordered by column, and within a column by row: a stable
+// sort of the nonzero indices k by column index (A's nonzeros are already in
+// row order). The sort is a least-significant-digit radix sort, 8 bits per
+// pass: per-tile digit counts, an exclusive scan of the counts (digit-major,
+// tile-minor), then a stable scatter in which each item's rank among equal
+// digits of its tile comes from wave ballots. Everything is integer work, so
+// the result is the host's counting sort exactly.
+
+#define TR_ITEMS 16                 // items per thread per tile
+#define TR_TILE (WG * TR_ITEMS)     // 4096 nonzeros per workgroup
+
+// rows[k] = the row of nonzero k (one wave per row).
+extern "C" __global__ __launch_bounds__(WG) void sp_tr_rows(const int* __restrict__ rp, int nrows,
+                                                            int* __restrict__ rows) {
+    const int row = (int)((blockIdx.x * WG + threadIdx.x) / WAVE), lane = threadIdx.x % WAVE;
+    if (row >= nrows) return;
+    for (int k = rp[row] + lane; k < rp[row + 1]; k += WAVE) rows[k] = row;
+}
+
+// hist[d·ntiles + tile] = items of the tile whose digit is d.
+extern "C" __global__ __launch_bounds__(WG) void sp_tr_hist(const int* __restrict__ keys, int nnz, int shift,
+                                                            int* __restrict__ hist, int ntiles) {
+    __shared__ int cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const i64 base = (i64)blockIdx.x * TR_TILE;
+    for (int c = 0; c < TR_ITEMS; ++c) {
+        const i64 idx = base + c * WG + threadIdx.x;
+        if (idx < nnz) atomicAdd(&cnt[(keys[idx] >> shift) & 255], 1);
+    }
+    __syncthreads();
+    hist[(i64)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Exclusive scan, level 1: each workgroup scans 4·WG ints in place and
+// writes its total to sums[block].
+extern "C" __global__ __launch_bounds__(WG) void sp_scan_tiles(int* __restrict__ a, i64 len,
+                                                               int* __restrict__ sums) {
+    __shared__ int wtot[NWAVES];
+    const i64 base = (i64)blockIdx.x * 4 * WG + 4 * threadIdx.x;
+    int v[4], t = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        v[u] = base + u < len ? a[base + u] : 0;
+        t += v[u];
+    }
+    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+    int incl = t;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const int y = __shfl_up(incl, o, WAVE);
+        if (lane >= o) incl += y;
+    }
+    if (lane == WAVE - 1) wtot[wave] = incl;
+    __syncthreads();
+    int before = 0;
+    for (int w = 0; w < wave; ++w) before += wtot[w];
+    int run = before + incl - t;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        if (base + u < len) a[base + u] = run;
+        run += v[u];
+    }
+    if (threadIdx.x == WG - 1) sums[blockIdx.x] = run;
+}
+
+// Level 2: a[i] += offsets[block of i].
+extern "C" __global__ __launch_bounds__(WG) void sp_scan_add(int* __restrict__ a, i64 len,
+                                                             const int* __restrict__ offsets) {
+    const int add = offsets[blockIdx.x];
+    const i64 base = (i64)blockIdx.x * 4 * WG;
+    for (int u = 0; u < 4; ++u) {
+        const i64 i = base + u * WG + threadIdx.x;
+        if (i < len) a[i] += add;
+    }
+}
+
+// The stable scatter of one pass: the item at position idx of the tile goes to
+// offsets[d·ntiles + tile] + (items of digit d earlier in the tile).
+extern "C" __global__ __launch_bounds__(WG) void sp_tr_scatter(const int* __restrict__ keys_in,
+                                                               const int* __restrict__ vals_in, int nnz, int shift,
+                                                               const int* __restrict__ offsets, int ntiles,
+                                                               int* __restrict__ keys_out,
+                                                               int* __restrict__ vals_out) {
+    __shared__ int run[256];
+    __shared__ int base[256];
+    __shared__ int wcount[NWAVES][256];
+    const int tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
+    run[tid] = 0;
+    base[tid] = offsets[(i64)tid * ntiles + blockIdx.x];
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int c = 0; c < TR_ITEMS; ++c) {
+#pragma unroll
+        for (int w = 0; w < NWAVES; ++w) wcount[w][tid] = 0;
+        __syncthreads();
+        const i64 idx = (i64)blockIdx.x * TR_TILE + c * WG + tid;
+        const bool valid = idx < nnz;
+        const int key = valid ? keys_in[idx] : 0;
+        const int val = valid ? vals_in[idx] : 0;
+        const int d = (key >> shift) & 255;
+        unsigned long long m = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            const unsigned long long on = __ballot(valid && ((d >> bit) & 1));
+            m &= ((d >> bit) & 1) ? on : ~on;
+        }
+        const int rank = __popcll(m & below);
+        if (valid && rank == 0) wcount[wave][d] = __popcll(m);
+        __syncthreads();
+        if (valid) {
+            int pos = base[d] + run[d] + rank;
+            for (int w = 0; w < wave; ++w) pos += wcount[w][d];
+            keys_out[pos] = key;
+            vals_out[pos] = val;
+        }
+        __syncthreads();
+        int add = 0;
+#pragma unroll
+        for (int w = 0; w < NWAVES; ++w) add += wcount[w][tid];
+        run[tid] += add;
+    }
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_tr_iota(int* __restrict__ a, i64 n) {
+    GRID_LOOP(i, n) { a[i] = (int)i; }
+}
+
+// Column pointers from the sorted column indices: tp[j] = first position of
+// column j (empty columns included), tp[ncols] = nnz.
+extern "C" __global__ __launch_bounds__(WG) void sp_tr_ptr(const int* __restrict__ keys, int nnz, int ncols,
+                                                           int* __restrict__ tp) {
+    const i64 q = (i64)blockIdx.x * WG + threadIdx.x;
+    if (q > nnz) return;
+    const int lo = q == 0 ? 0 : keys[q - 1] + 1;
+    const int hi = q == nnz ? ncols : keys[q];
+    for (int j = lo; j <= hi; ++j) tp[j] = (int)q;
+}
+
+// A'ᵀ's row indices and values from the sorted permutation.
+extern "C" __global__ __launch_bounds__(WG) void sp_tr_gather(const int* __restrict__ perm,
+                                                              const int* __restrict__ rows,
+                                                              const double* __restrict__ va, i64 nnz,
+                                                              int* __restrict__ tci, double* __restrict__ tva) {
+    GRID_LOOP(q, nnz) {
+        const int k = perm[q];
+        tci[q] = rows[k];
+        tva[q] = va[k];
+    }
+}
+
 // ---- the KKT checker (crates/saddlepoint-check/src/lib.rs:62-118) ---------------------
 //
 // One pass over rows then columns of a claimed (x, y) with Ax and Aᵀy of the

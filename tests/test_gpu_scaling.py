@@ -82,3 +82,44 @@ def test_device_scaled_solve_is_the_host_scaled_solve(gpu):
     a = solve(gpu, p, default_params(tolerance=1e-4))
     b = solve(gpu, p, default_params(tolerance=1e-4, gpu_scaling=1))
     assert a.iterations == b.iterations and np.array_equal(a.x, b.x)
+
+
+def _random_csr(rng, m, n, nnz, empty_rows=0, empty_cols=0):
+    rows = rng.integers(empty_rows, m, nnz)
+    cols = rng.integers(empty_cols, n, nnz)
+    from saddlepoint_amd.problem import CsrMatrix
+
+    return CsrMatrix.from_triplets(m, n, rows, cols, rng.normal(size=nnz))
+
+
+def transpose_cases():
+    """Built lazily, so that collecting the CPU tests does not generate them."""
+    from saddlepoint_amd.problem import CsrMatrix
+
+    rng = np.random.default_rng(17)
+    m = 50_000
+    return {
+        "afiro": lambda: netlib("afiro").a,
+        "israel": lambda: netlib("israel").a,
+        "25fv47": lambda: netlib("25fv47").a,
+        "config1@0.05": lambda: generate.config1(scale=0.05).a,
+        "config2@0.02": lambda: generate.config2(scale=0.02).a,  # Zipf(1.5) rows
+        "empty rows and columns": lambda: _random_csr(rng, 5000, 7000, 30000, empty_rows=100, empty_cols=200),
+        # One column in every row: a transpose row spanning many sort tiles.
+        "dense column": lambda: CsrMatrix.from_triplets(m, 3, np.arange(m), np.full(m, 1), rng.normal(size=m)),
+        # More than 2^24 columns: four radix passes.
+        "2^25 columns": lambda: _random_csr(rng, 1000, 1 << 25, 200_000),
+        "no nonzeros": lambda: CsrMatrix.from_triplets(4, 6, [], [], []),
+    }
+
+
+@pytest.mark.parametrize("name", list(transpose_cases()))
+def test_device_transpose_is_the_host_transpose(gpu, name):
+    from saddlepoint_amd.backend import transpose_on_device
+
+    a = transpose_cases()[name]()
+    h = a.transpose()
+    d = transpose_on_device(gpu, a)
+    assert np.array_equal(h.row_ptr, d.row_ptr), name
+    assert np.array_equal(h.col_idx, d.col_idx), name
+    assert np.array_equal(h.values, d.values), name
