@@ -244,8 +244,11 @@ sp_problem* Hip::load(const ScaledProblem& s, bool autotune) {
     return load_view(v, &ta, &tat);
 }
 
-sp_problem* Hip::load_scaled(const LpView& p, const ScalingParams& sp, bool autotune, ScaledProblem& s,
-                             const int64_t* rank) {
+sp_problem* Hip::load_scaled(const LpView& p_in, const ScalingParams& sp, bool autotune, ScaledProblem& s,
+                             ColumnOrder* order) {
+    LpView p = p_in;
+    RelabeledLp relabeled;  // c, lv, uv in the new column order (when relabeling)
+    const int64_t* rank = nullptr;
     const int64_t m = p.m, n = p.n;
     const SpmvTuning ta = autotune ? autotuned_plan(p.row_ptr, m) : SpmvTuning{};
     auto* prob = new sp_problem;
@@ -259,6 +262,24 @@ sp_problem* Hip::load_scaled(const LpView& p, const ScalingParams& sp, bool auto
     };
     try {
         upload_csr(b, prob->a, p.row_ptr, p.col_idx, p.values, m, n, p.nnz, ta);
+        if (order) {
+            // Column counts on the device, the order on the host (O(n)), then
+            // the column indices relabeled on the device.
+            std::vector<int> c32((size_t)n, 0);
+            int* dc = dupload(b->stream, c32.data(), c32.size());
+            if (p.nnz) launch(b, b->k_col_count, ew_grid(p.nnz), (const int*)prob->a.ci, (long long)p.nnz, dc);
+            hip_check(hipMemcpyAsync(c32.data(), dc, 4 * (size_t)n, hipMemcpyDeviceToHost, b->stream), "D2H");
+            hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+            *order = popularity_order_of_counts(std::vector<int64_t>(c32.begin(), c32.end()));
+            for (int64_t j = 0; j < n; ++j) c32[(size_t)j] = (int)order->rank[(size_t)j];
+            hip_check(hipMemcpyAsync(dc, c32.data(), 4 * (size_t)n, hipMemcpyHostToDevice, b->stream), "H2D");
+            if (p.nnz) launch(b, b->k_remap, ew_grid(p.nnz), prob->a.ci, (const int*)dc, (long long)p.nnz);
+            hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+            dfree(dc);
+            relabeled = relabel_vectors(p_in, *order);
+            p = relabeled.v;
+            rank = order->rank.data();
+        }
         std::vector<int64_t> trp;
         transpose(prob->a, prob->at, autotune, trp);
         DevCsr& A = prob->a;
@@ -1042,6 +1063,8 @@ sp_backend* sp_backend_open(int ordinal) {
         get(b->k_tr_ptr, "sp_tr_ptr");
         get(b->k_tr_gather, "sp_tr_gather");
         get(b->k_tr_iota, "sp_tr_iota");
+        get(b->k_remap, "sp_remap");
+        get(b->k_col_count, "sp_col_count");
         get(b->k_red_check, "sp_red_check_f64");
         get(b->k_red_finish7, "sp_red_finish7");
         get(b->k_spmv_t_primal, "sp_spmv_t_primal_f64");

@@ -126,7 +126,8 @@ struct sp_backend {
         k_chalpern, k_cdelta_to, k_cdelta_from, k_ccopy, k_cred_dot, k_cred_dist, k_cred_primal, k_cred_dual,
         k_cred_step, k_cred_pray, k_cred_dray, k_cred_finish;
     hipFunction_t k_sc_minmax, k_sc_powsum, k_sc_factor, k_sc_apply, k_sc_mul, k_red_check, k_red_finish7;
-    hipFunction_t k_tr_rows, k_tr_hist, k_scan_tiles, k_scan_add, k_tr_scatter, k_tr_ptr, k_tr_gather, k_tr_iota;
+    hipFunction_t k_tr_rows, k_tr_hist, k_scan_tiles, k_scan_add, k_tr_scatter, k_tr_ptr, k_tr_gather, k_tr_iota,
+        k_remap, k_col_count;
     hipFunction_t k_spmv, k_spmv_t_primal, k_spmv_t_primal_halpern, k_spmv_dual, k_spmv_dual_halpern;
     hipFunction_t k_primal_step, k_dual_step, k_halpern, k_sub, k_scale, k_fill;
     hipFunction_t k_probe_copy, k_probe_triad, k_probe_gather;
@@ -153,8 +154,11 @@ struct Hip {
     // ScaledProblem::new on the device (scaling.rs:53-103): uploads A and its
     // transpose unscaled, runs the scaling passes there, and fills `s` with
     // the factors and scaled vectors (its matrices stay empty).
+    // With `order`, the columns are relabeled by popularity on the device
+    // after the upload (counts, then the order, then the column indices), and
+    // `order` receives the relabeling.
     sp_problem* load_scaled(const LpView& p, const ScalingParams& sp, bool autotune, ScaledProblem& s,
-                            const int64_t* rank = nullptr);
+                            ColumnOrder* order = nullptr);
     // The transpose of a device CSR matrix on the device (problem.rs:88-109):
     // T gets row pointers, column indices, values and its SpMV plan (the
     // autotuned rule's or the default); its row pointers are also returned

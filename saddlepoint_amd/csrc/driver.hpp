@@ -29,8 +29,8 @@ struct DriverBackend : Hip {
     using HostBuffer = std::unique_ptr<double, HostDel>;
     Problem load(const ScaledProblem& s, bool autotune = false) { return Problem(Hip::load(s, autotune)); }
     Problem load_scaled(const LpView& p, const ScalingParams& sp, bool autotune, ScaledProblem& s,
-                        const int64_t* rank = nullptr) {
-        return Problem(Hip::load_scaled(p, sp, autotune, s, rank));
+                        ColumnOrder* order = nullptr) {
+        return Problem(Hip::load_scaled(p, sp, autotune, s, order));
     }
     Vector zeros(int64_t n) { return Vector(Hip::zeros(n)); }
     Vector upload(const double* d, int64_t n) { return Vector(Hip::upload(d, n)); }

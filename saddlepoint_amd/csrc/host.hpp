@@ -80,6 +80,8 @@ struct ColumnOrder {
     bool empty() const { return order.empty(); }
 };
 ColumnOrder popularity_order(const LpView& p);
+// The same order from the column counts.
+ColumnOrder popularity_order_of_counts(const std::vector<int64_t>& count);
 // The LP with relabeled columns; `v` points into the owned arrays.
 struct RelabeledLp {
     LpView v;
@@ -87,6 +89,9 @@ struct RelabeledLp {
     std::vector<double> c, lv, uv;
 };
 RelabeledLp relabel(const LpView& p, const ColumnOrder& o);
+// Only c, lv and uv relabeled (col_idx still the original's): for a caller
+// that relabels the column indices on the device.
+RelabeledLp relabel_vectors(const LpView& p, const ColumnOrder& o);
 // The same relabeling applied to a scaled problem built from the original LP.
 void relabel(ScaledProblem& s, const ColumnOrder& o);
 

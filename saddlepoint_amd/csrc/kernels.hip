@@ -1003,6 +1003,16 @@ extern "C" __global__ __launch_bounds__(WG) void sp_tr_scatter(const int* __rest
     }
 }
 
+// Nonzeros per column (integer counts: the same in any order).
+extern "C" __global__ __launch_bounds__(WG) void sp_col_count(const int* __restrict__ ci, i64 n, int* __restrict__ cnt) {
+    GRID_LOOP(k, n) { atomicAdd(cnt + ci[k], 1); }
+}
+
+// ci[k] ← rank[ci[k]] (the column relabeling of the driver, done on the device).
+extern "C" __global__ __launch_bounds__(WG) void sp_remap(int* __restrict__ ci, const int* __restrict__ rank, i64 n) {
+    GRID_LOOP(k, n) { ci[k] = rank[ci[k]]; }
+}
+
 extern "C" __global__ __launch_bounds__(WG) void sp_tr_iota(int* __restrict__ a, i64 n) {
     GRID_LOOP(i, n) { a[i] = (int)i; }
 }

@@ -221,16 +221,47 @@ std::vector<double> ScaledProblem::dual_residual_factors() const {
 }
 
 ColumnOrder popularity_order(const LpView& p) {
-    ColumnOrder o;
     std::vector<int64_t> count((size_t)p.n, 0);
     for (int64_t k = 0; k < p.nnz; ++k) count[(size_t)p.col_idx[k]] += 1;
-    o.order.resize((size_t)p.n);
-    std::iota(o.order.begin(), o.order.end(), int64_t(0));
-    std::stable_sort(o.order.begin(), o.order.end(),
-                     [&](int64_t a, int64_t b) { return count[(size_t)a] > count[(size_t)b]; });
-    o.rank.resize((size_t)p.n);
-    for (int64_t q = 0; q < p.n; ++q) o.rank[(size_t)o.order[(size_t)q]] = q;
+    return popularity_order_of_counts(count);
+}
+
+ColumnOrder popularity_order_of_counts(const std::vector<int64_t>& count) {
+    ColumnOrder o;
+    const int64_t n = (int64_t)count.size();
+    // A counting sort on the counts, largest first; columns of equal count
+    // keep their index order.
+    int64_t top = 0;
+    for (int64_t c : count) top = std::max(top, c);
+    std::vector<int64_t> start((size_t)top + 2, 0);
+    for (int64_t c : count) start[(size_t)(top - c) + 1] += 1;
+    for (size_t b = 1; b < start.size(); ++b) start[b] += start[b - 1];
+    o.order.resize((size_t)n);
+    o.rank.resize((size_t)n);
+    for (int64_t j = 0; j < n; ++j) {
+        const int64_t q = start[(size_t)(top - count[(size_t)j])]++;
+        o.order[(size_t)q] = j;
+        o.rank[(size_t)j] = q;
+    }
     return o;
+}
+
+RelabeledLp relabel_vectors(const LpView& p, const ColumnOrder& o) {
+    RelabeledLp r;
+    r.v = p;
+    r.c.resize((size_t)p.n);
+    r.lv.resize((size_t)p.n);
+    r.uv.resize((size_t)p.n);
+    for (int64_t q = 0; q < p.n; ++q) {
+        const int64_t j = o.order[(size_t)q];
+        r.c[(size_t)q] = p.c[j];
+        r.lv[(size_t)q] = p.lv[j];
+        r.uv[(size_t)q] = p.uv[j];
+    }
+    r.v.c = r.c.data();
+    r.v.lv = r.lv.data();
+    r.v.uv = r.uv.data();
+    return r;
 }
 
 RelabeledLp relabel(const LpView& p, const ColumnOrder& o) {

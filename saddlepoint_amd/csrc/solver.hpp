@@ -95,17 +95,15 @@ class Solver {
         c_norm_ = norm2(problem.c, problem.n);
         // Columns relabeled by popularity (the solution is returned in the
         // original order; see ColumnOrder).
-        if (params.reorder_columns) cols_ = popularity_order(problem);
         if (params.gpu_scaling) {
-            if (cols_.empty()) {
-                dev_ = b_.load_scaled(problem, scaling_params_of(params), params.spmv_autotune != 0, scaled_);
-            } else {
-                const RelabeledLp r = relabel(problem, cols_);
-                dev_ = b_.load_scaled(r.v, scaling_params_of(params), params.spmv_autotune != 0, scaled_,
-                                      cols_.rank.data());
-            }
+            // The relabeling (when asked for) is computed and applied on the device.
+            dev_ = b_.load_scaled(problem, scaling_params_of(params), params.spmv_autotune != 0, scaled_,
+                                  params.reorder_columns ? &cols_ : nullptr);
         } else {
-            if (!cols_.empty()) relabel(scaled_, cols_);
+            if (params.reorder_columns) {
+                cols_ = popularity_order(problem);
+                relabel(scaled_, cols_);
+            }
             dev_ = b_.load(scaled_, params.spmv_autotune != 0);
         }
         // The scaled matrices live on the device now; the host keeps the factors.
