@@ -102,12 +102,12 @@ def fifo_latencies(arrivals, service):
     return out
 
 
-def run_served(b, lps, params_of, arrivals=None, max_active=64, with_solution=False):
+def run_served(b, lps, params_of, arrivals=None, max_active=64, with_solution=False, use_graphs=None):
     """Serves `lps` (all at t = 0, or at the given arrival times on the
     server's clock); returns (makespan, latencies, results)."""
     from saddlepoint_amd.serving import Server
 
-    srv = Server(b, max_active=max_active)
+    srv = Server(b, max_active=max_active, use_graphs=use_graphs)
     n = len(lps)
     ids = [None] * n
     t_arr = [0.0] * n
@@ -134,6 +134,13 @@ def run_served(b, lps, params_of, arrivals=None, max_active=64, with_solution=Fa
     t_end = srv.clock()
     res = [srv.result(j, with_solution=with_solution) for j in ids]
     lat = [r.completed - ta for r, ta in zip(res, t_arr)]
+    # Saturated throughput: completions while LPs were still waiting for a
+    # slot (up to the last admission), per second of that window.
+    last_admit = max(r.admitted for r in res)
+    window = last_admit - t0
+    done_in_window = sum(1 for r in res if r.completed <= last_admit)
+    run_served.saturated = (done_in_window / window) if window > 0 and done_in_window > 0 else None
+    run_served.window = (window, [i for i, r in enumerate(res) if r.completed <= last_admit])
     srv.close()
     return t_end - t0, lat, [r.result for r in res]
 
@@ -179,12 +186,19 @@ def serving_bench(args) -> dict:
         pr = int(prec == "mixed")
         pm = (lambda i: default_params(tolerance=tols[i], precision=pr, time_limit_s=limit, max_iterations=it_cap,
                                        spmv_autotune=autotune * pr))
-        mk, lat, res = run_served(b, lps, pm, with_solution=args.check)
+        mk, lat, res = run_served(b, lps, pm, with_solution=args.check, max_active=args.max_active)
         st = {}
         for r in res:
             st[r.status] = st.get(r.status, 0) + 1
         entry = {"lps_per_s": len(lps) / mk, "makespan_s": mk, "p50_s": pct(lat, 50), "p99_s": pct(lat, 99),
-                 "statuses": st, "speedup_vs_one_at_a_time": (len(lps) / mk) / cap}
+                 "statuses": st, "speedup_vs_one_at_a_time": (len(lps) / mk) / cap,
+                 "lps_per_s_saturated": run_served.saturated}
+        # The same window measured in work: the one-at-a-time service time of
+        # the LPs finished in it, per second of it (LPs still running count for
+        # nothing, so this underestimates the steady-state speedup).
+        window, finished = run_served.window
+        if window > 0 and finished:
+            entry["work_speedup_saturated"] = sum(service[i] for i in finished) / window
         if args.check:
             # The KKT checker on every served solution (sp_check, on the device).
             from saddlepoint_amd.backend import check
@@ -205,10 +219,13 @@ def serving_bench(args) -> dict:
                                       "objective_gap": abs(r.objective - o.objective) / (1 + abs(o.objective)),
                                       "primal_res": r.primal_residual, "dual_res": r.dual_residual, "gap": r.gap}
                                      for i, (r, o) in enumerate(zip(res, one))]
+        sat = entry["lps_per_s_saturated"]
+        log(f"  saturated: {sat if sat is None else round(sat, 3)} LPs/s while LPs waited for a slot; "
+            f"work speedup {entry.get('work_speedup_saturated')}")
         log(f"served {prec}: {entry['lps_per_s']:.3f} LPs/s (x{entry['speedup_vs_one_at_a_time']:.2f}), "
             f"p50 {entry['p50_s']:.3f}s p99 {entry['p99_s']:.3f}s")
         if args.poisson and pr:
-            _, plat, _ = run_served(b, lps, pm, arrivals)
+            _, plat, _ = run_served(b, lps, pm, arrivals, max_active=args.max_active)
             entry["poisson"] = {"rate_lps_per_s": args.rate_frac * cap, "p50_s": pct(plat, 50),
                                 "p99_s": pct(plat, 99)}
             log(f"  poisson at {args.rate_frac} x capacity: p50 {pct(plat, 50):.3f}s p99 {pct(plat, 99):.3f}s "
@@ -283,6 +300,7 @@ def main() -> int:
                     help="serving stream instead: LPs/s and latency, served (FP64 and mixed) vs one at a time")
     ap.add_argument("--config", type=int, default=3, choices=[3, 4], help="serving stream (3, or 4 with large LPs)")
     ap.add_argument("--lps", type=int, default=64, help="serving: LPs in the sample")
+    ap.add_argument("--max-active", type=int, default=64, help="serving: LPs admitted at once")
     ap.add_argument("--poisson", action="store_true", help="serving: also Poisson arrivals (mixed precision)")
     ap.add_argument("--rate-frac", type=float, default=0.8, help="serving: arrival rate / one-at-a-time capacity")
     ap.add_argument("--lp-time-limit", type=float, default=0.0, help="serving: seconds per LP (0: none)")

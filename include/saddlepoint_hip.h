@@ -155,9 +155,9 @@ typedef struct {
                              and the returned solution stay FP64 */
     int spmv_autotune;    /* 0: every matrix gets the default SpMV plan; 1: the autotuned rule
                              (sp_spmv_tuning_for) picks each matrix's plan from its row lengths */
-    int gpu_scaling;      /* 0: ScaledProblem::new on the host; 1: A and its transpose are uploaded
-                             unscaled and the scaling passes run on the device (bitwise the host's
-                             result where the sums add in the host's order) */
+    int gpu_scaling;      /* 1 (default): A is uploaded unscaled; the transpose and the scaling
+                             passes run on the device (bitwise the host's result for
+                             pock_chambolle_alpha = 1). 0: ScaledProblem::new on the host */
     int reorder_columns;  /* 1 (default): the driver relabels columns by decreasing nonzero count
                              before loading, so the most gathered entries share cache lines; each
                              row keeps its nonzero order and the solution comes back in the

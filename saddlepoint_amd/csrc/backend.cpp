@@ -798,6 +798,11 @@ void Hip::sub(const sp_vec* a, const sp_vec* bv, sp_vec* out) {
         launch(b, b->k_sub, ew_grid(a->len), (const double*)a->d, (const double*)bv->d, out->d, (long long)a->len);
 }
 
+void Hip::scale_inv_sqrt(sp_vec* v, const double* ns) {
+    if (!v || !ns) throw Error(SP_ERR_INVALID, "scale_inv_sqrt: null argument");
+    if (v->len) launch(b, b->k_scale_inv_sqrt, ew_grid(v->len), v->d, ns, (long long)v->len);
+}
+
 void Hip::scale(sp_vec* v, double alpha) {
     if (!v) throw Error(SP_ERR_INVALID, "scale: null vector");
     if (v->len) launch(b, b->k_scale, ew_grid(v->len), v->d, alpha, (long long)v->len);
@@ -1065,6 +1070,7 @@ sp_backend* sp_backend_open(int ordinal) {
         get(b->k_tr_iota, "sp_tr_iota");
         get(b->k_remap, "sp_remap");
         get(b->k_col_count, "sp_col_count");
+        get(b->k_scale_inv_sqrt, "sp_scale_inv_sqrt_f64");
         get(b->k_red_check, "sp_red_check_f64");
         get(b->k_red_finish7, "sp_red_finish7");
         get(b->k_spmv_t_primal, "sp_spmv_t_primal_f64");

@@ -127,7 +127,7 @@ struct sp_backend {
         k_cred_step, k_cred_pray, k_cred_dray, k_cred_finish;
     hipFunction_t k_sc_minmax, k_sc_powsum, k_sc_factor, k_sc_apply, k_sc_mul, k_red_check, k_red_finish7;
     hipFunction_t k_tr_rows, k_tr_hist, k_scan_tiles, k_scan_add, k_tr_scatter, k_tr_ptr, k_tr_gather, k_tr_iota,
-        k_remap, k_col_count;
+        k_remap, k_col_count, k_scale_inv_sqrt;
     hipFunction_t k_spmv, k_spmv_t_primal, k_spmv_t_primal_halpern, k_spmv_dual, k_spmv_dual_halpern;
     hipFunction_t k_primal_step, k_dual_step, k_halpern, k_sub, k_scale, k_fill;
     hipFunction_t k_probe_copy, k_probe_triad, k_probe_gather;
@@ -247,6 +247,9 @@ struct Hip {
     static sp_dual_stats_t dual_from(const double* raw);
     static sp_ray_stats_t primal_ray_from(const double* raw);
     static sp_ray_stats_t dual_ray_from(const double* raw);
+
+    // v *= 1 / sqrt(*ns) with *ns on the device (a result slot).
+    void scale_inv_sqrt(sp_vec* v, const double* ns);
 
     void sync();
 

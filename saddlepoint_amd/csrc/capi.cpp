@@ -275,7 +275,7 @@ void sp_default_params(sp_params* p) {
     p->log_every_checks = 0;
     p->precision = 0;
     p->spmv_autotune = 0;
-    p->gpu_scaling = 0;
+    p->gpu_scaling = 1;
     p->reorder_columns = 1;
 }
 

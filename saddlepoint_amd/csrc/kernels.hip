@@ -578,6 +578,14 @@ extern "C" __global__ __launch_bounds__(WG) void sp_scale_f64(double* __restrict
     GRID_LOOP(i, n) { v[i] *= alpha; }
 }
 
+// v *= 1 / sqrt(*ns): the power iteration's normalisation with the norm left
+// on the device (sqrt and the division are correctly rounded, as on the host).
+extern "C" __global__ __launch_bounds__(WG) void sp_scale_inv_sqrt_f64(double* __restrict__ v,
+                                                                       const double* __restrict__ ns, i64 n) {
+    const double alpha = 1.0 / sqrt(*ns);
+    GRID_LOOP(i, n) { v[i] *= alpha; }
+}
+
 extern "C" __global__ __launch_bounds__(WG) void sp_fill_f64(double* __restrict__ v, double value, i64 n) {
     GRID_LOOP(i, n) { v[i] = value; }
 }

@@ -61,6 +61,8 @@ struct BInst {
 using Clock = std::chrono::steady_clock;
 
 constexpr uint64_t kChunk = 64;
+// LPs with fewer nonzeros are scaled by the prep threads on the host.
+constexpr int64_t kHostScalingBelow = 2'000'000;
 
 template <class T>
 T* dalloc(size_t n) {
@@ -100,6 +102,7 @@ struct Job {
     double t_submit = 0, t_admit = -1, t_done = -1;
     int state = kQueued;
     std::unique_ptr<ScaledProblem> scaled;
+    ColumnOrder cols;  // the relabeling already applied to `scaled` (host scaling)
     std::unique_ptr<Solver<DriverBackend>> solver;
     int64_t footprint = 0;
     sp_solution sol{};
@@ -247,15 +250,19 @@ class Server {
             try {
                 const LpView v = j->view();
                 validate(v);
-                ScalingParams s;
-                s.geometric_mean_passes = j->params.geometric_mean_passes;
-                s.ruiz_iterations = j->params.ruiz_iterations;
-                s.has_pock_chambolle = j->params.has_pock_chambolle != 0;
-                s.pock_chambolle_alpha = j->params.pock_chambolle_alpha;
-                s.bound_objective_rescaling = j->params.bound_objective_rescaling != 0;
-                // With device scaling, admission scales the LP on the GPU.
-                j->scaled = std::make_unique<ScaledProblem>(j->params.gpu_scaling ? ScaledProblem{}
-                                                                                   : ScaledProblem::build(v, s));
+                // Small LPs are scaled (and relabeled) here, off the GPU's
+                // stream; large ones by the device at admission. Both give
+                // bitwise the same scaled problem.
+                if (j->params.gpu_scaling && v.nnz < kHostScalingBelow) j->params.gpu_scaling = 0;
+                if (j->params.gpu_scaling) {
+                    j->scaled = std::make_unique<ScaledProblem>();
+                } else {
+                    j->scaled = std::make_unique<ScaledProblem>(ScaledProblem::build(v, scaling_params_of(j->params)));
+                    if (j->params.reorder_columns) {
+                        j->cols = popularity_order(v);
+                        relabel(*j->scaled, j->cols);
+                    }
+                }
                 std::lock_guard<std::mutex> g(mu_);
                 j->state = kPrepared;
                 ready_.push_back(j);
@@ -289,7 +296,8 @@ class Server {
                 j = f;
             }
             try {
-                j->solver = std::make_unique<Solver<DriverBackend>>(db_, j->view(), j->params, std::move(*j->scaled));
+                j->solver = std::make_unique<Solver<DriverBackend>>(db_, j->view(), j->params, std::move(*j->scaled),
+                                                                    std::move(j->cols));
                 j->scaled.reset();
                 j->solver->start();
                 j->t_admit = clock();

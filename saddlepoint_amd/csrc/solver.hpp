@@ -88,9 +88,11 @@ class Solver {
     // norms that run() needs.
     // With params.gpu_scaling, `scaled` is ignored: the backend uploads the LP
     // unscaled and scales it on the device, and scaled_ receives the factors.
-    Solver(B& b, const LpView& problem, const sp_params& params, ScaledProblem scaled)
-        : b_(b), p_(params), scaled_(std::move(scaled)), m_(problem.m), n_(problem.n), nnz_(problem.nnz),
-          c0_(problem.c0), maximize_(problem.maximize) {
+    // With host scaling, `relabeled` (when not empty) is the column order the
+    // caller already applied to `scaled`.
+    Solver(B& b, const LpView& problem, const sp_params& params, ScaledProblem scaled, ColumnOrder relabeled = {})
+        : b_(b), p_(params), scaled_(std::move(scaled)), cols_(std::move(relabeled)), m_(problem.m), n_(problem.n),
+          nnz_(problem.nnz), c0_(problem.c0), maximize_(problem.maximize) {
         b_norm_ = finite_bound_norm(problem.lc, problem.uc, problem.m);
         c_norm_ = norm2(problem.c, problem.n);
         // Columns relabeled by popularity (the solution is returned in the
@@ -100,7 +102,7 @@ class Solver {
             dev_ = b_.load_scaled(problem, scaling_params_of(params), params.spmv_autotune != 0, scaled_,
                                   params.reorder_columns ? &cols_ : nullptr);
         } else {
-            if (params.reorder_columns) {
+            if (params.reorder_columns && cols_.empty()) {
                 cols_ = popularity_order(problem);
                 relabel(scaled_, cols_);
             }
@@ -114,8 +116,8 @@ class Solver {
         x_anchor_ = b_.zeros(n), y_anchor_ = b_.zeros(m), ax_ = b_.zeros(m), aty_ = b_.zeros(n);
         tmp_n_ = b_.zeros(n), tmp_m_ = b_.zeros(m), tmp_m2_ = b_.zeros(m);
         c_ = b_.upload(scaled_.c.data(), n);
-        slots_ = b_.zeros(kSlotCount);
-        host_slots_ = b_.host_buffer(kSlotCount);
+        slots_ = b_.zeros(kSlotsAllocated);
+        host_slots_ = b_.host_buffer(kSlotsAllocated);
         if (params.precision == 1) {
             if (!params.fused) throw Error(SP_ERR_INVALID, "mixed precision needs the fused path");
             b_.ensure_fp32(dev_.get());
@@ -462,7 +464,9 @@ class Solver {
         kPrimalRay = 24,  // primal_ray_stats sums (4)
         kDistX = 28,      // ‖x_t - x_anchor‖² (1)
         kDistY = 29,      // ‖y_t - y_anchor‖² (1)
-        kSlotCount = 30
+        kSlotCount = 30,
+        kPower = 30,      // power iteration: one block of ‖v‖² (8)
+        kSlotsAllocated = 38
     };
 
     void enqueue_fixed_point(int base) {
@@ -697,15 +701,31 @@ class Solver {
         const double s = 1.0 / std::sqrt(b_.norm_sq(v.get()));
         b_.scale(v.get(), s);
         double lambda = 0.0;
-        for (int64_t it = 0; it < p_.power_iterations; ++it) {
-            b_.spmv(dev_.get(), v.get(), tmp_m_.get());
-            b_.spmv_t(dev_.get(), tmp_m_.get(), v.get());
-            const double norm = std::sqrt(b_.norm_sq(v.get()));
-            if (std::isnan(norm) || norm <= 0.0 || std::isinf(norm))
-                throw Error(SP_ERR_NUMERICAL, "numerical breakdown: power iteration collapsed");
-            b_.scale(v.get(), 1.0 / norm);
-            const bool done = std::fabs(norm - lambda) <= 1e-6 * norm;
-            lambda = norm;
+        // Rounds run in blocks of kPowerBlock without a host synchronization:
+        // each round's ‖v‖² goes to a result slot and v is normalised on the
+        // device by the same arithmetic as 1 / sqrt on the host. The host then
+        // walks the block's norms in order and stops where solver.rs:215-242
+        // stops; rounds after that only touched v and tmp_m, which are not
+        // used afterwards.
+        constexpr int kPowerBlock = 8;
+        for (int64_t it0 = 0; it0 < p_.power_iterations; it0 += kPowerBlock) {
+            const int rounds = (int)std::min<int64_t>(kPowerBlock, p_.power_iterations - it0);
+            for (int r = 0; r < rounds; ++r) {
+                b_.spmv(dev_.get(), v.get(), tmp_m_.get());
+                b_.spmv_t(dev_.get(), tmp_m_.get(), v.get());
+                b_.dot_into(v.get(), v.get(), slot(kPower + r));
+                b_.scale_inv_sqrt(v.get(), slot(kPower + r));
+            }
+            b_.fetch_vec_async(slots_.get(), kPower + rounds, host_slots_.get());
+            b_.sync();
+            bool done = false;
+            for (int r = 0; r < rounds && !done; ++r) {
+                const double norm = std::sqrt(host_slots_.get()[kPower + r]);
+                if (std::isnan(norm) || norm <= 0.0 || std::isinf(norm))
+                    throw Error(SP_ERR_NUMERICAL, "numerical breakdown: power iteration collapsed");
+                done = std::fabs(norm - lambda) <= 1e-6 * norm;
+                lambda = norm;
+            }
             if (done) break;
         }
         // Power iteration underestimates; pad slightly so the step stays safe.

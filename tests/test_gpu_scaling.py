@@ -74,12 +74,12 @@ def test_device_scaling_options(gpu, kw):
 def test_device_scaled_solve_is_the_host_scaled_solve(gpu):
     for name in ("afiro", "israel", "shell"):
         p = netlib(name)
-        a = solve(gpu, p, default_params())
+        a = solve(gpu, p, default_params(gpu_scaling=0))
         b = solve(gpu, p, default_params(gpu_scaling=1))
         assert a.status == b.status and a.iterations == b.iterations, name
         assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y), name
     p = generate.config0()
-    a = solve(gpu, p, default_params(tolerance=1e-4))
+    a = solve(gpu, p, default_params(tolerance=1e-4, gpu_scaling=0))
     b = solve(gpu, p, default_params(tolerance=1e-4, gpu_scaling=1))
     assert a.iterations == b.iterations and np.array_equal(a.x, b.x)
 
