@@ -205,3 +205,15 @@ def test_edge_case_lps_served_as_alone(gpu):
         assert r.status == g.status and r.iterations == g.iterations, name
         assert np.array_equal(r.x, g.x) and np.array_equal(r.y, g.y), name
     s.close()
+
+
+def test_fused_path_is_bitwise_the_method_path_with_long_rows(gpu):
+    """Rows of several SpMV chunks (config 1's dense rows): the fused A-side
+    kernel adds their chunk sums itself, in the finishing kernel's order, so
+    the iterates are still bitwise those of the method path."""
+    p = generate.config1(scale=0.1)
+    assert np.diff(p.a.row_ptr).max() > 4096
+    a = solve(gpu, p, default_params(fused=0, max_iterations=640))
+    b = solve(gpu, p, default_params(fused=1, max_iterations=640))
+    assert a.iterations == b.iterations == 640 and a.restarts == b.restarts
+    assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
