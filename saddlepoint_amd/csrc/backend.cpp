@@ -1055,6 +1055,7 @@ sp_backend* sp_backend_open(int ordinal) {
         get(b->k_cred_pray, "sp_cred_primal_ray_f64");
         get(b->k_cred_dray, "sp_cred_dual_ray_f64");
         get(b->k_cred_finish, "sp_cred_finish");
+        get(b->k_cscale_inv_sqrt, "sp_cscale_inv_sqrt_f64");
         get(b->k_sc_minmax, "sp_sc_minmax");
         get(b->k_sc_powsum, "sp_sc_powsum");
         get(b->k_sc_factor, "sp_sc_factor");

@@ -124,7 +124,7 @@ struct sp_backend {
     // Batched check operations (serving loop).
     hipFunction_t k_cspmv_store, k_cspmv_primal, k_cspmv_dual, k_cfin_store, k_cfin_primal, k_cfin_dual, k_csub,
         k_chalpern, k_cdelta_to, k_cdelta_from, k_ccopy, k_cred_dot, k_cred_dist, k_cred_primal, k_cred_dual,
-        k_cred_step, k_cred_pray, k_cred_dray, k_cred_finish;
+        k_cred_step, k_cred_pray, k_cred_dray, k_cred_finish, k_cscale_inv_sqrt;
     hipFunction_t k_sc_minmax, k_sc_powsum, k_sc_factor, k_sc_apply, k_sc_mul, k_red_check, k_red_finish7;
     hipFunction_t k_tr_rows, k_tr_hist, k_scan_tiles, k_scan_add, k_tr_scatter, k_tr_ptr, k_tr_gather, k_tr_iota,
         k_remap, k_col_count, k_scale_inv_sqrt;

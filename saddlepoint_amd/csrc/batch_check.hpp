@@ -459,6 +459,39 @@ inline void enqueue_plain_entries(Phase& P, const Hip& h, const std::vector<Plai
     P.elementwise(b->k_cdelta_from, d);
 }
 
+using PowerWork = Solver<DriverBackend>::PowerWork;
+
+// `rounds` power-iteration rounds of every entry (Solver::start's rounds):
+// tmp_m = A'v, v = A'ᵀtmp_m, slot = v·v, v *= 1/sqrt(slot).
+inline void enqueue_power_rounds(Phase& P, const Hip& h, const std::vector<PowerWork>& ws, int rounds) {
+    if (ws.empty()) return;
+    sp_backend* b = h.b;
+    std::vector<sp_problem*> probs;
+    for (const PowerWork& w : ws) probs.push_back(w.problem);
+    constexpr int base = Solver<DriverBackend>::slot_power();
+    for (int r = 0; r < rounds; ++r) {
+        std::vector<COp> a, t, d, s;
+        for (const PowerWork& w : ws) {
+            COp o;
+            o.in = w.v->d, o.o0 = w.tmp_m->d;
+            a.push_back(o);
+            COp q;
+            q.in = w.tmp_m->d, q.o0 = w.v->d;
+            t.push_back(q);
+            COp dd;
+            dd.a0 = w.v->d, dd.a1 = w.v->d, dd.n = w.v->len, dd.dst = w.slots + base + r;
+            d.push_back(dd);
+            COp sc;
+            sc.o0 = w.v->d, sc.a0 = w.slots + base + r, sc.n = w.v->len;
+            s.push_back(sc);
+        }
+        P.spmv(b->k_cspmv_store, b->k_cfin_store, a, probs, true);
+        P.spmv(b->k_cspmv_store, b->k_cfin_store, t, probs, false);
+        P.reduce(b->k_cred_dot, 1, d, h, [](const COp& o) { return (int64_t)o.n; });
+        P.elementwise(b->k_cscale_inv_sqrt, s);
+    }
+}
+
 // The restart copies of a check phase, in one launch.
 inline void enqueue_copies(Phase& P, const Hip& h, const std::vector<std::pair<sp_vec*, sp_vec*>>& copies) {
     std::vector<COp> v;

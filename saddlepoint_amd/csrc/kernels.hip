@@ -1406,6 +1406,14 @@ extern "C" __global__ __launch_bounds__(WG) void sp_cdelta_from(const CBlk* __re
     for (i64 i = i0 + threadIdx.x; i < i1; i += WG) o.f0[i] = (float)(o.a0[i] - o.a1[i]);
 }
 
+// o0 *= 1 / sqrt(a0[0]) (the power iteration's normalisation, a0 a result slot)
+extern "C" __global__ __launch_bounds__(WG) void sp_cscale_inv_sqrt_f64(const CBlk* __restrict__ blk,
+                                                                        const COp* __restrict__ ops) {
+    CELEM_HEAD
+    const double alpha = 1.0 / sqrt(o.a0[0]);
+    for (i64 i = i0 + threadIdx.x; i < i1; i += WG) o.o0[i] *= alpha;
+}
+
 // o0 = a0 (gathers every LP's result slots into one buffer for one readback,
 // and the restart copies)
 extern "C" __global__ __launch_bounds__(WG) void sp_ccopy_f64(const CBlk* __restrict__ blk,

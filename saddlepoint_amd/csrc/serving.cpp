@@ -284,14 +284,18 @@ class Server {
     }
 
     void admit() {
+        // Take LPs while the budget and the slots allow; their power
+        // iterations then run together.
+        std::vector<Job*> fresh;
         for (;;) {
             Job* j = nullptr;
             {
                 std::lock_guard<std::mutex> g(mu_);
-                if (ready_.empty()) return;
+                if (ready_.empty()) break;
                 Job* f = ready_.front();
-                const bool fits = used_ + f->footprint <= sp_.budget_bytes || active_.empty();
-                if ((int)active_.size() >= sp_.max_active || !fits) return;
+                const bool empty = active_.empty() && fresh.empty();
+                const bool fits = used_ + f->footprint <= sp_.budget_bytes || empty;
+                if ((int)(active_.size() + fresh.size()) >= sp_.max_active || !fits) break;
                 ready_.pop_front();
                 j = f;
             }
@@ -299,17 +303,79 @@ class Server {
                 j->solver = std::make_unique<Solver<DriverBackend>>(db_, j->view(), j->params, std::move(*j->scaled),
                                                                     std::move(j->cols));
                 j->scaled.reset();
-                j->solver->start();
-                j->t_admit = clock();
-                {
-                    std::lock_guard<std::mutex> g(mu_);
-                    j->state = kActive;
-                }
+                j->solver->power_begin();
                 used_ += j->footprint;
-                active_.push_back(j);
+                fresh.push_back(j);
             } catch (const Error& e) {
                 j->solver.reset();
                 fail(j, e.code, e.what());
+            }
+        }
+        if (fresh.empty()) return;
+        power_iterations(fresh);
+        for (Job* j : fresh) {
+            if (!j->failed_now) {
+                try {
+                    j->solver->power_end();
+                } catch (const Error& e) {
+                    mark_failed(j, e);
+                }
+            }
+            if (j->failed_now) {
+                j->solver.reset();
+                used_ -= j->footprint;
+                fail(j, j->err, j->msg);
+                continue;
+            }
+            j->t_admit = clock();
+            {
+                std::lock_guard<std::mutex> g(mu_);
+                j->state = kActive;
+            }
+            active_.push_back(j);
+        }
+    }
+
+    // The power iterations of newly admitted LPs, batched: blocks of rounds
+    // with one readback of every LP's norms per block (Solver::start's
+    // arithmetic, so each LP's step size is the one of its single solve).
+    void power_iterations(const std::vector<Job*>& fresh) {
+        constexpr int K = Solver<DriverBackend>::power_block();
+        constexpr int base = Solver<DriverBackend>::slot_power();
+        for (;;) {
+            std::vector<Job*> live;
+            std::vector<PowerWork> ws;
+            int rounds = 0;
+            for (Job* j : fresh) {
+                if (j->failed_now || j->solver->power_done()) continue;
+                const int r = j->solver->power_rounds_next();
+                if (live.empty()) rounds = r;
+                if (r != rounds) continue;  // a different cap: its own block later
+                live.push_back(j);
+                ws.push_back(j->solver->power_work());
+            }
+            if (live.empty()) return;
+            ensure_gather(live.size() * K);
+            Phase P(b_);
+            enqueue_power_rounds(P, db_, ws, rounds);
+            std::vector<COp> copy;
+            for (size_t i = 0; i < ws.size(); ++i) {
+                COp o;
+                o.a0 = ws[i].slots + base, o.o0 = gath_d_ + i * K, o.n = rounds;
+                copy.push_back(o);
+            }
+            P.elementwise(b_->k_ccopy, copy);
+            P.run(arena_, cpart_, cpart_cap_);
+            hip_check(hipMemcpyAsync(gath_h_, gath_d_, ws.size() * K * sizeof(double), hipMemcpyDeviceToHost,
+                                     b_->stream),
+                      "hipMemcpyAsync D2H");
+            hip_check(hipStreamSynchronize(b_->stream), "hipStreamSynchronize");
+            for (size_t i = 0; i < live.size(); ++i) {
+                try {
+                    live[i]->solver->power_take(gath_h_ + i * K, rounds);
+                } catch (const Error& e) {
+                    mark_failed(live[i], e);
+                }
             }
         }
     }

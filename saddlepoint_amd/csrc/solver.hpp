@@ -137,9 +137,85 @@ class Solver {
     // The part of run() before the loop (solver.rs:312-330): the clock starts,
     // power iteration sets η, and ω starts at 1 (or the norm ratio).
     void start() {
+        power_begin();
+        while (!power_done_) {
+            const int rounds = power_rounds_next();
+            for (int r = 0; r < rounds; ++r) {
+                b_.spmv(dev_.get(), power_v_.get(), tmp_m_.get());
+                b_.spmv_t(dev_.get(), tmp_m_.get(), power_v_.get());
+                b_.dot_into(power_v_.get(), power_v_.get(), slot(kPower + r));
+                b_.scale_inv_sqrt(power_v_.get(), slot(kPower + r));
+            }
+            b_.fetch_vec_async(slots_.get(), kPower + rounds, host_slots_.get());
+            b_.sync();
+            power_take(host_slots_.get() + kPower, rounds);
+        }
+        power_end();
+    }
+
+    // ---- start() in phases, for a caller that batches the power iteration
+    // of many solvers (the serving loop) ----
+    //
+    // Largest singular value of A' by power iteration on A'ᵀA'
+    // (solver.rs:215-242). power_begin() starts the clock and enqueues the
+    // start vector's normalisation; then, in blocks of at most power_block()
+    // rounds, the caller runs for r < power_rounds_next():
+    //     tmp_m = A' v;  v = A'ᵀ tmp_m;  slots[slot_power() + r] = v·v;  v *= 1 / sqrt(that)
+    // and passes the read-back ‖v‖² values to power_take(), until
+    // power_done(); power_end() sets η, ω and the PID. The rounds after the
+    // stopping one (inside a block) touch only v and tmp_m, which are not
+    // used afterwards, so the result is the reference's.
+    struct PowerWork {
+        sp_problem* problem;
+        sp_vec *v, *tmp_m;
+        double* slots;  // device result slots
+    };
+    static constexpr int power_block() { return kPowerBlock; }
+    static constexpr int slot_power() { return kPower; }
+    bool power_done() const { return power_done_; }
+    PowerWork power_work() { return PowerWork{dev_.get(), power_v_.get(), tmp_m_.get(), slots_->d}; }
+    int power_rounds_next() const {
+        return (int)std::min<int64_t>(kPowerBlock, p_.power_iterations - power_rounds_);
+    }
+
+    void power_begin() {
         start_ = clock::now();
         started_ = true;
-        eta_ = p_.step_size_factor / estimate_norm();
+        power_lambda_ = 0.0;
+        power_rounds_ = 0;
+        power_trivial_ = n_ == 0 || nnz_ == 0;
+        power_done_ = power_trivial_ || p_.power_iterations <= 0;
+        if (power_trivial_) return;
+        std::vector<double> start((size_t)n_);
+        for (int64_t j = 0; j < n_; ++j) {
+            const int64_t orig = cols_.empty() ? j : cols_.order[(size_t)j];
+            start[(size_t)j] = 1.0 + 0.01 * (double)(orig % 7);
+        }
+        power_v_ = b_.upload(start.data(), n_);
+        // v *= 1 / ‖v‖, on the device (the same arithmetic as on the host).
+        b_.dot_into(power_v_.get(), power_v_.get(), slot(kPowerStart));
+        b_.scale_inv_sqrt(power_v_.get(), slot(kPowerStart));
+    }
+
+    // `norms` holds v·v of the `rounds` rounds just run.
+    void power_take(const double* norms, int rounds) {
+        bool done = false;
+        for (int r = 0; r < rounds && !done; ++r) {
+            const double norm = std::sqrt(norms[r]);
+            if (std::isnan(norm) || norm <= 0.0 || std::isinf(norm))
+                throw Error(SP_ERR_NUMERICAL, "numerical breakdown: power iteration collapsed");
+            done = std::fabs(norm - power_lambda_) <= 1e-6 * norm;
+            power_lambda_ = norm;
+        }
+        power_rounds_ += rounds;
+        power_done_ = done || power_rounds_ >= p_.power_iterations;
+    }
+
+    void power_end() {
+        // Power iteration underestimates; pad slightly so the step stays safe.
+        const double norm = power_trivial_ ? 1.0 : std::sqrt(power_lambda_) * 1.01;
+        power_v_.reset();
+        eta_ = p_.step_size_factor / norm;
         omega_ = p_.bound_objective_rescaling ? 1.0 : (1.0 + c_norm_) / (1.0 + b_norm_);
         pid_ = PidState(omega_);
     }
@@ -465,9 +541,11 @@ class Solver {
         kDistX = 28,      // ‖x_t - x_anchor‖² (1)
         kDistY = 29,      // ‖y_t - y_anchor‖² (1)
         kSlotCount = 30,
-        kPower = 30,      // power iteration: one block of ‖v‖² (8)
-        kSlotsAllocated = 38
+        kPower = 30,      // power iteration: one block of ‖v‖² (kPowerBlock)
+        kPowerStart = 38, // power iteration: ‖v‖² of the start vector
+        kSlotsAllocated = 39
     };
+    static constexpr int kPowerBlock = 8;
 
     void enqueue_fixed_point(int base) {
         b_.sub(x_t_.get(), x_.get(), tmp_n_.get());
@@ -675,6 +753,11 @@ class Solver {
     bool pending_k_was_zero_ = false;
     bool check_pending_ = false;
     std::vector<std::pair<sp_vec*, sp_vec*>>* copy_sink_ = nullptr;  // restart copies deferred to a batching caller
+    // Power iteration state (start() or its phases).
+    Vec power_v_;
+    double power_lambda_ = 0.0;
+    int64_t power_rounds_ = 0;
+    bool power_done_ = false, power_trivial_ = false;
     double* slot(int i) { return slots_->d + i; }
 
    private:
@@ -686,50 +769,6 @@ class Solver {
         status_ = status;
         terminated_ = true;
         solve_time_ = elapsed();
-    }
-
-    // Largest singular value of A' by power iteration on A'ᵀA' (solver.rs:215-242).
-    double estimate_norm() {
-        const int64_t n = n_;
-        if (n == 0 || nnz_ == 0) return 1.0;
-        std::vector<double> start((size_t)n);
-        for (int64_t j = 0; j < n; ++j) {
-            const int64_t orig = cols_.empty() ? j : cols_.order[(size_t)j];
-            start[(size_t)j] = 1.0 + 0.01 * (double)(orig % 7);
-        }
-        Vec v = b_.upload(start.data(), n);
-        const double s = 1.0 / std::sqrt(b_.norm_sq(v.get()));
-        b_.scale(v.get(), s);
-        double lambda = 0.0;
-        // Rounds run in blocks of kPowerBlock without a host synchronization:
-        // each round's ‖v‖² goes to a result slot and v is normalised on the
-        // device by the same arithmetic as 1 / sqrt on the host. The host then
-        // walks the block's norms in order and stops where solver.rs:215-242
-        // stops; rounds after that only touched v and tmp_m, which are not
-        // used afterwards.
-        constexpr int kPowerBlock = 8;
-        for (int64_t it0 = 0; it0 < p_.power_iterations; it0 += kPowerBlock) {
-            const int rounds = (int)std::min<int64_t>(kPowerBlock, p_.power_iterations - it0);
-            for (int r = 0; r < rounds; ++r) {
-                b_.spmv(dev_.get(), v.get(), tmp_m_.get());
-                b_.spmv_t(dev_.get(), tmp_m_.get(), v.get());
-                b_.dot_into(v.get(), v.get(), slot(kPower + r));
-                b_.scale_inv_sqrt(v.get(), slot(kPower + r));
-            }
-            b_.fetch_vec_async(slots_.get(), kPower + rounds, host_slots_.get());
-            b_.sync();
-            bool done = false;
-            for (int r = 0; r < rounds && !done; ++r) {
-                const double norm = std::sqrt(host_slots_.get()[kPower + r]);
-                if (std::isnan(norm) || norm <= 0.0 || std::isinf(norm))
-                    throw Error(SP_ERR_NUMERICAL, "numerical breakdown: power iteration collapsed");
-                done = std::fabs(norm - lambda) <= 1e-6 * norm;
-                lambda = norm;
-            }
-            if (done) break;
-        }
-        // Power iteration underestimates; pad slightly so the step stays safe.
-        return std::sqrt(lambda) * 1.01;
     }
 
     // solver.rs:245-258. Leaves A'ᵀy_t in tmp_n.
