@@ -82,9 +82,20 @@ def cpu_baseline(problem, seconds_budget: float) -> dict:
         t, _ = O.time_iterations(problem, k, params)
     else:
         t = t64
+    # A per-core figure: the same restatement on one thread, 64 iterations.
+    O.set_threads(1)
+    t1, _ = O.time_iterations(problem, 64, params)
+    O.set_threads(cores)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": k / t, "unit": "iters/s", "cores": cores, "kind": "port",
             "sample": f"{k} iterations of the same instance (check work included, power iteration and "
-                      f"{scale_s:.1f}s of host scaling excluded), oracle C restatement with {cores} OpenMP threads"}
+                      f"{scale_s:.1f}s of host scaling excluded), oracle C restatement with {cores} OpenMP threads",
+            "single_thread_value": 64 / t1, "cpu_model": model, "nproc": os.cpu_count()}
 
 
 def pct(v, q):
@@ -280,6 +291,61 @@ def large_bench(args) -> dict:
     return out
 
 
+def share_bench(args) -> dict:
+    """One GPU's share of config 2 split across P GPUs by row blocks: A' in P
+    contiguous row blocks and A'ᵀ in P contiguous row blocks, each cut where
+    the nonzero count crosses a multiple of nnz/P. Every block is loaded on
+    this GPU at its real shape (all columns, so the gathered table is the
+    full x̄ or y) and its SpMV timed; the slowest block of each side is the
+    per-iteration critical path, beside the all-gathers such a split needs
+    (y and x̄ in full, 8(m+n) bytes, of which (P-1)/P arrive over xGMI)."""
+    from saddlepoint_amd import generate
+    from saddlepoint_amd.autotune import structural
+    from saddlepoint_amd.backend import HipBackend
+    from saddlepoint_amd.problem import CsrMatrix, LpProblem
+
+    P = args.share
+    t0 = time.time()
+    p = generate.config2(scale=args.scale)
+    at = p.a.transpose()
+    log(f"generated config2 m={p.nrows} n={p.ncols} nnz={p.a.nnz} in {time.time() - t0:.1f}s")
+    b = HipBackend(0)
+
+    def lp_of(a):
+        z = np.zeros
+        return LpProblem(a=a, c=z(a.ncols), lc=z(a.nrows), uc=z(a.nrows), lv=z(a.ncols), uv=z(a.ncols))
+
+    def timed(a, reps=20):
+        pg = b.load(structural(lp_of(a)))
+        try:
+            return b.spmv_time(pg, transpose=False, reps=reps)
+        finally:
+            pg.free()
+
+    out = {"m": p.nrows, "n": p.ncols, "nnz": p.a.nnz, "gpus": P, "scale": args.scale}
+    for side, mat in (("A", p.a), ("At", at)):
+        rp = mat.row_ptr
+        cuts = np.searchsorted(rp, np.arange(P + 1) * (rp[-1] / P), side="left")
+        cuts[0], cuts[-1] = 0, mat.nrows
+        blocks = []
+        for r0, r1 in zip(cuts[:-1], cuts[1:]):
+            k0, k1 = int(rp[r0]), int(rp[r1])
+            blk = CsrMatrix(int(r1 - r0), mat.ncols, (rp[r0:r1 + 1] - rp[r0]).copy(), mat.col_idx[k0:k1].copy(),
+                            mat.values[k0:k1].copy())
+            ms = timed(blk)
+            blocks.append({"rows": int(r1 - r0), "nnz": k1 - k0, "us": ms * 1e3})
+            log(f"  {side} block rows {r0}-{r1}: nnz {k1 - k0}, {ms * 1e3:.1f} us")
+        full = timed(mat, reps=10)
+        log(f"  {side} whole on one GPU: {full * 1e3:.1f} us")
+        out[side] = {"blocks": blocks, "max_block_us": max(x["us"] for x in blocks), "one_gpu_us": full * 1e3}
+    gather_bytes = 8 * (p.nrows + p.ncols) * (P - 1) / P
+    out["allgather_bytes_per_gpu"] = gather_bytes
+    out["spmv_critical_us"] = out["A"]["max_block_us"] + out["At"]["max_block_us"]
+    out["spmv_one_gpu_us"] = out["A"]["one_gpu_us"] + out["At"]["one_gpu_us"]
+    b.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -296,6 +362,8 @@ def main() -> int:
                     help="config 2 instead: FP64 vs mixed precision at 1e-4 and 1e-6 (use --scale to shrink)")
     ap.add_argument("--max-iterations", type=int, default=200_000)
     ap.add_argument("--time-limit", type=float, default=150.0, help="--large: seconds per solve")
+    ap.add_argument("--share", type=int, default=0,
+                    help="config 2's SpMVs on one GPU's share of a P-way row-block split (P = this value)")
     ap.add_argument("--serving", action="store_true",
                     help="serving stream instead: LPs/s and latency, served (FP64 and mixed) vs one at a time")
     ap.add_argument("--config", type=int, default=3, choices=[3, 4], help="serving stream (3, or 4 with large LPs)")
@@ -321,6 +389,11 @@ def main() -> int:
     if args.large:
         r = large_bench(args)
         print(json.dumps({"metric": METRIC, "large": r}), flush=True)
+        return 0
+
+    if args.share:
+        r = share_bench(args)
+        print(json.dumps({"metric": METRIC, "share": r}), flush=True)
         return 0
 
     if args.serving:
