@@ -73,15 +73,15 @@ def cpu_baseline(problem, seconds_budget: float) -> dict:
     cores = min(os.cpu_count() or 1, 16)
     O.set_threads(cores)
     params = O.default_params(tolerance=0.0)
-    # One 64-iteration window first, to size a sample of ~seconds_budget.
+    # One 64-iteration window first, to size five runs totalling ~seconds_budget.
     t64, scale_s = O.time_iterations(problem, 64, params)
     per_iteration = max(t64, 1e-9) / 64
-    k = int(seconds_budget / per_iteration) // 64 * 64
-    k = min(max(k, 64), 64 * 128)
-    if k > 64:
+    k = int(seconds_budget / 5 / per_iteration) // 64 * 64
+    k = min(max(k, 64), 64 * 32)
+    rates = []
+    for _ in range(5):
         t, _ = O.time_iterations(problem, k, params)
-    else:
-        t = t64
+        rates.append(k / t)
     # A per-core figure: the same restatement on one thread, 64 iterations.
     O.set_threads(1)
     t1, _ = O.time_iterations(problem, 64, params)
@@ -92,10 +92,12 @@ def cpu_baseline(problem, seconds_budget: float) -> dict:
             model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
     except OSError:
         pass
-    return {"value": k / t, "unit": "iters/s", "cores": cores, "kind": "port",
-            "sample": f"{k} iterations of the same instance (check work included, power iteration and "
-                      f"{scale_s:.1f}s of host scaling excluded), oracle C restatement with {cores} OpenMP threads",
-            "single_thread_value": 64 / t1, "cpu_model": model, "nproc": os.cpu_count()}
+    return {"value": float(np.median(rates)), "unit": "iters/s", "cores": cores, "kind": "port",
+            "sample": f"median of 5 runs of {k} iterations of the same instance (check work included, power "
+                      f"iteration and {scale_s:.1f}s of host scaling excluded), oracle C restatement with "
+                      f"{cores} OpenMP threads",
+            "runs": [round(r, 2) for r in rates], "single_thread_value": 64 / t1, "cpu_model": model,
+            "nproc": os.cpu_count()}
 
 
 def pct(v, q):
