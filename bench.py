@@ -163,7 +163,12 @@ def serving_bench(args) -> dict:
     from saddlepoint_amd.backend import HipBackend, default_params, solve
 
     t0 = time.time()
-    lps = generate.serving_stream(args.config, args.lps)
+    if args.small:
+        # Small LPs only (rows log-uniform 1k-5k), from their own seeds.
+        lps = [generate.serving_lp(generate.seed_of(args.config, 5000 + i), m_lo=1000, m_hi=5000)
+               for i in range(args.lps)]
+    else:
+        lps = generate.serving_stream(args.config, args.lps)
     tols = [1e-4 if i % 2 == 0 else 1e-6 for i in range(len(lps))]
     nnz = [p.a.nnz for p in lps]
     log(f"generated {len(lps)} LPs of the config-{args.config} stream in {time.time() - t0:.1f}s; "
@@ -198,7 +203,7 @@ def serving_bench(args) -> dict:
     for prec in ("fp64", "mixed"):
         pr = int(prec == "mixed")
         pm = (lambda i: default_params(tolerance=tols[i], precision=pr, time_limit_s=limit, max_iterations=it_cap,
-                                       spmv_autotune=autotune * pr))
+                                       spmv_autotune=autotune * pr, mixed_stall_checks=args.stall_checks))
         mk, lat, res = run_served(b, lps, pm, with_solution=args.check, max_active=args.max_active)
         st = {}
         for r in res:
@@ -370,6 +375,7 @@ def main() -> int:
                     help="serving stream instead: LPs/s and latency, served (FP64 and mixed) vs one at a time")
     ap.add_argument("--config", type=int, default=3, choices=[3, 4], help="serving stream (3, or 4 with large LPs)")
     ap.add_argument("--lps", type=int, default=64, help="serving: LPs in the sample")
+    ap.add_argument("--small", action="store_true", help="serving: small LPs only (rows log-uniform 1k-5k)")
     ap.add_argument("--max-active", type=int, default=64, help="serving: LPs admitted at once")
     ap.add_argument("--poisson", action="store_true", help="serving: also Poisson arrivals (mixed precision)")
     ap.add_argument("--rate-frac", type=float, default=0.8, help="serving: arrival rate / one-at-a-time capacity")
@@ -377,6 +383,8 @@ def main() -> int:
     ap.add_argument("--lp-max-iterations", type=int, default=1_000_000,
                     help="serving: iteration cap per LP, the same in every mode (a multiple of 64)")
     ap.add_argument("--check", action="store_true", help="serving: KKT-check every served solution on the device")
+    ap.add_argument("--stall-checks", type=int, default=0,
+                    help="serving, mixed precision: sp_params.mixed_stall_checks (0: the library's default)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_config1.txt"),
                     help="measured HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE summary)")
     args = ap.parse_args()
@@ -405,8 +413,9 @@ def main() -> int:
                           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
                           "data": "synthetic (seeded serving-stream generator)",
                           "config": {"workload": f"config{args.config} serving stream sample: {args.lps} LPs, rows "
-                                                 f"log-uniform 1k-200k" + (" plus 10% large (200k-1M rows, Zipf(1.5) "
-                                                 "row lengths)" if args.config == 4 else "") +
+                                                 f"log-uniform " + ("1k-5k" if args.small else "1k-200k") +
+                                                 (" plus 10% large (200k-1M rows, Zipf(1.5) "
+                                                  "row lengths)" if args.config == 4 and not args.small else "") +
                                                  ", tolerance 50/50 1e-4/1e-6, all submitted at t=0",
                                      "parallelism": "single GPU"},
                           "serving": r}), flush=True)

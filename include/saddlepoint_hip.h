@@ -163,7 +163,7 @@ typedef struct {
                              row keeps its nonzero order and the solution comes back in the
                              original order. 0: the LP's own column order */
     int64_t mixed_stall_checks; /* precision = 1: checks without halving the relative KKT error after
-                                   which the solve continues in FP64 (0: 64) */
+                                   which the solve continues in FP64 (0, the default: no such switch) */
 } sp_params;
 
 /* Status (solver.rs:95-102). */

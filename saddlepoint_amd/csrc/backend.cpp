@@ -631,18 +631,18 @@ void Hip::delta_to(const sp_vec32* dz, const sp_vec* anchor, sp_vec* z) {
                (long long)z->len);
 }
 
-void Hip::plain_iteration_mixed(const sp_problem* p, sp_vec32* dx, sp_vec32* dy, const sp_vec* x_anchor,
-                                const sp_vec* y_anchor, const sp_vec* atya, const sp_vec* axa, sp_vec32* dx_bar,
-                                double tau, double sigma, double w, double rho) {
+void Hip::plain_iteration_mixed(const sp_problem* p, sp_vec32* dx, sp_vec32* dy, const sp_vec* x_base,
+                                const sp_vec* y_base, const sp_vec* atya, const sp_vec* axa, const sp_vec32* ex,
+                                const sp_vec32* ey, sp_vec32* dx_bar, double tau, double sigma, double w, double rho) {
     if (!p->has_fp32) throw Error(SP_ERR_INVALID, "plain_iteration_mixed: FP32 values not prepared");
     const DevCsr& at = p->at;
     const DevCsr& a = p->a;
     spmv_launch(b, b->k_spmv_t_primal_mixed, b->k_fin_t_primal_mixed, at, at.va32, dy->d, dx->d,
-                (const double*)x_anchor->d, (const double*)p->c, (const double*)p->lv, (const double*)p->uv,
-                (const double*)atya->d, tau, w, rho, dx_bar->d);
+                (const double*)x_base->d, (const double*)p->c, (const double*)p->lv, (const double*)p->uv,
+                (const double*)atya->d, (const float*)ex->d, tau, w, rho, dx_bar->d);
     spmv_launch(b, b->k_spmv_dual_mixed, b->k_fin_dual_mixed, a, a.va32, dx_bar->d, dy->d,
-                (const double*)y_anchor->d, (const double*)p->lc, (const double*)p->uc, (const double*)axa->d, sigma,
-                w, rho);
+                (const double*)y_base->d, (const double*)p->lc, (const double*)p->uc, (const double*)axa->d,
+                (const float*)ey->d, sigma, w, rho);
 }
 
 // ---- bandwidth probe ---------------------------------------------------------------------

@@ -208,16 +208,17 @@ struct Hip {
     void plain_iteration(const sp_problem* p, sp_vec* x, sp_vec* y, const sp_vec* x_anchor, const sp_vec* y_anchor,
                          sp_vec* x_bar, double tau, double sigma, double w, double rho);
 
-    // Mixed precision (sp_params.precision = 1): iterates as FP64 anchor + FP32
-    // displacement between checks.
+    // Mixed precision (sp_params.precision = 1): iterates as an FP64 base (the
+    // iterate at the last check) + an FP32 displacement between checks.
     void ensure_fp32(sp_problem* p);
     sp_vec32* zeros32(int64_t len);
-    void delta_from(const sp_vec* z, const sp_vec* anchor, sp_vec32* dz);  // dz = z - anchor
-    void delta_to(const sp_vec32* dz, const sp_vec* anchor, sp_vec* z);    // z = anchor + dz
-    // plain_iteration in displacement form; atya = A'ᵀy_anchor, axa = A'x_anchor.
-    void plain_iteration_mixed(const sp_problem* p, sp_vec32* dx, sp_vec32* dy, const sp_vec* x_anchor,
-                               const sp_vec* y_anchor, const sp_vec* atya, const sp_vec* axa, sp_vec32* dx_bar,
-                               double tau, double sigma, double w, double rho);
+    void delta_from(const sp_vec* z, const sp_vec* base, sp_vec32* dz);  // dz = z - base
+    void delta_to(const sp_vec32* dz, const sp_vec* base, sp_vec* z);    // z = base + dz (base may be z)
+    // plain_iteration in displacement form: x = x_base + dx, y = y_base + dy;
+    // atya = A'ᵀy_base, axa = A'x_base; ex = x_anchor - x_base, ey = y_anchor - y_base.
+    void plain_iteration_mixed(const sp_problem* p, sp_vec32* dx, sp_vec32* dy, const sp_vec* x_base,
+                               const sp_vec* y_base, const sp_vec* atya, const sp_vec* axa, const sp_vec32* ex,
+                               const sp_vec32* ey, sp_vec32* dx_bar, double tau, double sigma, double w, double rho);
 
     // Average milliseconds of one probe kernel over `reps` launches: kind 0
     // copies n doubles, 1 is a triad over n doubles, 2 gathers n doubles at

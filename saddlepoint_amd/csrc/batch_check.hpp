@@ -246,12 +246,12 @@ inline void enqueue_step_work(Phase& P, const Hip& h, const std::vector<StepWork
     auto len_mn = [](const COp& o) { return (int64_t)(o.m + o.n); };
     auto slot = [](const StepWork& s, int i) { return s.slots + i; };
 
-    // Mixed precision: the FP64 iterates from anchor + displacement.
+    // Mixed precision: the FP64 iterates, base + displacement.
     P.elementwise(b->k_cdelta_to, each([](std::vector<COp>& v, const StepWork& s) {
         if (!s.rebuild) return;
         COp x, y;
-        x.o0 = s.x->d, x.a0 = s.x_anchor->d, x.f0 = s.dx->d, x.n = s.x->len;
-        y.o0 = s.y->d, y.a0 = s.y_anchor->d, y.f0 = s.dy->d, y.n = s.y->len;
+        x.o0 = s.x->d, x.a0 = s.x->d, x.f0 = s.dx->d, x.n = s.x->len;
+        y.o0 = s.y->d, y.a0 = s.y->d, y.f0 = s.dy->d, y.n = s.y->len;
         v.push_back(x);
         v.push_back(y);
     }));
@@ -425,36 +425,38 @@ inline void gather_slots(Phase& P, const Hip& h, const std::vector<StepWork>& ws
     P.elementwise(h.b->k_ccopy, v);
 }
 
-// Solver::enter_plain for every entry: the anchors' FP64 products where the
-// anchors moved, then the FP32 displacements.
+// Solver::enter_plain for every entry: the base's FP64 products, zero
+// displacements, and the anchors' offsets from the base.
 inline void enqueue_plain_entries(Phase& P, const Hip& h, const std::vector<PlainEntry>& es) {
     if (es.empty()) return;
     sp_backend* b = h.b;
     std::vector<COp> t, a;
     std::vector<sp_problem*> pt;
     for (const PlainEntry& e : es) {
-        if (!e.products) continue;
         COp o;
-        o.in = e.y_anchor->d, o.o0 = e.atya->d;
+        o.in = e.y->d, o.o0 = e.atya->d;
         t.push_back(o);
         pt.push_back(e.problem);
     }
-    // spmv_t(y_anchor → atya), then spmv(x_anchor → axa), as enter_plain.
+    // spmv_t(y → atya), then spmv(x → axa), as enter_plain.
     P.spmv(b->k_cspmv_store, b->k_cfin_store, t, pt, false);
     for (const PlainEntry& e : es) {
-        if (!e.products) continue;
         COp o;
-        o.in = e.x_anchor->d, o.o0 = e.axa->d;
+        o.in = e.x->d, o.o0 = e.axa->d;
         a.push_back(o);
     }
     P.spmv(b->k_cspmv_store, b->k_cfin_store, a, pt, true);
     std::vector<COp> d;
     for (const PlainEntry& e : es) {
-        COp x, y;
-        x.a0 = e.x->d, x.a1 = e.x_anchor->d, x.f0 = e.dx->d, x.n = e.x->len;
-        y.a0 = e.y->d, y.a1 = e.y_anchor->d, y.f0 = e.dy->d, y.n = e.y->len;
+        COp x, y, ex, ey;
+        x.a0 = e.x->d, x.a1 = e.x->d, x.f0 = e.dx->d, x.n = e.x->len;
+        y.a0 = e.y->d, y.a1 = e.y->d, y.f0 = e.dy->d, y.n = e.y->len;
+        ex.a0 = e.x_anchor->d, ex.a1 = e.x->d, ex.f0 = e.ex->d, ex.n = e.x->len;
+        ey.a0 = e.y_anchor->d, ey.a1 = e.y->d, ey.f0 = e.ey->d, ey.n = e.y->len;
         d.push_back(x);
         d.push_back(y);
+        d.push_back(ex);
+        d.push_back(ey);
     }
     P.elementwise(b->k_cdelta_from, d);
 }

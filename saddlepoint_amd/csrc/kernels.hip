@@ -441,51 +441,55 @@ extern "C" __global__ __launch_bounds__(WG) void sp_finish_dual_halpern_f64(
 
 // ---- mixed precision: FP32 matrix and displacements, FP64 anchors ------------
 //
-// Between checks the iterates are held as z = z_anchor + dz with the Halpern
-// anchor z_anchor in FP64 and the displacement dz in FP32. With
-// A'ᵀy = A'ᵀy_anchor + A'ᵀdy and A'x_bar = A'x_anchor + A'dx_bar, the FP64
-// products of the anchors are formed once per restart epoch and the per-
-// iteration SpMVs run on FP32 values with FP32 sums. Steps, projections and
-// the Halpern update are evaluated in FP64 on z_anchor + dz, so the rounding
-// is relative to the displacement, not to the iterate. In displacement form
-// the Halpern update z ← w((1+ρ)z_t - ρz) + (1-w)z_anchor reads
-// dz ← w((1+ρ)(z_t - z_anchor) - ρ dz).
+// Between checks the iterates are held as z = z_base + dz: z_base is the FP64
+// iterate at the last check and dz an FP32 displacement from it. With
+// A'ᵀy = A'ᵀy_base + A'ᵀdy and A'x_bar = A'x_base + A'dx_bar, the FP64
+// products of the base are formed when plain iterations resume after a check,
+// and the per-iteration SpMVs run on FP32 values with FP32 sums. Steps,
+// projections and the Halpern update are evaluated in FP64 on z_base + dz, so
+// the rounding is relative to at most a check interval's motion, not to the
+// iterate. In displacement form the Halpern update
+// z ← w((1+ρ)z_t - ρz) + (1-w)z_anchor reads
+// dz ← w((1+ρ)(z_t - z_base) - ρ dz) + (1-w)ez, with ez = z_anchor - z_base
+// held in FP32 (zero when the base is the anchor, right after a restart).
 
 struct EpiPrimalMixed {
     float* dx;
-    const double *xa, *c, *lv, *uv, *atya;
+    const double *xa, *c, *lv, *uv, *atya;  // xa: the base x
+    const float* ex;                         // x_anchor - xa
     double tau, w, rho;
     float* dx_bar;
     struct Pre {
-        float dx;
+        float dx, ex;
         double xa, c, lv, uv, atya;
     };
-    __device__ Pre load(int j) const { return {dx[j], xa[j], c[j], lv[j], uv[j], atya[j]}; }
+    __device__ Pre load(int j) const { return {dx[j], ex[j], xa[j], c[j], lv[j], uv[j], atya[j]}; }
     __device__ void apply(int j, float s, const Pre& q) const {
         const double d = (double)q.dx;
         const double aty = q.atya + (double)s;
         const double t = clampv(q.xa + d - tau * (q.c - aty), q.lv, q.uv) - q.xa;
         st(dx_bar + j, (float)(2.0 * t - d));
-        st(dx + j, (float)(w * ((1.0 + rho) * t - rho * d)));
+        st(dx + j, (float)(w * ((1.0 + rho) * t - rho * d) + (1.0 - w) * (double)q.ex));
     }
 };
 
 struct EpiDualMixed {
     float* dy;
-    const double *ya, *lc, *uc, *axa;
+    const double *ya, *lc, *uc, *axa;  // ya: the base y
+    const float* ey;                    // y_anchor - ya
     double sigma, w, rho;
     struct Pre {
-        float dy;
+        float dy, ey;
         double ya, lc, uc, axa;
     };
-    __device__ Pre load(int i) const { return {dy[i], ya[i], lc[i], uc[i], axa[i]}; }
+    __device__ Pre load(int i) const { return {dy[i], ey[i], ya[i], lc[i], uc[i], axa[i]}; }
     __device__ void apply(int i, float s, const Pre& q) const {
         const double d = (double)q.dy;
         const double y = q.ya + d;
         const double ax = q.axa + (double)s;
         const double v = ax - y / sigma;
         const double t = y - sigma * (ax - clampv(v, q.lc, q.uc)) - q.ya;
-        st(dy + i, (float)(w * ((1.0 + rho) * t - rho * d)));
+        st(dy + i, (float)(w * ((1.0 + rho) * t - rho * d) + (1.0 - w) * (double)q.ey));
     }
 };
 
@@ -499,17 +503,18 @@ extern "C" __global__ __launch_bounds__(WG) void sp_spmv_f32(SPMV32_ARGS, float*
 
 extern "C" __global__ __launch_bounds__(WG) void sp_spmv_t_primal_mixed(
     SPMV32_ARGS, float* __restrict__ dx, const double* __restrict__ xa, const double* __restrict__ c,
-    const double* __restrict__ lv, const double* __restrict__ uv, const double* __restrict__ atya, double tau,
-    double w, double rho, float* __restrict__ dx_bar) {
+    const double* __restrict__ lv, const double* __restrict__ uv, const double* __restrict__ atya,
+    const float* __restrict__ ex, double tau, double w, double rho, float* __restrict__ dx_bar) {
     spmv_body<SP_NNZ_F64, float, float, float>(desc, rp, ci, va, xin, partials,
-                                               EpiPrimalMixed{dx, xa, c, lv, uv, atya, tau, w, rho, dx_bar});
+                                               EpiPrimalMixed{dx, xa, c, lv, uv, atya, ex, tau, w, rho, dx_bar});
 }
 
 extern "C" __global__ __launch_bounds__(WG) void sp_spmv_dual_mixed(
     SPMV32_ARGS, float* __restrict__ dy, const double* __restrict__ ya, const double* __restrict__ lc,
-    const double* __restrict__ uc, const double* __restrict__ axa, double sigma, double w, double rho) {
+    const double* __restrict__ uc, const double* __restrict__ axa, const float* __restrict__ ey, double sigma,
+    double w, double rho) {
     spmv_body<SP_NNZ_F64, float, float, float>(desc, rp, ci, va, xin, partials,
-                                               EpiDualMixed{dy, ya, lc, uc, axa, sigma, w, rho});
+                                               EpiDualMixed{dy, ya, lc, uc, axa, ey, sigma, w, rho});
 }
 
 
@@ -519,15 +524,16 @@ extern "C" __global__ __launch_bounds__(WG) void sp_finish_f32(FIN_ARGS(float), 
 
 extern "C" __global__ __launch_bounds__(WG) void sp_finish_t_primal_mixed(
     FIN_ARGS(float), float* __restrict__ dx, const double* __restrict__ xa, const double* __restrict__ c,
-    const double* __restrict__ lv, const double* __restrict__ uv, const double* __restrict__ atya, double tau,
-    double w, double rho, float* __restrict__ dx_bar) {
-    finish_body<float>(longs, nlong, partials, EpiPrimalMixed{dx, xa, c, lv, uv, atya, tau, w, rho, dx_bar});
+    const double* __restrict__ lv, const double* __restrict__ uv, const double* __restrict__ atya,
+    const float* __restrict__ ex, double tau, double w, double rho, float* __restrict__ dx_bar) {
+    finish_body<float>(longs, nlong, partials, EpiPrimalMixed{dx, xa, c, lv, uv, atya, ex, tau, w, rho, dx_bar});
 }
 
 extern "C" __global__ __launch_bounds__(WG) void sp_finish_dual_mixed(
     FIN_ARGS(float), float* __restrict__ dy, const double* __restrict__ ya, const double* __restrict__ lc,
-    const double* __restrict__ uc, const double* __restrict__ axa, double sigma, double w, double rho) {
-    finish_body<float>(longs, nlong, partials, EpiDualMixed{dy, ya, lc, uc, axa, sigma, w, rho});
+    const double* __restrict__ uc, const double* __restrict__ axa, const float* __restrict__ ey, double sigma,
+    double w, double rho) {
+    finish_body<float>(longs, nlong, partials, EpiDualMixed{dy, ya, lc, uc, axa, ey, sigma, w, rho});
 }
 
 // ---- element-wise (cpu/src/lib.rs:147-209) ------------------------------------
@@ -603,9 +609,9 @@ extern "C" __global__ __launch_bounds__(WG) void sp_delta_from(const double* __r
     GRID_LOOP(i, n) { dz[i] = (float)(z[i] - anchor[i]); }
 }
 
-extern "C" __global__ __launch_bounds__(WG) void sp_delta_to(const float* __restrict__ dz,
-                                                             const double* __restrict__ anchor,
-                                                             double* __restrict__ z, i64 n) {
+// z = base + dz; base may be z itself.
+extern "C" __global__ __launch_bounds__(WG) void sp_delta_to(const float* __restrict__ dz, const double* anchor,
+                                                             double* z, i64 n) {
     GRID_LOOP(i, n) { z[i] = anchor[i] + (double)dz[i]; }
 }
 
@@ -1176,11 +1182,14 @@ struct BInst {
     const double *c, *lc, *uc, *lv, *uv;
     double *x, *y, *x_bar;
     const double *x_anchor, *y_anchor;
-    float *dx, *dy, *dx_bar;       // mixed precision displacements
-    const double *atya, *axa;      // mixed precision anchor products
+    float *dx, *dy, *dx_bar;       // mixed precision: displacements from the base (x, y),
+    const double *atya, *axa;      //   the base's FP64 products,
+    const float *ex, *ey;          //   anchor - base
     double tau, sigma, rho;
     unsigned long long k0;         // Halpern counter k at step 0 of the chunk
 };
+
+static_assert(sizeof(BInst) == 248, "BInst layout is shared with the host");
 
 struct BDesc {
     Desc d;
@@ -1246,7 +1255,7 @@ extern "C" __global__ __launch_bounds__(WG) void sp_bspmv_t_primal_mixed(const B
     const BInst& I = insts[bd.inst];
     spmv_block<SP_NNZ_F64, float, float, float>(
         bd.d, I.t_rp, I.t_ci, I.t_va32, (const float*)I.dy, (float*)I.t_part,
-        EpiPrimalMixed{I.dx, I.x_anchor, I.c, I.lv, I.uv, I.atya, I.tau, halpern_w(I.k0, it), I.rho, I.dx_bar});
+        EpiPrimalMixed{I.dx, I.x, I.c, I.lv, I.uv, I.atya, I.ex, I.tau, halpern_w(I.k0, it), I.rho, I.dx_bar});
 }
 
 extern "C" __global__ __launch_bounds__(WG) void sp_bspmv_dual_mixed(const BDesc* __restrict__ desc,
@@ -1255,7 +1264,7 @@ extern "C" __global__ __launch_bounds__(WG) void sp_bspmv_dual_mixed(const BDesc
     const BInst& I = insts[bd.inst];
     spmv_block<SP_NNZ_F64, float, float, float>(
         bd.d, I.a_rp, I.a_ci, I.a_va32, (const float*)I.dx_bar, (float*)I.a_part,
-        EpiDualMixed{I.dy, I.y_anchor, I.lc, I.uc, I.axa, I.sigma, halpern_w(I.k0, it), I.rho});
+        EpiDualMixed{I.dy, I.y, I.lc, I.uc, I.axa, I.ey, I.sigma, halpern_w(I.k0, it), I.rho});
 }
 
 extern "C" __global__ __launch_bounds__(WG) void sp_bfinish_t_primal_mixed(const BLong* __restrict__ longs, int nlong,
@@ -1266,7 +1275,7 @@ extern "C" __global__ __launch_bounds__(WG) void sp_bfinish_t_primal_mixed(const
     const BInst& I = insts[L.inst];
     const LongRow lr{L.row, L.pbase, L.nch, 0};
     finish_body<float>(&lr, 1, (const float*)I.t_part,
-                       EpiPrimalMixed{I.dx, I.x_anchor, I.c, I.lv, I.uv, I.atya, I.tau, halpern_w(I.k0, it), I.rho,
+                       EpiPrimalMixed{I.dx, I.x, I.c, I.lv, I.uv, I.atya, I.ex, I.tau, halpern_w(I.k0, it), I.rho,
                                       I.dx_bar},
                        0);
 }
@@ -1279,7 +1288,7 @@ extern "C" __global__ __launch_bounds__(WG) void sp_bfinish_dual_mixed(const BLo
     const BInst& I = insts[L.inst];
     const LongRow lr{L.row, L.pbase, L.nch, 0};
     finish_body<float>(&lr, 1, (const float*)I.a_part,
-                       EpiDualMixed{I.dy, I.y_anchor, I.lc, I.uc, I.axa, I.sigma, halpern_w(I.k0, it), I.rho}, 0);
+                       EpiDualMixed{I.dy, I.y, I.lc, I.uc, I.axa, I.ey, I.sigma, halpern_w(I.k0, it), I.rho}, 0);
 }
 
 // ---- batched check iterations for the serving loop --------------------------------
@@ -1392,14 +1401,14 @@ extern "C" __global__ __launch_bounds__(WG) void sp_chalpern_f64(const CBlk* __r
         o.o0[i] = w * ((1.0 + rho) * o.a0[i] - rho * o.o0[i]) + (1.0 - w) * o.a1[i];
 }
 
-// o0 = a0 + f0 (anchor + FP32 displacement)
+// o0 = a0 + f0 (base + FP32 displacement; a0 may be o0)
 extern "C" __global__ __launch_bounds__(WG) void sp_cdelta_to(const CBlk* __restrict__ blk,
                                                               const COp* __restrict__ ops) {
     CELEM_HEAD
     for (i64 i = i0 + threadIdx.x; i < i1; i += WG) o.o0[i] = o.a0[i] + (double)o.f0[i];
 }
 
-// f0 = (float)(a0 - a1) (iterate - anchor)
+// f0 = (float)(a0 - a1)
 extern "C" __global__ __launch_bounds__(WG) void sp_cdelta_from(const CBlk* __restrict__ blk,
                                                                 const COp* __restrict__ ops) {
     CELEM_HEAD

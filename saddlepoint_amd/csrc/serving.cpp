@@ -55,9 +55,11 @@ struct BInst {
     const double *x_anchor, *y_anchor;
     float *dx, *dy, *dx_bar;
     const double *atya, *axa;
+    const float *ex, *ey;
     double tau, sigma, rho;
     unsigned long long k0;
 };
+static_assert(sizeof(BInst) == 248, "BInst layout is shared with kernels.hip");
 using Clock = std::chrono::steady_clock;
 
 constexpr uint64_t kChunk = 64;
@@ -133,11 +135,11 @@ struct Job {
 // Device bytes one admitted LP holds (the admission test): both matrices with
 // int32 indices and FP64 values (+ FP32 values in mixed precision), the
 // solver's 13 vectors, the problem's 7 vectors and the mixed-precision
-// displacements and anchor products.
+// displacements, anchor offsets and base products.
 static int64_t footprint(int64_t m, int64_t n, int64_t nnz, bool mixed) {
     int64_t bytes = 2 * (12 * nnz + 4 * (m + 1)) + 8 * (7 * n + 6 * m) + 8 * (4 * n + 3 * m);
     bytes += 2 * 32 * (nnz / 1024 + 64);  // plans
-    if (mixed) bytes += 2 * 4 * nnz + 4 * (2 * n + m) + 8 * (n + m);
+    if (mixed) bytes += 2 * 4 * nnz + 4 * (3 * n + 2 * m) + 8 * (n + m);
     return bytes;
 }
 
@@ -627,6 +629,8 @@ class Server {
                 I.dx_bar = s.dx_bar->d;
                 I.atya = s.atya->d;
                 I.axa = s.axa->d;
+                I.ex = s.ex->d;
+                I.ey = s.ey->d;
             }
             I.tau = s.tau;
             I.sigma = s.sigma;

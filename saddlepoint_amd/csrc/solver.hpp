@@ -122,6 +122,7 @@ class Solver {
             if (!params.fused) throw Error(SP_ERR_INVALID, "mixed precision needs the fused path");
             b_.ensure_fp32(dev_.get());
             dx32_ = b_.zeros32(n), dy32_ = b_.zeros32(m), dx_bar32_ = b_.zeros32(n);
+            ex32_ = b_.zeros32(n), ey32_ = b_.zeros32(m);
             atya_ = b_.zeros(n), axa_ = b_.zeros(m);
         }
     }
@@ -233,8 +234,9 @@ class Solver {
                 const double w = (double)(k_ + 1) / (double)(k_ + 2);
                 if (mixed_now()) {
                     enter_plain();
-                    b_.plain_iteration_mixed(dev_.get(), dx32_.get(), dy32_.get(), x_anchor_.get(), y_anchor_.get(),
-                                             atya_.get(), axa_.get(), dx_bar32_.get(), tau, sigma, w, p.reflection);
+                    b_.plain_iteration_mixed(dev_.get(), dx32_.get(), dy32_.get(), x_.get(), y_.get(), atya_.get(),
+                                             axa_.get(), ex32_.get(), ey32_.get(), dx_bar32_.get(), tau, sigma, w,
+                                             p.reflection);
                 } else {
                     b_.plain_iteration(dev_.get(), x_.get(), y_.get(), x_anchor_.get(), y_anchor_.get(),
                                        x_bar_.get(), tau, sigma, w, p.reflection);
@@ -246,8 +248,8 @@ class Solver {
             }
             if (shadow_live_) {
                 // Checks and restarts work on the FP64 iterates.
-                b_.delta_to(dx32_.get(), x_anchor_.get(), x_.get());
-                b_.delta_to(dy32_.get(), y_anchor_.get(), y_.get());
+                b_.delta_to(dx32_.get(), x_.get(), x_.get());
+                b_.delta_to(dy32_.get(), y_.get(), y_.get());
                 shadow_live_ = false;
             }
             if (p.fused) {
@@ -339,8 +341,8 @@ class Solver {
     void check_begin() {
         if (!check_due()) throw Error(SP_ERR_INVALID, "check_begin: no check is due");
         if (shadow_live_) {
-            b_.delta_to(dx32_.get(), x_anchor_.get(), x_.get());
-            b_.delta_to(dy32_.get(), y_anchor_.get(), y_.get());
+            b_.delta_to(dx32_.get(), x_.get(), x_.get());
+            b_.delta_to(dy32_.get(), y_.get(), y_.get());
             shadow_live_ = false;
         }
         enqueue_step(true, eta_ / omega_, eta_ * omega_, total_ + 1);
@@ -363,7 +365,7 @@ class Solver {
     // appended to `copies` (source, destination) for the caller to run.
     struct StepWork {
         bool check = false, k_was_zero = false, step_implied = false;
-        bool rebuild = false;  // mixed precision: x, y = anchor + displacement first
+        bool rebuild = false;  // mixed precision: x, y += displacement first
         int fix_base = 0;      // slot of the fixed-point sums
         sp_problem* problem = nullptr;
         sp_vec *x, *y, *x_t, *y_t, *x_bar, *x_anchor, *y_anchor, *ax, *aty, *tmp_n, *tmp_m, *tmp_m2, *c;
@@ -434,22 +436,20 @@ class Solver {
         }
     }
 
-    // What enter_plain() would enqueue, described for a batching caller:
-    // the anchors' FP64 products (when the anchors moved) and the FP32
-    // displacements. Returns false when there is nothing to do.
+    // What enter_plain() would enqueue, described for a batching caller: the
+    // base's FP64 products, zero displacements and the anchors' offsets from
+    // the base. Returns false when there is nothing to do.
     struct PlainEntry {
-        bool products = false;
         sp_problem* problem = nullptr;
         sp_vec *x, *y, *x_anchor, *y_anchor, *atya, *axa;
-        sp_vec32 *dx, *dy;
+        sp_vec32 *dx, *dy, *ex, *ey;
     };
     bool begin_plain(PlainEntry& e) {
         if (!mixed_now() || shadow_live_) return false;
-        e.products = anchor_epoch_ != restarts_;
-        anchor_epoch_ = restarts_;
         e.problem = dev_.get();
         e.x = x_.get(), e.y = y_.get(), e.x_anchor = x_anchor_.get(), e.y_anchor = y_anchor_.get();
         e.atya = atya_.get(), e.axa = axa_.get(), e.dx = dx32_.get(), e.dy = dy32_.get();
+        e.ex = ex32_.get(), e.ey = ey32_.get();
         shadow_live_ = true;
         return true;
     }
@@ -458,18 +458,17 @@ class Solver {
     bool mixed_now() const { return p_.precision == 1 && !fp64_fallback_; }
 
     // Brings the iterates into the form plain iterations use: in mixed
-    // precision, the FP32 displacements from the anchors (and the anchors'
-    // FP64 products after a restart).
+    // precision, the current FP64 iterates become the base, with their FP64
+    // products, zero FP32 displacements and the anchors' FP32 offsets from the
+    // base.
     void enter_plain() {
         if (!mixed_now() || shadow_live_) return;
-        if (anchor_epoch_ != restarts_) {
-            // The anchors moved: refresh their FP64 products.
-            b_.spmv_t(dev_.get(), y_anchor_.get(), atya_.get());
-            b_.spmv(dev_.get(), x_anchor_.get(), axa_.get());
-            anchor_epoch_ = restarts_;
-        }
-        b_.delta_from(x_.get(), x_anchor_.get(), dx32_.get());
-        b_.delta_from(y_.get(), y_anchor_.get(), dy32_.get());
+        b_.spmv_t(dev_.get(), y_.get(), atya_.get());
+        b_.spmv(dev_.get(), x_.get(), axa_.get());
+        b_.delta_from(x_.get(), x_.get(), dx32_.get());
+        b_.delta_from(y_.get(), y_.get(), dy32_.get());
+        b_.delta_from(x_anchor_.get(), x_.get(), ex32_.get());
+        b_.delta_from(y_anchor_.get(), y_.get(), ey32_.get());
         shadow_live_ = true;
     }
 
@@ -477,15 +476,28 @@ class Solver {
     struct PlainState {
         sp_problem* problem;
         sp_vec *x, *y, *x_bar, *x_anchor, *y_anchor, *atya, *axa;
-        sp_vec32 *dx, *dy, *dx_bar;
+        sp_vec32 *dx, *dy, *dx_bar, *ex, *ey;
         double tau, sigma, rho;
         uint64_t k;
     };
     PlainState plain_state() {
-        return PlainState{dev_.get(), x_.get(), y_.get(), x_bar_.get(), x_anchor_.get(), y_anchor_.get(),
-                          atya_ ? atya_.get() : nullptr, axa_ ? axa_.get() : nullptr,
-                          dx32_ ? dx32_.get() : nullptr, dy32_ ? dy32_.get() : nullptr,
-                          dx_bar32_ ? dx_bar32_.get() : nullptr, eta_ / omega_, eta_ * omega_, p_.reflection, k_};
+        return PlainState{dev_.get(),
+                          x_.get(),
+                          y_.get(),
+                          x_bar_.get(),
+                          x_anchor_.get(),
+                          y_anchor_.get(),
+                          atya_ ? atya_.get() : nullptr,
+                          axa_ ? axa_.get() : nullptr,
+                          dx32_ ? dx32_.get() : nullptr,
+                          dy32_ ? dy32_.get() : nullptr,
+                          dx_bar32_ ? dx_bar32_.get() : nullptr,
+                          ex32_ ? ex32_.get() : nullptr,
+                          ey32_ ? ey32_.get() : nullptr,
+                          eta_ / omega_,
+                          eta_ * omega_,
+                          p_.reflection,
+                          k_};
     }
 
     // Accounts for `steps` plain iterations that the caller ran.
@@ -834,17 +846,19 @@ class Solver {
             fp64_fallback_ = true;
     }
 
-    // Mixed precision only: FP64 takes over when the relative KKT error has not
-    // improved on its best for kStallChecks checks in a row.
-    static constexpr uint64_t kStallChecks = 64;  // when params.mixed_stall_checks is 0
+    // Mixed precision only, with params.mixed_stall_checks = N > 0: FP64 takes
+    // over when the relative KKT error has not halved its best for N checks in
+    // a row. Off by default: with the displacement re-based at every check the
+    // FP32 rounding stays far below the progress of a check interval, and a
+    // slow LP stalls by this measure in FP64 just as often.
     void watch_progress(const Kkt& k) {
-        if (p_.precision != 1 || fp64_fallback_) return;
+        if (p_.precision != 1 || fp64_fallback_ || p_.mixed_stall_checks <= 0) return;
         const double err = std::max({k.primal_res / (1.0 + b_norm_), k.dual_res / (1.0 + c_norm_),
                                      k.gap() / (1.0 + std::fabs(k.primal_obj) + std::fabs(k.dual_obj))});
         if (err < 0.5 * best_err_) {
             best_err_ = err;
             stalled_checks_ = 0;
-        } else if (++stalled_checks_ >= (p_.mixed_stall_checks > 0 ? (uint64_t)p_.mixed_stall_checks : kStallChecks)) {
+        } else if (++stalled_checks_ >= (uint64_t)p_.mixed_stall_checks) {
             fp64_fallback_ = true;
         }
     }
@@ -869,12 +883,12 @@ class Solver {
     Kkt kkt_;
     Prob dev_;
     Vec x_, y_, x_t_, y_t_, x_bar_, x_anchor_, y_anchor_, ax_, aty_, tmp_n_, tmp_m_, tmp_m2_, c_;
-    // Mixed precision: FP32 displacements from the anchors, live between
-    // checks, and the anchors' FP64 products A'ᵀy_anchor, A'x_anchor.
-    typename B::Vector32 dx32_, dy32_, dx_bar32_;
+    // Mixed precision: between checks x = x_ + dx32_, y = y_ + dy32_ (x_, y_
+    // are the iterates at the last check, the base), with the base's FP64
+    // products A'ᵀy_, A'x_ and the anchors' offsets from the base.
+    typename B::Vector32 dx32_, dy32_, dx_bar32_, ex32_, ey32_;
     Vec atya_, axa_;
     bool shadow_live_ = false;
-    uint64_t anchor_epoch_ = UINT64_MAX;
     bool fp64_fallback_ = false;
     double best_err_ = INFINITY;
     uint64_t stalled_checks_ = 0;

@@ -126,6 +126,33 @@ def test_mixed_precision_config0(gpu):
     assert abs(c.objective - g.objective) / (1 + abs(c.objective) + abs(g.objective)) <= 2 * TOL
 
 
+def test_mixed_precision_follows_fp64(gpu):
+    """With the displacement re-based on the FP64 iterate at every check, mixed
+    precision takes the same number of iterations as FP64 on these LPs and ends
+    within 1e-8 of its objective (the FP32 rounding stays far below the
+    progress of a check interval)."""
+    lps = [("config0", generate.config0())] + [(f"c3-{i}", p) for i, p in
+                                               enumerate(generate.serving_stream(3, 12)) if i in (0, 2, 9, 11)]
+    for name, p in lps:
+        f = solve(gpu, p, default_params(tolerance=TOL))
+        g = solve(gpu, p, default_params(tolerance=TOL, precision=1))
+        gap = abs(f.objective - g.objective) / (1 + abs(f.objective))
+        print(f"{name}: fp64 {f.iterations} it, mixed {g.iterations} it "
+              f"({g.mixed_fp64_iterations} in fp64), objective gap {gap:.1e}", flush=True)
+        assert g.status == f.status == "Optimal", name
+        assert g.iterations == f.iterations and g.mixed_fp64_iterations == 0, name
+        assert gap <= 1e-8, name
+
+
+def test_mixed_precision_stall_window_switches_to_fp64(gpu):
+    # sp_params.mixed_stall_checks = N: FP64 takes over after N checks without
+    # halving the relative KKT error; the answer is still checked.
+    p = generate.config0()
+    g = solve(gpu, p, default_params(tolerance=TOL, precision=1, mixed_stall_checks=2))
+    assert g.status == "Optimal" and 0 < g.mixed_fp64_iterations < g.iterations
+    O.check(p, g.x, g.y).verify(2 * TOL)
+
+
 def test_mixed_precision_needs_fused(gpu):
     from saddlepoint_amd._lib import BackendError
 
