@@ -115,12 +115,13 @@ def fifo_latencies(arrivals, service):
     return out
 
 
-def run_served(b, lps, params_of, arrivals=None, max_active=64, with_solution=False, use_graphs=None):
+def run_served(b, lps, params_of, arrivals=None, max_active=64, with_solution=False, use_graphs=None,
+               chunk_work=None):
     """Serves `lps` (all at t = 0, or at the given arrival times on the
     server's clock); returns (makespan, latencies, results)."""
     from saddlepoint_amd.serving import Server
 
-    srv = Server(b, max_active=max_active, use_graphs=use_graphs)
+    srv = Server(b, max_active=max_active, use_graphs=use_graphs, chunk_work=chunk_work)
     n = len(lps)
     ids = [None] * n
     t_arr = [0.0] * n
@@ -204,7 +205,8 @@ def serving_bench(args) -> dict:
         pr = int(prec == "mixed")
         pm = (lambda i: default_params(tolerance=tols[i], precision=pr, time_limit_s=limit, max_iterations=it_cap,
                                        spmv_autotune=autotune * pr, mixed_stall_checks=args.stall_checks))
-        mk, lat, res = run_served(b, lps, pm, with_solution=args.check, max_active=args.max_active)
+        mk, lat, res = run_served(b, lps, pm, with_solution=args.check, max_active=args.max_active,
+                                  chunk_work=args.chunk_work)
         st = {}
         for r in res:
             st[r.status] = st.get(r.status, 0) + 1
@@ -243,7 +245,7 @@ def serving_bench(args) -> dict:
         log(f"served {prec}: {entry['lps_per_s']:.3f} LPs/s (x{entry['speedup_vs_one_at_a_time']:.2f}), "
             f"p50 {entry['p50_s']:.3f}s p99 {entry['p99_s']:.3f}s")
         if args.poisson and pr:
-            _, plat, _ = run_served(b, lps, pm, arrivals, max_active=args.max_active)
+            _, plat, _ = run_served(b, lps, pm, arrivals, max_active=args.max_active, chunk_work=args.chunk_work)
             entry["poisson"] = {"rate_lps_per_s": args.rate_frac * cap, "p50_s": pct(plat, 50),
                                 "p99_s": pct(plat, 99)}
             log(f"  poisson at {args.rate_frac} x capacity: p50 {pct(plat, 50):.3f}s p99 {pct(plat, 99):.3f}s "
@@ -376,6 +378,8 @@ def main() -> int:
     ap.add_argument("--config", type=int, default=3, choices=[3, 4], help="serving stream (3, or 4 with large LPs)")
     ap.add_argument("--lps", type=int, default=64, help="serving: LPs in the sample")
     ap.add_argument("--small", action="store_true", help="serving: small LPs only (rows log-uniform 1k-5k)")
+    ap.add_argument("--chunk-work", type=int, default=0,
+                    help="serving: sp_server_params.chunk_work (0: the default; negative: every LP every chunk)")
     ap.add_argument("--max-active", type=int, default=64, help="serving: LPs admitted at once")
     ap.add_argument("--poisson", action="store_true", help="serving: also Poisson arrivals (mixed precision)")
     ap.add_argument("--rate-frac", type=float, default=0.8, help="serving: arrival rate / one-at-a-time capacity")

@@ -112,6 +112,8 @@ struct Job {
     int err = SP_OK;
     std::string msg;
     bool failed_now = false;  // an error in this chunk; retired at its end
+    int64_t cost = 0;         // per-iteration work, for the chunk schedule: nnz + m + n
+    int skipped = 0;          // chunks in a row this LP was left out of
 
     LpView view() const {
         LpView v;
@@ -190,6 +192,7 @@ class Server {
         job->params.fused = 1;
         job->params.max_iterations = (params.max_iterations + kChunk - 1) / kChunk * kChunk;
         job->footprint = footprint(job->m, job->n, nnz, params.precision == 1);
+        job->cost = nnz + job->m + job->n;
         job->t_submit = clock();
         std::lock_guard<std::mutex> g(mu_);
         job->id = ++next_id_;
@@ -393,6 +396,36 @@ class Server {
         }
     }
 
+    // The LPs this chunk runs: the cheapest first (cost, then id) until their
+    // costs reach the chunk's work target, and every LP that was left out of
+    // the last kMaxSkipped chunks; in id order. The rest keep their state.
+    static constexpr int64_t kChunkWork = 32'000'000;
+    static constexpr int kMaxSkipped = 8;
+    std::vector<Job*> chunk_members() {
+        std::vector<Job*> live;
+        for (Job* j : active_)
+            if (!j->failed_now && !j->solver->terminated()) live.push_back(j);
+        if (sp_.chunk_work < 0) return live;
+        const int64_t target = sp_.chunk_work > 0 ? sp_.chunk_work : kChunkWork;
+        std::vector<Job*> order = live;
+        std::stable_sort(order.begin(), order.end(), [](const Job* a, const Job* b) {
+            return a->cost != b->cost ? a->cost < b->cost : a->id < b->id;
+        });
+        int64_t work = 0;
+        std::vector<Job*> run;
+        for (Job* j : order) {
+            if (work < target || j->skipped >= kMaxSkipped) {
+                run.push_back(j);
+                work += j->cost;
+                j->skipped = 0;
+            } else {
+                ++j->skipped;
+            }
+        }
+        std::sort(run.begin(), run.end(), [](const Job* a, const Job* b) { return a->id < b->id; });
+        return run;
+    }
+
     void chunk() {
         // Everything of the previous chunk has left the stream (its last
         // work, the restart copies, is waited for here), so the pinned
@@ -401,9 +434,10 @@ class Server {
         arena_.reset();
         staged_ = 0;
         ensure_staging(2 * active_.size());
+        const std::vector<Job*> members = chunk_members();
         // Step 1.
         std::vector<Job*> first, batch1;
-        for (Job* j : active_) {
+        for (Job* j : members) {
             if (j->failed_now || j->solver->terminated()) continue;
             if (j->solver->plain_run() == 0) {
                 if (!j->solver->check_due()) first.push_back(j);
@@ -416,7 +450,7 @@ class Server {
         // Steps 2-63: everyone that is still running and in step.
         std::vector<Job*> all;
         uint64_t steps = UINT64_MAX;
-        for (Job* j : active_) {
+        for (Job* j : members) {
             if (j->failed_now || j->solver->terminated()) continue;
             all.push_back(j);
             steps = std::min(steps, j->solver->plain_run());
@@ -425,7 +459,7 @@ class Server {
             run_batched(all, (int)steps, sp_.use_graphs != 0, batch64_, batch32_);
         // Step 64: every LP's check.
         std::vector<Job*> due;
-        for (Job* j : active_) {
+        for (Job* j : members) {
             while (!j->failed_now && !j->solver->terminated() && j->solver->plain_run() == 0 &&
                    !j->solver->check_due())
                 own_step(j);  // a restart epoch's first iteration (not reached with aligned chunks)
@@ -521,8 +555,10 @@ class Server {
             const std::vector<Job*>& g = mixed ? g32 : g64;
             if (g.empty()) continue;
             Batch& bt = mixed ? b32 : b64;
-            build(bt, g, mixed != 0);
-            const bool use_graph = graph && steps > 1;
+            const bool same = build(bt, g, mixed != 0);
+            // A graph pays off only when the membership carries over: a new
+            // one is captured on the second chunk of a membership.
+            const bool use_graph = graph && steps > 1 && same;
             launch_steps(bt, steps, mixed != 0, use_graph);
             for (Job* j : g) j->solver->plain_done((uint64_t)steps);
         }
@@ -558,10 +594,12 @@ class Server {
             "hipHostMalloc");
     }
 
-    void build(Batch& bt, const std::vector<Job*>& g, bool mixed) {
+    // Returns whether the membership is that of the previous build.
+    bool build(Batch& bt, const std::vector<Job*>& g, bool mixed) {
         std::vector<uint64_t> ids;
         for (Job* j : g) ids.push_back(j->id);
-        if (bt.members != ids) {
+        const bool same = bt.members == ids;
+        if (!same) {
             // The previous uploads from these host vectors were ordered on the
             // stream before any work of this chunk; a check synchronized since.
             bt.h_dt.clear();
@@ -639,6 +677,7 @@ class Server {
         }
         hip_check(hipMemcpyAsync(bt.inst.d, stage, g.size() * sizeof(BInst), hipMemcpyHostToDevice, b_->stream),
                   "hipMemcpyAsync H2D");
+        return same;
     }
 
     template <class... Args>
@@ -785,6 +824,7 @@ void sp_default_server_params(sp_server_params* p) {
     p->max_active = 64;
     p->prep_threads = 8;
     p->use_graphs = 1;
+    p->chunk_work = 0;
 }
 
 int sp_server_create(sp_backend* b, const sp_server_params* params, sp_server** out) {

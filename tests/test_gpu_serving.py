@@ -66,6 +66,26 @@ def test_served_results_are_bitwise_single_solves(gpu, precision):
     s.close()
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_chunk_schedule_changes_timing_not_results(gpu, precision):
+    """With chunk_work = 1 a chunk runs the cheapest LP and those left out of
+    the last 8 chunks, so the batch membership changes from chunk to chunk
+    (tables rebuilt, graphs captured and dropped); every LP still ends with the
+    bits it gets when every LP runs in every chunk."""
+    items = problems()
+    res = {}
+    for work in (1, -1):
+        s = Server(gpu, chunk_work=work)
+        ids = [s.submit(p, default_params(precision=precision)) for _, p in items]
+        s.drain(ids)
+        res[work] = [s.result(i).result for i in ids]
+        s.close()
+    for (name, _), a, b in zip(items, res[1], res[-1]):
+        print(f"{name:>10}: {a.status} {a.iterations} it (chunk_work 1) / {b.iterations} it (every chunk)", flush=True)
+        assert a.status == b.status and a.iterations == b.iterations and a.restarts == b.restarts, name
+        assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y), name
+
+
 def test_budget_limits_admission(gpu):
     items = problems()[:4]
     s = Server(gpu, budget_bytes=1, max_active=8)  # one LP at a time (an empty server always admits one)
