@@ -355,6 +355,51 @@ def share_bench(args) -> dict:
     return out
 
 
+# The reference's published GPU rows (bench/results/2026-11-03-rtx-a1000-pid.csv; RTX A1000 Laptop GPU,
+# MIPLIB 2017 LP relaxations at 1e-4; the two rows that hit the 300 s limit are left out): instance, rows,
+# cols, nnz, iterations, solve seconds.
+REFERENCE_GPU_ROWS = [
+    ("rail507", 509, 63_019, 468_878, 3_456, 0.758),
+    ("irish-electricity", 104_259, 61_728, 523_257, 86_592, 26.416),
+    ("supportcase10", 165_684, 14_770, 555_082, 19_264, 6.681),
+    ("ex10", 69_608, 17_680, 1_162_000, 448, 0.272),
+    ("s100", 14_733, 364_417, 1_777_917, 174_848, 176.365),
+    ("savsched1", 295_989, 328_575, 1_770_507, 704, 0.914),
+    ("neos-5052403-cygnet", 38_268, 32_868, 4_898_304, 22_720, 28.627),
+]
+
+
+def reference_shapes_bench(args) -> dict:
+    """Per-iteration time on random LPs of the published instances' sizes
+    (the instance files are not available here), beside the reference's
+    published GPU time per iteration (solve seconds / iterations, power
+    iteration and checks included). Ours: `steps` iterations after `warmup`,
+    checks every 64 included, power iteration excluded."""
+    from saddlepoint_amd import generate
+    from saddlepoint_amd.backend import HipBackend, Solver, default_params
+
+    b = HipBackend(0)
+    rows = []
+    for i, (name, m, n, nnz, its, secs) in enumerate(REFERENCE_GPU_ROWS):
+        p = generate.shaped_lp(generate.seed_of(9, i), m, n, nnz)
+        s = Solver(b, p, default_params(tolerance=0.0, fused=1))
+        s.advance(64)
+        b.sync()
+        t = time.perf_counter()
+        s.advance(args.steps)
+        b.sync()
+        el = time.perf_counter() - t
+        s.close()
+        ours = 1e6 * el / args.steps
+        ref = 1e6 * secs / its
+        rows.append({"instance": name, "m": m, "n": n, "nnz_published": nnz, "nnz": int(p.a.nnz),
+                     "reference_us_per_iteration": round(ref, 1), "us_per_iteration": round(ours, 2),
+                     "ratio": round(ref / ours, 1)})
+        log(f"  {name:>20}: {m}x{n}, {p.a.nnz} nnz: {ours:.1f} us/iteration (reference GPU {ref:.1f}, x{ref / ours:.1f})")
+    b.close()
+    return {"rows": rows, "steps": args.steps}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -371,6 +416,8 @@ def main() -> int:
                     help="config 2 instead: FP64 vs mixed precision at 1e-4 and 1e-6 (use --scale to shrink)")
     ap.add_argument("--max-iterations", type=int, default=200_000)
     ap.add_argument("--time-limit", type=float, default=150.0, help="--large: seconds per solve")
+    ap.add_argument("--reference-shapes", action="store_true",
+                    help="us/iteration on random LPs of the reference's published instance sizes, beside its figures")
     ap.add_argument("--share", type=int, default=0,
                     help="config 2's SpMVs on one GPU's share of a P-way row-block split (P = this value)")
     ap.add_argument("--serving", action="store_true",
@@ -403,6 +450,11 @@ def main() -> int:
     if args.large:
         r = large_bench(args)
         print(json.dumps({"metric": METRIC, "large": r}), flush=True)
+        return 0
+
+    if args.reference_shapes:
+        r = reference_shapes_bench(args)
+        print(json.dumps({"metric": METRIC, "reference_shapes": r}), flush=True)
         return 0
 
     if args.share:

@@ -200,8 +200,14 @@ __device__ __forceinline__ void spmv_block(const Desc d, const int* __restrict__
         cval[tid] = acc;
         __syncthreads();
         if (first >= 0 && tid > 0 && crow[tid - 1] == first) {
-            int q = tid - 1;
-            while (q > 0 && crow[q - 1] == first) --q;
+            // The partial sums of row `first` left open by the threads before
+            // this one, in thread order. They start at the thread whose piece
+            // holds the row's first nonzero, merged position P = first + (its
+            // offset): the last thread t with t·L/WG <= P. (Threads before it
+            // that end with this row open have added nothing to it: their +0
+            // leaves the sum's bits unchanged, so they need not be walked.)
+            const long long P = (long long)first + (first > 0 ? rend[first - 1] : 0);
+            int q = (int)(((P + 1) * WG - 1) / L);
             A t = 0;
             for (; q < tid; ++q) t += cval[q];
             sums[first] = t + sums[first];

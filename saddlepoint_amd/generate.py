@@ -137,6 +137,23 @@ def config2(seed: int = seed_of(2), scale: float = 1.0) -> LpProblem:
     return _finish(rng, a, eq_frac=0.6, name=f"config2-{scale:g}")
 
 
+def shaped_lp(seed: int, m: int, n: int, nnz: int) -> LpProblem:
+    """A feasible LP of exactly m rows and n columns with about nnz nonzeros at
+    uniformly random positions (every column gets nnz/n of them, every row at
+    least one), values ±10^U(-3, 3), 60% equality rows: a random stand-in of a
+    given size for an instance that is not available."""
+    rng = np.random.default_rng(seed)
+    per = max(1, nnz // n)
+    cols = np.repeat(np.arange(n), per)
+    extra = nnz - cols.size
+    if extra > 0:
+        cols = np.concatenate([cols, rng.integers(0, n, extra)])
+    rows = rng.integers(0, m, cols.size)
+    rows[: min(m, rows.size)] = np.arange(min(m, rows.size))  # no empty row
+    a = _csr_from_rows_cols(m, n, rows, cols, _values(rng, cols.size, -3.0, 3.0))
+    return _finish(rng, a, eq_frac=0.6)
+
+
 def serving_lp(seed: int, m_lo: int = 1000, m_hi: int = 200_000, zipf_rows: float | None = None) -> LpProblem:
     """One LP of the serving stream: rows log-uniform in [m_lo, m_hi], columns
     U(1.5, 3)× rows, per-column nonzeros uniform in 4..30 (or Zipf row lengths

@@ -199,3 +199,12 @@ def test_native_mps_reader_sections_and_errors(tmp_path):
     assert e.value.line == 5
     with pytest.raises(OSError):
         mps.read_mps_file_native(tmp_path / "missing.mps")
+
+
+def test_shaped_lp_has_the_requested_shape():
+    from saddlepoint_amd import generate
+
+    p = generate.shaped_lp(5, 50, 400, 3000)
+    assert (p.nrows, p.ncols) == (50, 400)
+    assert 0.9 * 3000 <= p.a.nnz <= 3000
+    assert np.diff(p.a.row_ptr).min() >= 1 and np.bincount(p.a.col_idx, minlength=400).min() >= 1
