@@ -208,3 +208,13 @@ def test_shaped_lp_has_the_requested_shape():
     assert (p.nrows, p.ncols) == (50, 400)
     assert 0.9 * 3000 <= p.a.nnz <= 3000
     assert np.diff(p.a.row_ptr).min() >= 1 and np.bincount(p.a.col_idx, minlength=400).min() >= 1
+
+
+def test_default_server_params():
+    import ctypes as C
+
+    from saddlepoint_amd._lib import ServerParams, lib
+
+    p = ServerParams()
+    lib().sp_default_server_params(C.byref(p))
+    assert (p.budget_bytes, p.max_active, p.prep_threads, p.use_graphs, p.chunk_work) == (64 << 30, 64, 8, 1, 0)
