@@ -150,9 +150,10 @@ typedef struct {
                   A-SpMV+dual+Halpern kernel per iteration (bitwise equal to 0) */
     int log_every_checks; /* 0 = silent (solver.rs:49-50) */
     int precision;        /* 0: FP64 throughout. 1: mixed (needs fused = 1): between checks the
-                             matrix values, iterates and SpMV sums are FP32, while the Halpern
-                             anchors, every check (KKT, restart residuals, PID, infeasibility)
-                             and the returned solution stay FP64 */
+                             matrix values, the SpMV sums and the iterates' displacements from
+                             the FP64 iterate at the last check are FP32, while that base, the
+                             Halpern anchors, every check (KKT, restart residuals, PID,
+                             infeasibility) and the returned solution stay FP64 */
     int spmv_autotune;    /* 0: every matrix gets the default SpMV plan; 1: the autotuned rule
                              (sp_spmv_tuning_for) picks each matrix's plan from its row lengths */
     int gpu_scaling;      /* 1 (default): A is uploaded unscaled; the transpose and the scaling

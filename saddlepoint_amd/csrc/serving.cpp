@@ -1,10 +1,12 @@
 // Continuous-batching serving loop on one GPU.
 //
-// LPs are submitted at any time. Host worker threads validate and scale them
-// (ScaledProblem::new on the host); the loop admits prepared LPs while the
-// HBM budget and the slot count allow, keeps every admitted LP's iterates,
-// anchors, ω, step sizes and restart state on the device, and retires LPs as
-// they terminate.
+// LPs are submitted at any time. Host worker threads validate them and scale
+// the small ones (ScaledProblem::new on the host; large ones are scaled on the
+// device at admission); the loop admits prepared LPs while the HBM budget and
+// the slot count allow, keeps every admitted LP's iterates, anchors, ω, step
+// sizes and restart state on the device, and retires LPs as they terminate.
+// Each chunk runs the admitted LPs that the chunk schedule selects (the
+// cheapest up to a work target, and any LP left out of the last chunks).
 //
 // The loop runs in chunks of 64 iterations aligned to every LP's own check
 // cadence (an LP is admitted at a chunk boundary with its iteration count at
@@ -537,8 +539,8 @@ class Server {
     // `steps` batched plain iterations of `group`, through the FP64 and the
     // mixed-precision batch tables.
     void run_batched(const std::vector<Job*>& group, int steps, bool graph, Batch& b64, Batch& b32) {
-        // Mixed-precision LPs entering plain iterations: anchor products and
-        // displacements, for all of them at once.
+        // Mixed-precision LPs entering plain iterations: the base's products,
+        // zero displacements and the anchors' offsets, for all of them at once.
         std::vector<PlainEntry> entries;
         for (Job* j : group) {
             PlainEntry e;
