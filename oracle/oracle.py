@@ -50,7 +50,8 @@ class SolverParams(C.Structure):
                 ("time_limit_s", _d), ("check_every", _u64), ("step_size_factor", _d), ("reflection", _d),
                 ("restart_sufficient", _d), ("restart_necessary", _d), ("restart_artificial", _d),
                 ("pid_kp", _d), ("pid_ki", _d), ("pid_kd", _d), ("pid_discount", _d),
-                ("power_iterations", _i64), ("scaling", ScalingParams), ("termination", C.c_int)]
+                ("power_iterations", _i64), ("scaling", ScalingParams), ("termination", C.c_int),
+                ("log_every_checks", C.c_int)]
 
 
 class Solution(C.Structure):

@@ -558,6 +558,7 @@ void or_default_solver_params(or_solver_params *p) {
     p->pid_ki = 0.01;
     p->pid_kd = 0.0;
     p->pid_discount = 0.3;
+    p->log_every_checks = 0;
     p->power_iterations = 200;
     or_default_scaling_params(&p->scaling);
     p->termination = 0;
@@ -748,7 +749,7 @@ static int run(solver_t *S, or_solution *sol, double t_start, double *x_out, dou
     double omega = p->scaling.bound_objective_rescaling ? 1.0 : (1.0 + c_norm) / (1.0 + b_norm);
     or_pid pid;
     or_pid_init(&pid, omega);
-    uint64_t total = 0, k = 0, restarts = 0;
+    uint64_t total = 0, k = 0, restarts = 0, checks = 0;
     double anchor_residual = INFINITY, prev_residual = INFINITY;
     int status;
     kkt_t kk;
@@ -780,6 +781,15 @@ static int run(solver_t *S, or_solution *sol, double t_start, double *x_out, dou
         or_halpern_step(S->y, S->y_t, S->y_anchor, w, p->reflection, m);
         k += 1;
         if (!check) continue;
+        checks += 1;
+        if (p->log_every_checks > 0 && checks % (uint64_t)p->log_every_checks == 0) {
+            /* solver.rs:367-378 */
+            fprintf(stderr, "%9llu %14.6e %14.6e %9.2e %9.2e %9.2e %9.2e %5llu %8.2fs %7llu %9.2e\n",
+                    (unsigned long long)total, kk.primal_obj, kk.dual_obj, kk.primal_res, kk.dual_res,
+                    kkt_gap(&kk), omega, (unsigned long long)restarts, now_s() - start,
+                    (unsigned long long)k, residual / anchor_residual);
+            fflush(stderr);
+        }
         kkt_t test = kk;
         if (p->termination == 1) {
             test.dual_obj = step_obj;

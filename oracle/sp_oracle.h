@@ -73,6 +73,7 @@ typedef struct {
     int64_t power_iterations; /* 200 */
     or_scaling_params scaling;
     int termination; /* 0 projected, 1 step-implied */
+    int log_every_checks; /* 0: silent; else the progress line of solver.rs:367-378 on stderr */
 } or_solver_params;
 
 enum { OR_OPTIMAL = 0, OR_PRIMAL_INFEASIBLE = 1, OR_DUAL_INFEASIBLE = 2, OR_ITERATION_LIMIT = 3, OR_TIME_LIMIT = 4 };
