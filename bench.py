@@ -178,6 +178,22 @@ def serving_bench(args) -> dict:
     limit = args.lp_time_limit if args.lp_time_limit > 0 else -1.0
     it_cap = args.lp_max_iterations
     autotune = int(args.config == 4)  # config 4: each LP's SpMV plans from the autotuned rule
+    modes = [m.strip() for m in args.modes.split(",") if m.strip()]
+    if "one" not in modes:
+        # Served modes only (for profiling): no baseline, no ratios.
+        out = {"lps": len(lps), "config": args.config}
+        for prec in [m for m in modes if m in ("fp64", "mixed")]:
+            pr = int(prec == "mixed")
+            pm = (lambda i: default_params(tolerance=tols[i], precision=pr, time_limit_s=limit,
+                                           max_iterations=it_cap, spmv_autotune=autotune * pr,
+                                           mixed_stall_checks=args.stall_checks))
+            mk, lat, res = run_served(b, lps, pm, max_active=args.max_active, chunk_work=args.chunk_work,
+                                      use_graphs=not args.no_graphs)
+            out["served_" + prec] = {"lps_per_s": len(lps) / mk, "makespan_s": mk, "p50_s": pct(lat, 50),
+                                     "p99_s": pct(lat, 99)}
+            log(f"served {prec}: {len(lps) / mk:.3f} LPs/s, makespan {mk:.1f}s")
+        b.close()
+        return out
     # Plain FP64 one at a time: every LP's whole solve (validate, scale, load,
     # solve, download), back to back.
     service, one = [], []
@@ -474,6 +490,9 @@ def main() -> int:
     ap.add_argument("--config", type=int, default=3, choices=[3, 4], help="serving stream (3, or 4 with large LPs)")
     ap.add_argument("--lps", type=int, default=64, help="serving: LPs in the sample")
     ap.add_argument("--small", action="store_true", help="serving: small LPs only (rows log-uniform 1k-5k)")
+    ap.add_argument("--modes", default="one,fp64,mixed",
+                    help="serving: modes to run (without 'one' only the served ones, no ratios: for profiling)")
+    ap.add_argument("--no-graphs", action="store_true", help="serving: eager launches instead of hipGraph replay")
     ap.add_argument("--chunk-work", type=int, default=0,
                     help="serving: sp_server_params.chunk_work (0: the default; negative: every LP every chunk)")
     ap.add_argument("--max-active", type=int, default=64, help="serving: LPs admitted at once")
@@ -518,6 +537,9 @@ def main() -> int:
 
     if args.serving:
         r = serving_bench(args)
+        if "one_at_a_time_fp64" not in r:
+            print(json.dumps({"metric": METRIC, "serving_modes": r}), flush=True)
+            return 0
         sm = r["served_mixed"]
         print(json.dumps({"metric": METRIC, "value": round(sm["lps_per_s"], 4), "unit": "LPs/s", "n_gpus": 1,
                           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
