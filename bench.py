@@ -418,9 +418,7 @@ def reference_shapes_bench(args) -> dict:
 
 def full_solve_bench(args) -> dict:
     """Config 1 solved to 1e-6 on the GPU (plain FP64, the default options),
-    its KKT measures from the device checker; with --with-oracle also the
-    oracle's solve of the same LP on the host's cores, for the status,
-    iteration and restart counts and the objective side by side."""
+    its KKT measures from the device checker."""
     from saddlepoint_amd import generate
     from saddlepoint_amd.backend import HipBackend, check, default_params, solve
 
@@ -440,23 +438,6 @@ def full_solve_bench(args) -> dict:
         "kkt_relative_dual": k.relative_dual, "kkt_relative_gap": k.relative_gap}}
     log(f"  gpu: {r.status} {r.iterations} it, {r.restarts} restarts, solve {r.solve_time:.2f}s, "
         f"objective {r.objective:.10e}, kkt max {k.max_relative():.2e}")
-    if args.with_oracle:
-        from oracle import oracle as O
-
-        cores = min(os.cpu_count() or 1, 16)
-        O.set_threads(cores)
-        t = time.perf_counter()
-        c, cx, cy = O.solve(p, O.default_params(tolerance=1e-6, max_iterations=args.max_iterations,
-                                                time_limit_s=args.oracle_time_limit, log_every_checks=20))
-        owall = time.perf_counter() - t
-        ok = O.check(p, cx, cy)
-        out["oracle"] = {"status": O.STATUS[c.status], "iterations": c.iterations, "restarts": c.restarts,
-                         "objective": c.objective, "wall_s": owall, "threads": cores,
-                         "kkt_max_relative": ok.max_relative()}
-        out["iteration_ratio"] = r.iterations / max(1, c.iterations)
-        out["objective_gap"] = abs(r.objective - c.objective) / (1 + abs(c.objective))
-        log(f"  oracle ({cores} threads): {O.STATUS[c.status]} {c.iterations} it, {c.restarts} restarts, "
-            f"{owall:.1f}s, objective {c.objective:.10e}, kkt max {ok.max_relative():.2e}")
     b.close()
     return out
 
@@ -479,8 +460,6 @@ def main() -> int:
     ap.add_argument("--time-limit", type=float, default=150.0, help="--large: seconds per solve")
     ap.add_argument("--full-solve", action="store_true",
                     help="config 1 solved to 1e-6 on the GPU (KKT from the device checker)")
-    ap.add_argument("--with-oracle", action="store_true", help="--full-solve: also the oracle's solve, side by side")
-    ap.add_argument("--oracle-time-limit", type=float, default=900.0, help="--full-solve: seconds for the oracle")
     ap.add_argument("--reference-shapes", action="store_true",
                     help="us/iteration on random LPs of the reference's published instance sizes, beside its figures")
     ap.add_argument("--share", type=int, default=0,
