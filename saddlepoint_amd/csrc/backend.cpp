@@ -1005,7 +1005,8 @@ sp_backend* sp_backend_open(int ordinal) {
         hipDeviceProp_t prop{};
         sp::hip_check(hipGetDeviceProperties(&prop, ordinal), "hipGetDeviceProperties");
         b->cus = prop.multiProcessorCount;
-        b->name = std::string("hip (") + prop.name + ", " + prop.gcnArchName + ")";
+        const std::string dev = prop.name[0] ? std::string(prop.name) : "device " + std::to_string(ordinal);
+        b->name = "hip (" + dev + ", " + prop.gcnArchName + ")";
         // One default-priority stream carries every operation of this backend.
         sp::hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
         const std::string path = code_object_path();
