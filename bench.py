@@ -115,7 +115,7 @@ def fifo_latencies(arrivals, service):
     return out
 
 
-def run_served(b, lps, params_of, arrivals=None, max_active=64, with_solution=False, use_graphs=None,
+def run_served(b, lps, params_of, arrivals=None, max_active=256, with_solution=False, use_graphs=None,
                chunk_work=None):
     """Serves `lps` (all at t = 0, or at the given arrival times on the
     server's clock); returns (makespan, latencies, results)."""
@@ -474,7 +474,7 @@ def main() -> int:
     ap.add_argument("--no-graphs", action="store_true", help="serving: eager launches instead of hipGraph replay")
     ap.add_argument("--chunk-work", type=int, default=0,
                     help="serving: sp_server_params.chunk_work (0: the default; negative: every LP every chunk)")
-    ap.add_argument("--max-active", type=int, default=64, help="serving: LPs admitted at once")
+    ap.add_argument("--max-active", type=int, default=256, help="serving: LPs admitted at once")
     ap.add_argument("--poisson", action="store_true", help="serving: also Poisson arrivals (mixed precision)")
     ap.add_argument("--rate-frac", type=float, default=0.8, help="serving: arrival rate / one-at-a-time capacity")
     ap.add_argument("--lp-time-limit", type=float, default=0.0, help="serving: seconds per LP (0: none)")

@@ -299,7 +299,7 @@ int sp_spmv_time(sp_backend *b, const sp_problem *p, int transpose, int reps, do
 typedef struct sp_server sp_server;
 typedef struct {
     int64_t budget_bytes; /* device bytes the admitted LPs may hold (default 64 GiB) */
-    int max_active;       /* most LPs admitted at once (default 64) */
+    int max_active;       /* most LPs admitted at once (default 256) */
     int prep_threads;     /* host threads validating and scaling (default 8) */
     int use_graphs;       /* replay the batched steps of a chunk from a hipGraph (default 1) */
     int64_t chunk_work;   /* which admitted LPs a chunk runs: the cheapest ones (cost nnz + m + n)

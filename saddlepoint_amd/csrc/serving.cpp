@@ -823,7 +823,7 @@ extern "C" {
 void sp_default_server_params(sp_server_params* p) {
     if (!p) return;
     p->budget_bytes = (int64_t)64 << 30;
-    p->max_active = 64;
+    p->max_active = 256;
     p->prep_threads = 8;
     p->use_graphs = 1;
     p->chunk_work = 0;
