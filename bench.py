@@ -2,14 +2,24 @@
 
 Workload: the plain-FP64 single solve at relaxation scale (400k rows × 900k
 columns, ~8M nonzeros, Zipf(1.3) column counts, 20 dense rows of 20k-60k
-nonzeros; seeded synthetic data). A step is one PDHG iteration of the full
-driver, including the KKT/restart/infeasibility check work every 64
-iterations; the timed region holds K steps with the problem already resident
-in HBM.
+nonzeros; seeded synthetic data); ``--workload config2`` measures the large
+instance instead (4M × 10M, ~100M nonzeros). A step is one PDHG iteration of
+the full driver. The warm-up and the timed window are whole 64-iteration
+check windows (the requested counts rounded up; ``steps`` in the line is the
+number actually timed) and the window starts on a check boundary, so the
+rate includes the KKT/restart/infeasibility check every 64 iterations; the
+problem is resident in HBM when the window starts.
 
 Also reported, on one JSON line:
   roofline     the dominant kernel's algorithmic bytes per launch over its
-               average HIP-event duration, against the 8 TB/s HBM peak;
+               average duration (two HIP events around 20 back-to-back
+               launches on the solver's stream), against the 8 TB/s HBM
+               peak; ``gather`` puts its gathers per second against the
+               replay of the same matrix's column indices with nothing but
+               the gathers (the ceiling its access pattern allows);
+  serving      a fixed config-3 sample (the first 16 LPs of the stream)
+               solved one at a time in FP64 and served in FP64 and mixed
+               precision: LPs/s, p50/p99 latency;
   cpu_baseline the oracle (a C restatement of the reference's CPU backend and
                driver) on the host cores, over a bounded sample of the same
                workload.
@@ -449,9 +459,13 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=128)
     ap.add_argument("--scale", type=float, default=1.0, help="shrink the workload (tests only)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="config1",
+                    help="the LP the iteration rate is measured on (config1: the headline)")
+    ap.add_argument("--serving-sample", type=int, default=16,
+                    help="LPs of the config-3 stream served beside the headline (0: none)")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--probe", choices=["copy", "triad", "gather"],
+    ap.add_argument("--probe", choices=["copy", "triad", "gather", "copy_f32", "triad_f32"],
                     help="run only a bandwidth probe (copy: 2^27 doubles, the HBM counter calibration)")
     ap.add_argument("--table", type=int, default=900_000, help="gather probe: table length in doubles")
     ap.add_argument("--large", action="store_true",
@@ -491,8 +505,7 @@ def main() -> int:
               "multi-GPU path; LPs shard across GPUs as independent instances)", file=sys.stderr)
         return 2
 
-    from saddlepoint_amd import generate
-    from saddlepoint_amd.backend import HipBackend, Solver, default_params
+    from saddlepoint_amd.backend import HipBackend
 
     if args.large:
         r = large_bench(args)
@@ -537,103 +550,193 @@ def main() -> int:
         n = 1 << 27
         table = args.table if args.probe == "gather" else 0
         ms = b.probe(args.probe, n, table, reps=10)
-        per = {"copy": 16, "triad": 24, "gather": 12}[args.probe]
+        per = {"copy": 16, "triad": 24, "gather": 12, "copy_f32": 8, "triad_f32": 12}[args.probe]
         print(json.dumps({"probe": args.probe, "elements": n, "table": table, "ms_per_launch": ms,
                           "bytes_per_launch": per * n, "gbs": per * n / ms / 1e6}), flush=True)
         return 0
 
+    out = iteration_bench(args)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+CHECK = 64  # the driver's check cadence (solver.rs:340): windows start and end on a check
+
+
+def whole_windows(k: int) -> int:
+    """k rounded up to whole 64-iteration check windows (at least one)."""
+    return max(CHECK, -(-int(k) // CHECK) * CHECK)
+
+
+WORKLOADS = {
+    "config1": "config1 plain-FP64 single solve: 400k x 900k LP, ~8M nnz, Zipf(1.3) columns + 20 dense rows "
+               "(columns relabeled by popularity, the default)",
+    "config2": "config2 large single instance: 4M x 10M LP, ~100M nnz, Zipf(1.5) row lengths, uniform columns "
+               "(columns relabeled by popularity, the default)",
+}
+
+
+def iteration_bench(args) -> dict:
+    """Iterations per second of the full driver on one LP resident in HBM.
+
+    The warm-up and the timed window are whole 64-iteration check windows
+    (the requested counts rounded up), and the timed window starts on a check
+    boundary, so the rate includes the check work at its real share: one
+    check (KKT, restart residuals, infeasibility rays, PID) per 64
+    iterations."""
+    from saddlepoint_amd import generate
+    from saddlepoint_amd.backend import HipBackend, Solver, default_params
+
+    mixed = args.precision == "mixed"
     t0 = time.time()
-    p = generate.config1(scale=args.scale)
+    p = generate.config1(scale=args.scale) if args.workload == "config1" else generate.config2(scale=args.scale)
     m, n, nnz = p.nrows, p.ncols, p.a.nnz
-    log(f"generated config1 m={m} n={n} nnz={nnz} in {time.time() - t0:.1f}s")
+    log(f"generated {args.workload} m={m} n={n} nnz={nnz} in {time.time() - t0:.1f}s")
 
     b = HipBackend(0)
     log(f"backend: {b.name()}")
     t0 = time.time()
-    s = Solver(b, p, default_params(tolerance=1e-6, fused=1, precision=int(args.precision == "mixed")))
+    s = Solver(b, p, default_params(tolerance=1e-6, fused=1, precision=int(mixed)))
     log(f"solver ready (scale + load + power iteration) in {time.time() - t0:.1f}s")
-    s.advance(args.warmup)
+    warmup, steps = whole_windows(args.warmup), whole_windows(args.steps)
+    s.advance(warmup)
     b.sync()
-    log(f"warmup {args.warmup} iterations done")
+    log(f"warmup {warmup} iterations done (requested {args.warmup})")
 
-    # Timed region: K iterations, nothing but the solver on the stream.
+    # Timed region: whole check windows, nothing but the solver on the stream.
     it0 = s.result(with_solution=False).iterations
+    assert it0 % CHECK == 0, it0
     t0 = time.perf_counter()
-    done = s.advance(args.steps)
+    done = s.advance(steps)
     b.sync()
     t1 = time.perf_counter()
     res = s.result(with_solution=False)
-    steps = res.iterations - it0
+    ran = res.iterations - it0
     elapsed = t1 - t0
-    log(f"timed {steps} iterations in {elapsed:.3f}s (terminated={done}, status={res.status})")
-    # The same number of iterations again with a HIP event pair around every
-    # launch on the solver's stream, for per-kernel durations.
+    log(f"timed {ran} iterations (iterations {it0 + 1}-{it0 + ran}, {ran // CHECK} checks; requested {args.steps}) "
+        f"in {elapsed:.3f}s (terminated={done}, status={res.status})")
+
+    # Per-kernel breakdown: the same number of iterations again with a HIP
+    # event pair around every launch (adds a little to each kernel).
     b.profile(True)
-    s.advance(args.steps)
+    s.advance(steps)
     b.sync()
     prof = b.profile_read()
     b.profile(False)
-
     kb = kernel_bytes(m, n, nnz)
     per_kernel = {}
     for name, (ms, launches) in prof.items():
         if launches:
-            avg_us = 1e3 * ms / launches
-            entry = {"launches": launches, "avg_us": round(avg_us, 2), "total_ms": round(ms, 3)}
-            if name in kb:
-                entry["gbs"] = round(kb[name] / (avg_us * 1e-6) / 1e9, 1)
-            per_kernel[name] = entry
-    dom = max((k for k in per_kernel if k in kb), key=lambda k: per_kernel[k]["total_ms"])
+            per_kernel[name] = {"launches": launches, "avg_us_event_bracketed": round(1e3 * ms / launches, 2),
+                                "total_ms": round(ms, 3)}
+    sides = {0: "sp_spmv_t_primal_mixed" if mixed else "sp_spmv_t_primal_halpern_f64",
+             1: "sp_spmv_dual_mixed" if mixed else "sp_spmv_dual_halpern_f64"}
+    # Each side's main kernel timed without per-launch events: two events
+    # around 20 back-to-back launches on copies of the vectors it writes.
+    for side, name in sides.items():
+        us = 1e3 * s.kernel_time(side, reps=20)
+        e = per_kernel.setdefault(name, {})
+        e["avg_us"] = round(us, 2)
+        e["gbs"] = round(kb[name] / (us * 1e-6) / 1e9, 1)
+        e["bytes_per_launch"] = kb[name]
+    dom = max(sides.values(), key=lambda k: per_kernel[k]["avg_us"])
     ach = per_kernel[dom]["gbs"]
-    b_iter = 24 * nnz + 52 * m + 68 * n if args.precision == "fp64" else 16 * nnz + 48 * m + 60 * n
-    traffic = read_traffic(args.traffic, dom) if args.scale == 1.0 and args.precision == "fp64" else None
+    b_iter = 24 * nnz + 52 * m + 68 * n if not mixed else 16 * nnz + 48 * m + 60 * n
+    traffic = None
+    if args.scale == 1.0:
+        traffic = read_traffic(os.path.join(ROOT, "profiles", f"traffic_{args.workload}{'_mixed' if mixed else ''}.txt"),
+                               dom)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "bytes_per_launch": kb[dom]}
-    # The kernel's practical ceiling: one random gather per nonzero from the
-    # gathered vector (x-bar on the A side, y on the A-transpose side); the
-    # probe measures the rate of random 8-byte gathers from a table that size.
-    gathered = n if "dual" in dom or dom == "sp_spmv_f64" else m
-    if "mixed" in dom:
-        gathered = max(1, gathered // 2)  # FP32 values: the table of a double probe half as long
-    probe_ms = b.probe("gather", 1 << 26, gathered, reps=10)
-    probe_rate = (1 << 26) / (probe_ms * 1e-3)
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "bytes_per_launch": kb[dom]}
+    # The kernel's practical ceiling: the same matrix's gathers alone, in the
+    # SpMV's order and tiling, from a table as long as the gathered vector
+    # (x-bar on the A side, y on the A-transpose side).
+    side = 1 if dom == sides[1] else 0
+    replay_ms = b.gather_replay_time(s.problem(), transpose=side == 0, fp32=mixed, reps=20)
     kernel_rate = nnz / (per_kernel[dom]["avg_us"] * 1e-6)
-    roofline["gather"] = {"gathers_per_launch": nnz, "table_bytes": 8 * gathered,
-                          "achieved_per_s": round(kernel_rate / 1e9, 1), "probe_per_s": round(probe_rate / 1e9, 1),
-                          "unit": "G gathers/s", "frac": round(kernel_rate / probe_rate, 3)}
+    replay_rate = nnz / (replay_ms * 1e-3)
+    gathered = n if side == 1 else m
+    roofline["gather"] = {"gathers_per_launch": nnz, "table_bytes": (4 if mixed else 8) * gathered,
+                          "achieved_per_s": round(kernel_rate / 1e9, 1), "ceiling_per_s": round(replay_rate / 1e9, 1),
+                          "ceiling": "the matrix's own column-index stream replayed, gathers only",
+                          "unit": "G gathers/s", "frac": round(kernel_rate / replay_rate, 3)}
+    other = sides[1 - side]
+    other_replay = b.gather_replay_time(s.problem(), transpose=side == 1, fp32=mixed, reps=20)
+    roofline["gather_other_side"] = {
+        "kernel": other, "achieved_per_s": round(nnz / (per_kernel[other]["avg_us"] * 1e-6) / 1e9, 1),
+        "ceiling_per_s": round(nnz / (other_replay * 1e-3) / 1e9, 1),
+        "frac": round(other_replay * 1e3 / per_kernel[other]["avg_us"], 3)}
 
     out = {
         "metric": METRIC,
-        "value": round(steps / elapsed, 2),
+        "value": round(ran / elapsed, 2),
         "unit": "iters/s",
         "n_gpus": 1,
-        "steps": steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(1e3 * elapsed / steps, 4),
+        "steps": ran,
+        "warmup": warmup,
+        "steps_requested": args.steps,
+        "warmup_requested": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / ran, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "dtype": "f32+f64" if mixed else "f64",
         "data": "synthetic (seeded generator; random instance of the named shape)",
-        "config": {"workload": "config1 plain-FP64 single solve: 400k x 900k LP, ~8M nnz, Zipf(1.3) columns + "
-                               "20 dense rows (columns relabeled by popularity, the default)", "m": m, "n": n,
-                   "nnz": nnz, "scale": args.scale,
-                   "parallelism": "single GPU"},
-        "iteration_gbs": round(b_iter * steps / elapsed / 1e9, 1),
-        "iteration_roofline_frac": round(b_iter * steps / elapsed / 1e9 / HBM_PEAK_GBS, 4),
+        "config": {"workload": WORKLOADS[args.workload] + ("; mixed precision" if mixed else ""), "m": m, "n": n,
+                   "nnz": nnz, "scale": args.scale, "timed_window": f"iterations {it0 + 1}-{it0 + ran}, "
+                   f"{ran // CHECK} whole check windows (checks included)", "parallelism": "single GPU"},
+        "iteration_gbs": round(b_iter * ran / elapsed / 1e9, 1),
+        "iteration_roofline_frac": round(b_iter * ran / elapsed / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": roofline,
         "kernels": per_kernel,
     }
-    if not args.no_cpu:
+    s.close()
+    if args.serving_sample > 0:
+        out["serving"] = serving_sample(b, args.serving_sample)
+    b.close()
+    if not args.no_cpu and args.workload == "config1":
         log("timing the CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline(p, args.cpu_seconds)
     else:
         out["cpu_baseline"] = None
-    print(json.dumps(out), flush=True)
-    s.close()
-    b.close()
-    return 0
+    return out
+
+
+def serving_sample(b, count: int) -> dict:
+    """The headline's serving figures on a fixed sample: the first `count`
+    LPs of the config-3 stream (rows log-uniform 1k-200k, tolerance 50/50
+    1e-4 / 1e-6), all submitted at t = 0, solved one at a time in plain FP64
+    and served (continuous batching) in FP64 and in mixed precision."""
+    from saddlepoint_amd import generate
+    from saddlepoint_amd.backend import default_params, solve
+
+    t0 = time.time()
+    lps = generate.serving_stream(3, count)
+    tols = [1e-4 if i % 2 == 0 else 1e-6 for i in range(count)]
+    log(f"serving sample: {count} config-3 LPs generated in {time.time() - t0:.1f}s")
+    service, one = [], []
+    for p, tol in zip(lps, tols):
+        t = time.perf_counter()
+        one.append(solve(b, p, default_params(tolerance=tol)))
+        service.append(time.perf_counter() - t)
+    total = sum(service)
+    lat_one = fifo_latencies([0.0] * count, service)
+    out = {"lps": count, "workload": f"config-3 stream instances 0-{count - 1}, all submitted at t=0",
+           "tolerances": "50/50 1e-4 / 1e-6",
+           "one_at_a_time_fp64": {"lps_per_s": count / total, "makespan_s": total, "p50_s": pct(lat_one, 50),
+                                  "p99_s": pct(lat_one, 99)}}
+    log(f"  one at a time: {count / total:.3f} LPs/s, makespan {total:.1f}s")
+    for prec in ("fp64", "mixed"):
+        pr = int(prec == "mixed")
+        mk, lat, res = run_served(b, lps, lambda i: default_params(tolerance=tols[i], precision=pr))
+        st = {}
+        for r in res:
+            st[r.status] = st.get(r.status, 0) + 1
+        out["served_" + prec] = {"lps_per_s": count / mk, "makespan_s": mk, "p50_s": pct(lat, 50),
+                                 "p99_s": pct(lat, 99), "speedup_vs_one_at_a_time": total / mk, "statuses": st}
+        log(f"  served {prec}: {count / mk:.3f} LPs/s (x{total / mk:.2f}), p50 {pct(lat, 50):.2f}s "
+            f"p99 {pct(lat, 99):.2f}s")
+    return out
 
 
 if __name__ == "__main__":

@@ -77,8 +77,10 @@ int sp_backend_create(int ordinal, sp_backend **out);
 void sp_backend_destroy(sp_backend *b);
 /* Backend::name (backend.rs:48). The string lives as long as the backend. */
 const char *sp_backend_name(const sp_backend *b);
-/* Message of the last failed call on this backend ("" if none). b may be NULL
- * for failures of sp_backend_create. */
+/* Message of the calling thread's last failed call on this backend ("" if
+ * none, or if its last call succeeded). Messages are kept per thread, so a
+ * call failing on one thread never changes the text another thread reads.
+ * b may be NULL for failures of sp_backend_create. */
 const char *sp_last_error(const sp_backend *b);
 
 /* ---- problem -------------------------------------------------------------- */
@@ -202,6 +204,17 @@ int sp_solver_create(sp_backend *b, const sp_lp_view *lp, const sp_params *param
 int sp_solver_advance(sp_solver *s, uint64_t iterations, int *done);
 int sp_solver_result(sp_solver *s, sp_solution *out, double *x_out, double *y_out);
 void sp_solver_free(sp_solver *s);
+/* Measurement (needs params.fused = 1): average milliseconds of one launch of
+ * the plain iteration's main kernel of one side (0: the A'ᵀ SpMV with the
+ * primal step and Halpern update, 1: the A' SpMV with the dual step and
+ * Halpern update; the FP32 forms in mixed precision), from two events around
+ * `reps` back-to-back launches on copies of what the kernel writes. The
+ * solver's iterates are unchanged. */
+int sp_solver_kernel_time(sp_solver *s, int side, int reps, double *ms);
+/* The solver's loaded problem (scaled, columns relabeled when the solver
+ * relabels them), owned by the solver and valid until sp_solver_free: for
+ * measurements such as sp_gather_replay_time on the matrices a solve uses. */
+const sp_problem *sp_solver_problem(const sp_solver *s);
 
 /* ---- measurement ------------------------------------------------------------- */
 
@@ -213,12 +226,20 @@ int sp_backend_profile(sp_backend *b, int enable);
 int sp_backend_profile_read(sp_backend *b, int index, char *name, int name_len, double *total_ms,
                             int64_t *launches);
 /* Bandwidth probe (the counterpart of the reference's sp-bandwidth,
- * crates/saddlepoint-cuda/src/probe.rs:13-44): average milliseconds per
- * launch of kind 0 = copy of n doubles (16 bytes per element), 1 = triad
- * c = a + s·b over n doubles (24 bytes per element), 2 = n random 8-byte
- * gathers from a table of table_len doubles (indices streamed, results
- * written: 12 bytes per element plus the gather). */
+ * crates/saddlepoint-cuda/src/probe.rs:13-44, whose copy and triad kernels
+ * come in f64 and f32, probe.rs:38-41): average milliseconds per launch of
+ * kind 0 = copy of n doubles (16 bytes per element), 1 = triad c = a + s·b
+ * over n doubles (24 bytes per element), 2 = n random 8-byte gathers from a
+ * table of table_len doubles (indices streamed, results written: 12 bytes
+ * per element plus the gather), 3 = copy of n floats (8 bytes per element),
+ * 4 = triad over n floats (12 bytes per element). */
 int sp_probe(sp_backend *b, int kind, int64_t n, int64_t table_len, int reps, double *ms_per_launch);
+/* No counterpart in the reference: the ceiling a matrix's own access pattern
+ * puts on its SpMV's gathers. Average milliseconds of one launch that loads,
+ * for every nonzero of A' (transpose = 0) or A'ᵀ (1) of a loaded problem and
+ * in the SpMV's order and tiling, the table entry its column index names
+ * (a double, or a float with fp32 = 1) and nothing else. */
+int sp_gather_replay_time(sp_backend *b, const sp_problem *p, int transpose, int fp32, int reps, double *ms);
 /* Waits for all work queued on the backend's stream. */
 int sp_backend_sync(sp_backend *b);
 
@@ -294,8 +315,10 @@ int sp_spmv_time(sp_backend *b, const sp_problem *p, int transpose, int reps, do
  * per-LP checks) and admits and retires LPs at its edges, by HBM budget and
  * slot count. Each LP's result is bitwise the one it gets alone. Submitted
  * params need check_every = 64; max_iterations is rounded up to a multiple
- * of 64. Not thread-safe per server except sp_server_submit and
- * sp_server_state, which may be called while another thread runs chunks. */
+ * of 64 (a limit above the largest multiple, such as UINT64_MAX for "no
+ * limit", becomes that multiple). Not thread-safe per server except
+ * sp_server_submit and sp_server_state, which may be called while another
+ * thread runs chunks (each thread reads its own sp_last_error). */
 typedef struct sp_server sp_server;
 typedef struct {
     int64_t budget_bytes; /* device bytes the admitted LPs may hold (default 64 GiB) */

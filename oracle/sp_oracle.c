@@ -112,7 +112,9 @@ static void csr_mul(const or_csr *a, const double *x, double *out) {
 }
 
 static void csr_scale(or_csr *a, const double *row, const double *col) {
-    /* problem.rs:120-126: v *= row_i * col_j (the product is formed first). */
+    /* problem.rs:120-126: v *= row_i * col_j (the product is formed first).
+     * Every element is its own product, so threads change nothing. */
+#pragma omp parallel for schedule(dynamic, 1024) num_threads(g_threads) if (par(a->nnz))
     for (int64_t i = 0; i < a->nrows; ++i) {
         double ri = row[i];
         for (int64_t k = a->row_ptr[i]; k < a->row_ptr[i + 1]; ++k) a->values[k] *= ri * col[a->col_idx[k]];
@@ -165,8 +167,63 @@ static double inv_sqrt(double v) { return (v > 0.0 && isfinite(v)) ? 1.0 / sqrt(
 static double fmax_r(double a, double b) { return a > b ? a : b; } /* f64::max for non-NaN */
 static double fmin_r(double a, double b) { return a < b ? a : b; }
 
-/* min_max (scaling.rs:159-178) */
-static void min_max(const or_csr *a, double *rmin, double *rmax, double *cmin, double *cmax) {
+/* Column-major positions of a's nonzeros, each column's in row order (the
+ * order of the counting sort, problem.rs:88-109). A column fold that walks
+ * them visits its terms in the order the serial row-major loop of the
+ * reference adds them, so the parallel passes below give the serial bits. */
+typedef struct {
+    int64_t *ptr, *pos;
+} colmap;
+
+static int colmap_build(const or_csr *a, colmap *cm) {
+    cm->ptr = calloc((size_t)(a->ncols + 1), sizeof(int64_t));
+    cm->pos = malloc(sizeof(int64_t) * (size_t)(a->nnz ? a->nnz : 1));
+    int64_t *next = malloc(sizeof(int64_t) * (size_t)(a->ncols + 1));
+    if (!cm->ptr || !cm->pos || !next) {
+        free(cm->ptr), free(cm->pos), free(next);
+        cm->ptr = cm->pos = NULL;
+        return 0;
+    }
+    for (int64_t k = 0; k < a->nnz; ++k) cm->ptr[a->col_idx[k] + 1] += 1;
+    for (int64_t j = 0; j < a->ncols; ++j) cm->ptr[j + 1] += cm->ptr[j];
+    memcpy(next, cm->ptr, sizeof(int64_t) * (size_t)(a->ncols + 1));
+    for (int64_t k = 0; k < a->nnz; ++k) cm->pos[next[a->col_idx[k]]++] = k;
+    free(next);
+    return 1;
+}
+
+static void colmap_free(colmap *cm) {
+    free(cm->ptr), free(cm->pos);
+}
+
+/* min_max (scaling.rs:159-178). With a column map (large matrices, several
+ * threads) rows and columns are folded in parallel, each in the serial order. */
+static void min_max(const or_csr *a, const colmap *cm, double *rmin, double *rmax, double *cmin, double *cmax) {
+    if (cm) {
+#pragma omp parallel for schedule(dynamic, 1024) num_threads(g_threads)
+        for (int64_t i = 0; i < a->nrows; ++i) {
+            double lo = INFINITY, hi = 0.0;
+            for (int64_t k = a->row_ptr[i]; k < a->row_ptr[i + 1]; ++k) {
+                double v = fabs(a->values[k]);
+                lo = fmin_r(lo, v);
+                hi = fmax_r(hi, v);
+            }
+            rmin[i] = hi == 0.0 ? 0.0 : lo;
+            rmax[i] = hi;
+        }
+#pragma omp parallel for schedule(dynamic, 1024) num_threads(g_threads)
+        for (int64_t j = 0; j < a->ncols; ++j) {
+            double lo = INFINITY, hi = 0.0;
+            for (int64_t q = cm->ptr[j]; q < cm->ptr[j + 1]; ++q) {
+                double v = fabs(a->values[cm->pos[q]]);
+                lo = fmin_r(lo, v);
+                hi = fmax_r(hi, v);
+            }
+            cmin[j] = hi == 0.0 ? 0.0 : lo;
+            cmax[j] = hi;
+        }
+        return;
+    }
     for (int64_t i = 0; i < a->nrows; ++i) rmin[i] = INFINITY, rmax[i] = 0.0;
     for (int64_t j = 0; j < a->ncols; ++j) cmin[j] = INFINITY, cmax[j] = 0.0;
     for (int64_t i = 0; i < a->nrows; ++i) {
@@ -185,9 +242,30 @@ static void min_max(const or_csr *a, double *rmin, double *rmax, double *cmin, d
         if (cmax[j] == 0.0) cmin[j] = 0.0;
 }
 
-/* abs_stats (scaling.rs:144-155) with the three folds the code uses. */
+/* abs_stats (scaling.rs:144-155) with the three folds the code uses. A sum
+ * adds a line's terms left to right (a column's in row order), also in the
+ * parallel form. */
 enum { FOLD_MAX, FOLD_POW };
-static void abs_stats(const or_csr *a, int fold, double e, double *rows, double *cols) {
+static double fold_term(int fold, double e, double acc, double v) {
+    return fold == FOLD_MAX ? fmax_r(acc, v) : acc + pow(v, e);
+}
+
+static void abs_stats(const or_csr *a, const colmap *cm, int fold, double e, double *rows, double *cols) {
+    if (cm) {
+#pragma omp parallel for schedule(dynamic, 1024) num_threads(g_threads)
+        for (int64_t i = 0; i < a->nrows; ++i) {
+            double s = 0.0;
+            for (int64_t k = a->row_ptr[i]; k < a->row_ptr[i + 1]; ++k) s = fold_term(fold, e, s, fabs(a->values[k]));
+            rows[i] = s;
+        }
+#pragma omp parallel for schedule(dynamic, 1024) num_threads(g_threads)
+        for (int64_t j = 0; j < a->ncols; ++j) {
+            double s = 0.0;
+            for (int64_t q = cm->ptr[j]; q < cm->ptr[j + 1]; ++q) s = fold_term(fold, e, s, fabs(a->values[cm->pos[q]]));
+            cols[j] = s;
+        }
+        return;
+    }
     for (int64_t i = 0; i < a->nrows; ++i) rows[i] = 0.0;
     for (int64_t j = 0; j < a->ncols; ++j) cols[j] = 0.0;
     for (int64_t i = 0; i < a->nrows; ++i) {
@@ -259,26 +337,28 @@ or_scaled *or_scale(const or_lp *p, const or_scaling_params *params) {
     for (int64_t j = 0; j < n; ++j) s->col_scale[j] = 1.0, ones_n[j] = 1.0;
     double *rmin = malloc(mb), *rmax = malloc(mb), *cmin = malloc(nb), *cmax = malloc(nb);
     double *r = malloc(mb), *c = malloc(nb);
+    colmap cmap = {NULL, NULL};
+    const colmap *cm = par(s->a.nnz) && colmap_build(&s->a, &cmap) ? &cmap : NULL;
 
     for (int64_t pass = 0; pass < params->geometric_mean_passes; ++pass) {
-        min_max(&s->a, rmin, rmax, cmin, cmax);
+        min_max(&s->a, cm, rmin, rmax, cmin, cmax);
         for (int64_t i = 0; i < m; ++i) r[i] = inv_sqrt(rmin[i] * rmax[i]);
         apply(&s->a, s->row_scale, s->col_scale, r, ones_n);
-        min_max(&s->a, rmin, rmax, cmin, cmax);
+        min_max(&s->a, cm, rmin, rmax, cmin, cmax);
         for (int64_t j = 0; j < n; ++j) c[j] = inv_sqrt(cmin[j] * cmax[j]);
         apply(&s->a, s->row_scale, s->col_scale, ones_m, c);
     }
     for (int64_t it = 0; it < params->ruiz_iterations; ++it) {
-        abs_stats(&s->a, FOLD_MAX, 0.0, rmax, cmax);
+        abs_stats(&s->a, cm, FOLD_MAX, 0.0, rmax, cmax);
         for (int64_t i = 0; i < m; ++i) r[i] = inv_sqrt(rmax[i]);
         for (int64_t j = 0; j < n; ++j) c[j] = inv_sqrt(cmax[j]);
         apply(&s->a, s->row_scale, s->col_scale, r, c);
     }
     if (params->has_pock_chambolle) {
         double alpha = params->pock_chambolle_alpha;
-        abs_stats(&s->a, FOLD_POW, alpha, rmax, cmax); /* rows use |a|^α */
+        abs_stats(&s->a, cm, FOLD_POW, alpha, rmax, cmax); /* rows use |a|^α */
         for (int64_t i = 0; i < m; ++i) r[i] = inv_sqrt(rmax[i]);
-        abs_stats(&s->a, FOLD_POW, 2.0 - alpha, rmin, cmax); /* columns use |a|^(2-α) */
+        abs_stats(&s->a, cm, FOLD_POW, 2.0 - alpha, rmin, cmax); /* columns use |a|^(2-α) */
         for (int64_t j = 0; j < n; ++j) c[j] = inv_sqrt(cmax[j]);
         apply(&s->a, s->row_scale, s->col_scale, r, c);
     }
@@ -310,6 +390,7 @@ or_scaled *or_scale(const or_lp *p, const or_scaling_params *params) {
     }
     csr_transpose_alloc(&s->a, &s->at);
     compute_factors(s);
+    colmap_free(&cmap);
     free(ones_m), free(ones_n), free(rmin), free(rmax), free(cmin), free(cmax), free(r), free(c);
     return s;
 }

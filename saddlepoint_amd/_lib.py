@@ -141,6 +141,9 @@ SIGNATURES = {
     "sp_backend_profile_read": (C.c_int, [_vp, C.c_int, C.c_char_p, C.c_int, _pd, C.POINTER(_i64)]),
     "sp_backend_sync": (C.c_int, [_vp]),
     "sp_probe": (C.c_int, [_vp, C.c_int, _i64, _i64, C.c_int, _pd]),
+    "sp_gather_replay_time": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _pd]),
+    "sp_solver_kernel_time": (C.c_int, [_vp, C.c_int, C.c_int, _pd]),
+    "sp_solver_problem": (_vp, [_vp]),
     "sp_spmv_tuning_default": (None, [C.c_void_p]),
     "sp_spmv_tuning_for": (None, [C.POINTER(_i64), _i64, C.c_void_p]),
     "sp_problem_load_tuned": (C.c_int, [_vp, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(_vp)]),

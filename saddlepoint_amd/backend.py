@@ -167,6 +167,14 @@ class Problem:
         self.free()
 
 
+class _BorrowedProblem:
+    """A problem handle owned by someone else (a solver): never freed here."""
+
+    def __init__(self, handle, m: int, n: int):
+        self.h = handle
+        self.m, self.n = m, n
+
+
 class HipBackend:
     """``impl Backend`` for one MI355X device (backend.rs:43-137)."""
 
@@ -274,11 +282,21 @@ class HipBackend:
     def sync(self) -> None:
         self._ok(lib().sp_backend_sync(self.h))
 
+    PROBES = {"copy": 0, "triad": 1, "gather": 2, "copy_f32": 3, "triad_f32": 4}
+
     def probe(self, kind: str, n: int, table_len: int = 0, reps: int = 20) -> float:
-        """Milliseconds per launch of a bandwidth probe: "copy", "triad" or "gather"."""
+        """Milliseconds per launch of a bandwidth probe: "copy", "triad" (f64),
+        "copy_f32", "triad_f32", or "gather" (random 8-byte gathers)."""
         ms = C.c_double()
-        k = {"copy": 0, "triad": 1, "gather": 2}[kind]
-        self._ok(lib().sp_probe(self.h, k, int(n), int(table_len), int(reps), C.byref(ms)))
+        self._ok(lib().sp_probe(self.h, self.PROBES[kind], int(n), int(table_len), int(reps), C.byref(ms)))
+        return ms.value
+
+    def gather_replay_time(self, p: Problem, transpose: bool = False, fp32: bool = False, reps: int = 20) -> float:
+        """Milliseconds per launch of the gathers alone of a loaded matrix's
+        SpMV (its own column indices, in the SpMV's order and tiling)."""
+        ms = C.c_double()
+        self._ok(lib().sp_gather_replay_time(self.h, p.h, int(bool(transpose)), int(bool(fp32)), int(reps),
+                                             C.byref(ms)))
         return ms.value
 
     def distance_sq(self, a: Vector, b: Vector) -> float:
@@ -416,6 +434,17 @@ class Solver:
                            sol.dual_residual, sol.gap, int(sol.iterations), int(sol.restarts), sol.solve_time_s,
                            sol.total_time_s, sol.setup_time_s, sol.final_omega,
                        int(sol.mixed_fp64_iterations))
+
+    def problem(self) -> "_BorrowedProblem":
+        """The solver's loaded problem (owned by the solver)."""
+        return _BorrowedProblem(lib().sp_solver_problem(self.h), self.p.nrows, self.p.ncols)
+
+    def kernel_time(self, side: int, reps: int = 20) -> float:
+        """Milliseconds per launch of the plain iteration's main kernel of
+        side 0 (A'ᵀ) or 1 (A'), over back-to-back launches on copies."""
+        ms = C.c_double()
+        self.b._ok(lib().sp_solver_kernel_time(self.h, int(side), int(reps), C.byref(ms)))
+        return ms.value
 
     def close(self):
         if getattr(self, "h", None):

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <memory>
 #include <climits>
+#include <unordered_map>
 #include <cstring>
 
 namespace sp {
@@ -20,6 +21,25 @@ void hip_check(hipError_t e, const char* what) {
         throw Error(SP_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
     }
 }
+
+namespace {
+thread_local std::unordered_map<const sp_backend*, std::string> t_errors;
+}  // namespace
+
+void set_error(const sp_backend* b, const std::string& msg) {
+    if (msg.empty()) {
+        t_errors.erase(b);
+        return;
+    }
+    t_errors[b] = msg;
+}
+
+const char* last_error(const sp_backend* b) {
+    auto it = t_errors.find(b);
+    return it == t_errors.end() ? "" : it->second.c_str();
+}
+
+void forget_errors(const sp_backend* b) { t_errors.erase(b); }
 
 namespace {
 
@@ -649,6 +669,44 @@ void Hip::plain_iteration_mixed(const sp_problem* p, sp_vec32* dx, sp_vec32* dy,
 
 double Hip::probe(int kind, int64_t n, int64_t table_len, int reps, uint64_t seed) {
     if (n <= 0 || reps <= 0 || (kind == 2 && table_len <= 0)) throw Error(SP_ERR_INVALID, "probe: bad size");
+    if (kind == 3 || kind == 4) {
+        // The float copy and triad run on the doubles' buffers, reinterpreted.
+        const int64_t half = (n + 1) / 2;
+        double *a = nullptr, *bb = nullptr, *c = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        float ms = 0.0f;
+        hipError_t err = hipSuccess;
+        try {
+            a = dmalloc<double>((size_t)half), bb = dmalloc<double>((size_t)half), c = dmalloc<double>((size_t)half);
+            launch(b, b->k_fill, ew_grid(half), a, 1.0, (long long)half);
+            launch(b, b->k_fill, ew_grid(half), bb, 2.0, (long long)half);
+            const unsigned grid = (unsigned)std::min<int64_t>((n + kWG - 1) / kWG, 65536);
+            auto once = [&] {
+                if (kind == 3) launch(b, b->k_probe_copy32, grid, (const float*)a, (float*)c, (long long)n);
+                else launch(b, b->k_probe_triad32, grid, (const float*)a, (const float*)bb, (float*)c, 3.0f, (long long)n);
+            };
+            once();
+            hip_check(hipEventCreate(&e0), "hipEventCreate");
+            hip_check(hipEventCreate(&e1), "hipEventCreate");
+            err = hipEventRecord(e0, b->stream);
+            for (int r = 0; r < reps && err == hipSuccess; ++r) once();
+            if (err == hipSuccess) err = hipEventRecord(e1, b->stream);
+            if (err == hipSuccess) err = hipEventSynchronize(e1);
+            if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+        } catch (...) {
+            (void)hipStreamSynchronize(b->stream);
+            for (void* q : {(void*)a, (void*)bb, (void*)c}) dfree(q);
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipStreamSynchronize(b->stream);
+        for (void* q : {(void*)a, (void*)bb, (void*)c}) dfree(q);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        hip_check(err, "probe events");
+        return (double)ms / reps;
+    }
     double *a = nullptr, *bb = nullptr, *c = nullptr, *table = nullptr;
     int* idx = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -722,6 +780,126 @@ double Hip::spmv_time(const sp_problem* p, bool transpose, int reps) {
     (void)hipEventDestroy(e1);
     hip_check(err, "spmv_time events");
     return (double)ms / reps;
+}
+
+namespace {
+
+// Milliseconds of `reps` calls of `once` between two events on the stream.
+template <class F>
+double time_reps(sp_backend* b, int reps, F&& once) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms = 0.0f;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hipError_t err = hipEventCreate(&e1);
+    if (err == hipSuccess) err = hipEventRecord(e0, b->stream);
+    try {
+        for (int r = 0; r < reps && err == hipSuccess; ++r) once();
+    } catch (...) {
+        (void)hipStreamSynchronize(b->stream);
+        (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        throw;
+    }
+    if (err == hipSuccess) err = hipEventRecord(e1, b->stream);
+    if (err == hipSuccess) err = hipEventSynchronize(e1);
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    hip_check(err, "timing events");
+    return (double)ms;
+}
+
+}  // namespace
+
+double Hip::gather_replay_time(const sp_problem* p, bool transpose, bool fp32, int reps) {
+    if (!p || reps < 1) throw Error(SP_ERR_INVALID, "gather_replay_time: bad argument");
+    const DevCsr& M = transpose ? p->at : p->a;
+    std::unique_ptr<sp_vec, void (*)(sp_vec*)> table(zeros(std::max(M.ncols, 1)), free_vec);
+    double* sink = dmalloc<double>(1);
+    const long long nnz = (long long)M.nnz;
+    const unsigned grid = (unsigned)std::max<int64_t>(1, (M.nnz + kStreamNnzF64 - 1) / kStreamNnzF64);
+    hipFunction_t k = fp32 ? b->k_replay32 : b->k_replay64;
+    auto once = [&] {
+        if (nnz) launch(b, k, grid, (const int*)M.ci, nnz, (const void*)table->d, sink);
+    };
+    double ms = 0.0;
+    try {
+        once();  // warm-up
+        ms = time_reps(b, reps, once) / reps;
+    } catch (...) {
+        dfree(sink);
+        throw;
+    }
+    (void)hipStreamSynchronize(b->stream);
+    dfree(sink);
+    return ms;
+}
+
+double Hip::plain_kernel_time(const sp_problem* p, int side, bool mixed, const PlainArgs& a, int reps) {
+    if (!p || reps < 1 || side < 0 || side > 1) throw Error(SP_ERR_INVALID, "plain_kernel_time: bad argument");
+    if (mixed && !p->has_fp32) throw Error(SP_ERR_INVALID, "plain_kernel_time: FP32 values not prepared");
+    const DevCsr& M = side == 0 ? p->at : p->a;
+    if (!M.nblocks) return 0.0;
+    // Copies of what the kernel writes: the iterate it updates in place (and
+    // x̄ on the A'ᵀ side).
+    std::vector<void*> scratch;
+    auto copy_of = [&](const void* src, size_t bytes) {
+        void* d = dmalloc<char>(std::max<size_t>(bytes, 1));
+        scratch.push_back(d);
+        if (bytes) hip_check(hipMemcpyAsync(d, src, bytes, hipMemcpyDeviceToDevice, b->stream), "hipMemcpyAsync D2D");
+        return d;
+    };
+    const size_t n = (size_t)p->n, m = (size_t)p->m;
+    double ms = 0.0;
+    try {
+        if (!mixed && side == 0) {
+            double* x = (double*)copy_of(a.x->d, 8 * n);
+            double* xb = (double*)copy_of(a.x_bar->d, 8 * n);
+            auto once = [&] {
+                launch(b, b->k_spmv_t_primal_halpern, (unsigned)M.nblocks, M.desc, M.rp, M.ci, (const void*)M.va,
+                       (const void*)a.y->d, (void*)M.partials, x, (const double*)p->c, (const double*)p->lv,
+                       (const double*)p->uv, (const double*)a.x_anchor->d, a.tau, a.w, a.rho, xb);
+            };
+            once();
+            ms = time_reps(b, reps, once) / reps;
+        } else if (!mixed) {
+            double* y = (double*)copy_of(a.y->d, 8 * m);
+            auto once = [&] {
+                launch(b, b->k_spmv_dual_halpern, (unsigned)M.nblocks, M.desc, M.rp, M.ci, (const void*)M.va,
+                       (const void*)a.x_bar->d, (void*)M.partials, y, (const double*)p->lc, (const double*)p->uc,
+                       (const double*)a.y_anchor->d, a.sigma, a.w, a.rho);
+            };
+            once();
+            ms = time_reps(b, reps, once) / reps;
+        } else if (side == 0) {
+            float* dx = (float*)copy_of(a.dx->d, 4 * n);
+            float* dxb = (float*)copy_of(a.dx_bar->d, 4 * n);
+            auto once = [&] {
+                launch(b, b->k_spmv_t_primal_mixed, (unsigned)M.nblocks, M.desc, M.rp, M.ci, (const void*)M.va32,
+                       (const void*)a.dy->d, (void*)M.partials, dx, (const double*)a.x->d, (const double*)p->c,
+                       (const double*)p->lv, (const double*)p->uv, (const double*)a.atya->d, (const float*)a.ex->d,
+                       a.tau, a.w, a.rho, dxb);
+            };
+            once();
+            ms = time_reps(b, reps, once) / reps;
+        } else {
+            float* dy = (float*)copy_of(a.dy->d, 4 * m);
+            auto once = [&] {
+                launch(b, b->k_spmv_dual_mixed, (unsigned)M.nblocks, M.desc, M.rp, M.ci, (const void*)M.va32,
+                       (const void*)a.dx_bar->d, (void*)M.partials, dy, (const double*)a.y->d, (const double*)p->lc,
+                       (const double*)p->uc, (const double*)a.axa->d, (const float*)a.ey->d, a.sigma, a.w, a.rho);
+            };
+            once();
+            ms = time_reps(b, reps, once) / reps;
+        }
+    } catch (...) {
+        (void)hipStreamSynchronize(b->stream);
+        for (void* q : scratch) dfree(q);
+        throw;
+    }
+    (void)hipStreamSynchronize(b->stream);
+    for (void* q : scratch) dfree(q);
+    return ms;
 }
 
 // ---- SpMV --------------------------------------------------------------------
@@ -1089,6 +1267,10 @@ sp_backend* sp_backend_open(int ordinal) {
         get(b->k_probe_copy, "sp_probe_copy");
         get(b->k_probe_triad, "sp_probe_triad");
         get(b->k_probe_gather, "sp_probe_gather");
+        get(b->k_probe_copy32, "sp_probe_copy_f32");
+        get(b->k_probe_triad32, "sp_probe_triad_f32");
+        get(b->k_replay64, "sp_probe_replay_f64");
+        get(b->k_replay32, "sp_probe_replay_f32");
         get(b->k_spmv_t_primal_mixed, "sp_spmv_t_primal_mixed");
         get(b->k_spmv_dual_mixed, "sp_spmv_dual_mixed");
         get(b->k_to_f32, "sp_f64_to_f32");
@@ -1120,6 +1302,7 @@ sp_backend* sp_backend_open(int ordinal) {
 
 void sp_backend_close(sp_backend* b) {
     if (!b) return;
+    sp::forget_errors(b);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     for (const auto& pe : b->pending) b->event_pool.push_back(pe.start), b->event_pool.push_back(pe.stop);
     for (hipEvent_t e : b->event_pool) (void)hipEventDestroy(e);

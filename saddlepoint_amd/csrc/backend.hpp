@@ -92,7 +92,6 @@ struct sp_backend {
     hipModule_t module = nullptr;
     int cus = 0;
     std::string name;
-    std::string err;
     // Reduction scratch: per-workgroup partials, the result slots and their
     // pinned host mirror.
     double* red_partials = nullptr;
@@ -130,7 +129,8 @@ struct sp_backend {
         k_remap, k_col_count, k_scale_inv_sqrt;
     hipFunction_t k_spmv, k_spmv_t_primal, k_spmv_t_primal_halpern, k_spmv_dual, k_spmv_dual_halpern;
     hipFunction_t k_primal_step, k_dual_step, k_halpern, k_sub, k_scale, k_fill;
-    hipFunction_t k_probe_copy, k_probe_triad, k_probe_gather;
+    hipFunction_t k_probe_copy, k_probe_triad, k_probe_gather, k_probe_copy32, k_probe_triad32, k_replay64,
+        k_replay32;
     hipFunction_t k_spmv32, k_spmv_t_primal_mixed, k_spmv_dual_mixed, k_to_f32, k_delta_from, k_delta_to;
     hipFunction_t k_dot, k_dist, k_primal_stats, k_dual_stats, k_step_dual, k_primal_ray, k_dual_ray;
 };
@@ -139,6 +139,13 @@ namespace sp {
 
 // Checks a HIP status and throws sp::Error(SP_ERR_HIP) with the call site.
 void hip_check(hipError_t e, const char* what);
+
+// The message of the calling thread's last failed call on backend `b` (""
+// after a call that succeeded). Kept per thread, so a call that fails on one
+// thread (a serving submit) never touches the text another thread reads.
+void set_error(const sp_backend* b, const std::string& msg);
+const char* last_error(const sp_backend* b);
+void forget_errors(const sp_backend* b);
 
 // The Backend methods. Each throws sp::Error on failure.
 struct Hip {
@@ -222,8 +229,28 @@ struct Hip {
 
     // Average milliseconds of one probe kernel over `reps` launches: kind 0
     // copies n doubles, 1 is a triad over n doubles, 2 gathers n doubles at
-    // pseudo-random positions of a table of table_len doubles.
+    // pseudo-random positions of a table of table_len doubles, 3 and 4 are
+    // the copy and the triad over n floats.
     double probe(int kind, int64_t n, int64_t table_len, int reps, uint64_t seed);
+    // Average milliseconds of one launch of the gather-only replay of a
+    // loaded matrix's column indices (A' or, with transpose, A'ᵀ): every
+    // nonzero's entry of a table as long as the matrix has columns is loaded,
+    // in the SpMV's order and tiling, and nothing else is read or written.
+    // fp32: the table holds floats (the mixed-precision iterates).
+    double gather_replay_time(const sp_problem* p, bool transpose, bool fp32, int reps);
+
+    // The operands of the plain iteration's two fused kernels.
+    struct PlainArgs {
+        const sp_vec *x, *y, *x_anchor, *y_anchor, *x_bar;
+        const sp_vec32 *dx, *dy, *dx_bar, *ex, *ey;  // mixed precision
+        const sp_vec *atya, *axa;
+        double tau, sigma, w, rho;
+    };
+    // Average milliseconds of one launch of a plain iteration's main fused
+    // kernel (side 0: the A'ᵀ side, 1: the A' side; mixed: the FP32 forms),
+    // from two events around `reps` back-to-back launches. The kernel writes
+    // copies of the vectors it updates, so the operands are left unchanged.
+    double plain_kernel_time(const sp_problem* p, int side, bool mixed, const PlainArgs& a, int reps);
 
     // Reductions into device result slots, read back together by fetch().
     double* slot(int i);

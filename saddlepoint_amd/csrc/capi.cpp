@@ -21,16 +21,16 @@ int guarded(sp_backend* b, F&& f) {
     if (!b) return SP_ERR_INVALID;
     try {
         f();
-        b->err.clear();
+        sp::set_error(b, "");
         return SP_OK;
     } catch (const sp::Error& e) {
-        b->err = e.what();
+        sp::set_error(b, e.what());
         return e.code;
     } catch (const std::bad_alloc&) {
-        b->err = "host out of memory";
+        sp::set_error(b, "host out of memory");
         return SP_ERR_MEMORY;
     } catch (const std::exception& e) {
-        b->err = e.what();
+        sp::set_error(b, e.what());
         return SP_ERR_INVALID;
     }
 }
@@ -98,7 +98,7 @@ void sp_backend_destroy(sp_backend* b) { sp_backend_close(b); }
 
 const char* sp_backend_name(const sp_backend* b) { return b ? b->name.c_str() : ""; }
 
-const char* sp_last_error(const sp_backend* b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+const char* sp_last_error(const sp_backend* b) { return b ? sp::last_error(b) : g_create_error.c_str(); }
 
 int sp_problem_load(sp_backend* b, const sp_scaled_view* p, sp_problem** out) {
     return guarded(b, [&] {
@@ -367,8 +367,26 @@ int sp_backend_profile_read(sp_backend* b, int index, char* name, int name_len, 
 
 int sp_probe(sp_backend* b, int kind, int64_t n, int64_t table_len, int reps, double* ms_per_launch) {
     return guarded(b, [&] {
-        if (!ms_per_launch || kind < 0 || kind > 2) throw sp::Error(SP_ERR_INVALID, "probe: bad argument");
+        if (!ms_per_launch || kind < 0 || kind > 4) throw sp::Error(SP_ERR_INVALID, "probe: bad argument");
         *ms_per_launch = sp::Hip{b}.probe(kind, n, table_len, reps, 1);
+    });
+}
+
+int sp_gather_replay_time(sp_backend* b, const sp_problem* p, int transpose, int fp32, int reps, double* ms) {
+    return guarded(b, [&] {
+        check_problem(b, p);
+        if (!ms) throw sp::Error(SP_ERR_INVALID, "gather_replay_time: null output");
+        *ms = sp::Hip{b}.gather_replay_time(p, transpose != 0, fp32 != 0, reps);
+    });
+}
+
+const sp_problem* sp_solver_problem(const sp_solver* s) { return s ? s->solver.problem() : nullptr; }
+
+int sp_solver_kernel_time(sp_solver* s, int side, int reps, double* ms) {
+    if (!s) return SP_ERR_INVALID;
+    return guarded(s->b, [&] {
+        if (!ms) throw sp::Error(SP_ERR_INVALID, "solver_kernel_time: null output");
+        *ms = s->solver.plain_kernel_time(side, reps);
     });
 }
 

@@ -624,10 +624,12 @@ extern "C" __global__ __launch_bounds__(WG) void sp_delta_to(const float* __rest
 // ---- fused reductions (cpu/src/lib.rs:189-259) ---------------------------------
 //
 // Each kernel runs over a combined index range (rows, then columns when a
-// method reduces both), accumulates K sums per thread in a grid-stride loop,
-// reduces them per workgroup in a fixed order, and the last workgroup to
-// finish adds the per-workgroup partials in workgroup order and writes the K
-// totals to `result`. One launch and one 8·K-byte readback per method.
+// method reduces both), accumulates K sums per thread in a grid-stride loop
+// and reduces them per workgroup in a fixed order. A one-workgroup launch
+// writes its K totals to `result`; otherwise each workgroup writes its K
+// partials and the separate finishing kernel sp_red_finish adds them in
+// workgroup order (no fence or atomic: the kernel boundary orders them).
+// Two launches at most, and one 8·K-byte readback per method.
 
 // Workgroup b of a G-workgroup reduction over [0, n).
 template <int K, class F>
@@ -1162,6 +1164,57 @@ extern "C" __global__ __launch_bounds__(WG) void sp_probe_gather(const int* __re
             if (i < n) out[i] = v[u];
         }
     }
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_probe_copy_f32(const float* __restrict__ a, float* __restrict__ b,
+                                                                   i64 n) {
+    GRID_LOOP(i, n) { b[i] = __builtin_nontemporal_load(a + i); }
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_probe_triad_f32(const float* __restrict__ a,
+                                                                    const float* __restrict__ b, float* __restrict__ c,
+                                                                    float s, i64 n) {
+    GRID_LOOP(i, n) { c[i] = __builtin_nontemporal_load(a + i) + s * __builtin_nontemporal_load(b + i); }
+}
+
+// The gathers of an SpMV and nothing else: workgroup b takes nonzeros
+// [b·SP_NNZ_F64, (b+1)·SP_NNZ_F64) of a matrix's column indices with the
+// stream blocks' straight-line loads (the same lanes, the same order, all
+// in flight) and loads table[col] for each. Values, row pointers and the
+// epilogue are left out, so its rate is the ceiling that this matrix's
+// access pattern puts on the SpMV's gathers. The sum is stored only if it
+// is NaN, which a table of finite values never gives.
+template <class T>
+__device__ __forceinline__ void replay_body(const int* __restrict__ ci, i64 nnz, const T* __restrict__ table,
+                                            double* __restrict__ sink) {
+    constexpr int PER = SP_NNZ_F64 / WG;
+    const i64 k0 = (i64)blockIdx.x * SP_NNZ_F64;
+    const int n = (int)min((i64)SP_NNZ_F64, nnz - k0);
+    const int last = n - 1;
+    const int tid = threadIdx.x;
+    int c[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) c[u] = __builtin_nontemporal_load(ci + k0 + min(tid + u * WG, last));
+    T v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) v[u] = table[c[u]];
+    double s = 0.0;
+#pragma unroll
+    for (int u = 0; u < PER; ++u)
+        if (tid + u * WG < n) s += (double)v[u];
+    if (s != s) sink[0] = s;
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_probe_replay_f64(const int* __restrict__ ci, i64 nnz,
+                                                                     const double* __restrict__ table,
+                                                                     double* __restrict__ sink) {
+    replay_body<double>(ci, nnz, table, sink);
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_probe_replay_f32(const int* __restrict__ ci, i64 nnz,
+                                                                     const float* __restrict__ table,
+                                                                     double* __restrict__ sink) {
+    replay_body<float>(ci, nnz, table, sink);
 }
 
 // ---- batched plain iterations for the serving loop --------------------------------

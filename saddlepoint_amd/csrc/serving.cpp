@@ -192,7 +192,10 @@ class Server {
         // iteration limit is rounded up to a whole number of chunks.
         if (params.check_every != kChunk) throw Error(SP_ERR_INVALID, "serving needs check_every = 64");
         job->params.fused = 1;
-        job->params.max_iterations = (params.max_iterations + kChunk - 1) / kChunk * kChunk;
+        // Limits past the largest multiple of 64 (u64::MAX, "no limit") become it.
+        const uint64_t last = UINT64_MAX / kChunk * kChunk;
+        job->params.max_iterations =
+            params.max_iterations > last ? last : (params.max_iterations + kChunk - 1) / kChunk * kChunk;
         job->footprint = footprint(job->m, job->n, nnz, params.precision == 1);
         job->cost = nnz + job->m + job->n;
         job->t_submit = clock();
@@ -802,16 +805,16 @@ int guarded_server(sp_server* sv, F&& f) {
     if (!sv) return SP_ERR_INVALID;
     try {
         f();
-        sv->b->err.clear();
+        sp::set_error(sv->b, "");
         return SP_OK;
     } catch (const sp::Error& e) {
-        sv->b->err = e.what();
+        sp::set_error(sv->b, e.what());
         return e.code;
     } catch (const std::bad_alloc&) {
-        sv->b->err = "host out of memory";
+        sp::set_error(sv->b, "host out of memory");
         return SP_ERR_MEMORY;
     } catch (const std::exception& e) {
-        sv->b->err = e.what();
+        sp::set_error(sv->b, e.what());
         return SP_ERR_INVALID;
     }
 }
@@ -840,7 +843,7 @@ int sp_server_create(sp_backend* b, const sp_server_params* params, sp_server** 
         *out = sv;
         return SP_OK;
     } catch (const std::exception& e) {
-        b->err = e.what();
+        sp::set_error(b, e.what());
         return SP_ERR_INVALID;
     }
 }

@@ -311,6 +311,8 @@ class Solver {
     }
 
     uint64_t iterations() const { return total_; }
+    // The loaded (scaled, relabeled) problem.
+    const sp_problem* problem() const { return dev_.get(); }
     bool terminated() const { return terminated_; }
 
     // ---- for a caller that runs the plain iterations itself (the serving loop) ----
@@ -498,6 +500,25 @@ class Solver {
                           eta_ * omega_,
                           p_.reflection,
                           k_};
+    }
+
+    // Average milliseconds of one launch of the plain iteration's main kernel
+    // of `side` (0: A'ᵀ, 1: A'), launched `reps` times back to back on copies
+    // of what it writes (measurement only: the solver's state is unchanged).
+    double plain_kernel_time(int side, int reps) {
+        if (!p_.fused) throw Error(SP_ERR_INVALID, "plain_kernel_time: needs the fused path");
+        const bool mixed = mixed_now();
+        if (mixed) enter_plain();
+        typename B::PlainArgs a{};
+        a.x = x_.get(), a.y = y_.get(), a.x_anchor = x_anchor_.get(), a.y_anchor = y_anchor_.get();
+        a.x_bar = x_bar_.get();
+        if (mixed) {
+            a.dx = dx32_.get(), a.dy = dy32_.get(), a.dx_bar = dx_bar32_.get(), a.ex = ex32_.get();
+            a.ey = ey32_.get(), a.atya = atya_.get(), a.axa = axa_.get();
+        }
+        a.tau = eta_ / omega_, a.sigma = eta_ * omega_, a.w = (double)(k_ + 1) / (double)(k_ + 2);
+        a.rho = p_.reflection;
+        return b_.plain_kernel_time(dev_.get(), side, mixed, a, reps);
     }
 
     // Accounts for `steps` plain iterations that the caller ran.

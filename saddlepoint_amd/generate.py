@@ -219,3 +219,42 @@ def kat_lp(seed: int, m: int = 300, n: int = 500, per_col: int = 4):
     c = A.T @ y + z
     p = LpProblem(a=a, c=c, lc=lc, uc=uc, lv=lv, uv=uv, name=f"kat-{seed}")
     return p, float(c @ x), x, y
+
+
+def unbounded(p: LpProblem, pairs: int = 3, seed: int = 7) -> LpProblem:
+    """``p`` made dual infeasible (primal unbounded): `pairs` pairs of new
+    columns (u, v), each a copy of a random existing column of A, with
+    u ∈ [0, ∞), v ∈ (-∞, 0] and c_u < c_v. Along d = e_u - e_v the rows do
+    not move (A d = 0), both bounds hold for every step, and cᵀd = c_u - c_v
+    < 0, so a feasible ``p`` stays feasible and becomes unbounded below. The
+    new columns are not empty, so the ray runs through the rows' coupling."""
+    rng = np.random.default_rng(seed)
+    a = p.a
+    m, n = a.nrows, a.ncols
+    rows = np.repeat(np.arange(m, dtype=np.int64), np.diff(a.row_ptr))
+    r_all, c_all, v_all = [rows], [a.col_idx.astype(np.int64)], [a.values]
+    c, lv, uv = list(p.c), list(p.lv), list(p.uv)
+    nonempty = np.unique(a.col_idx)
+    for q in range(pairs):
+        src = int(nonempty[rng.integers(0, nonempty.size)])
+        k = np.flatnonzero(a.col_idx == src)
+        for j in (n + 2 * q, n + 2 * q + 1):
+            r_all.append(rows[k])
+            c_all.append(np.full(k.size, j, dtype=np.int64))
+            v_all.append(a.values[k])
+        cu = float(rng.uniform(-1.0, 1.0))
+        c += [cu, cu + float(rng.uniform(0.5, 1.5))]
+        lv += [0.0, -INF]
+        uv += [INF, 0.0]
+    a2 = CsrMatrix.from_triplets(m, n + 2 * pairs, np.concatenate(r_all), np.concatenate(c_all),
+                                 np.concatenate(v_all))
+    return LpProblem(a=a2, c=c, lc=p.lc, uc=p.uc, lv=lv, uv=uv, c0=p.c0, name=f"{p.name}+unbounded")
+
+
+def maximized(p: LpProblem) -> LpProblem:
+    """The maximization whose optimum is minus ``p``'s: maximize (-c)ᵀx - c0.
+    ``LpProblem`` keeps c and c0 negated for a max problem (as the MPS reader
+    stores them), so the arrays are ``p``'s own and only the sense changes;
+    the solver reports the objective with the sign flipped back."""
+    return LpProblem(a=p.a, c=p.c, lc=p.lc, uc=p.uc, lv=p.lv, uv=p.uv, c0=p.c0, maximize=True,
+                     name=f"{p.name}-max")

@@ -61,6 +61,27 @@ def test_host_scaling_is_bitwise_the_oracle(name):
     assert np.array_equal(g.dual_residual_factors(), o["dual_factor"])
 
 
+@pytest.mark.parametrize("alpha", [1.0, 0.6])
+def test_oracle_scaling_threads_change_nothing(alpha):
+    """Above the parallel threshold the oracle folds rows and columns on
+    several threads, each line in the serial order: the scaled problem is
+    bitwise the one-thread result, and both are bitwise the host library's."""
+    p = generate.config1(scale=0.05)
+    assert p.a.nnz > 32768
+    sc = O.default_scaling(pock_chambolle_alpha=alpha)
+    O.set_threads(1)
+    one = O.ScaledProblem.of(p, sc).arrays()
+    O.set_threads(8)
+    many = O.ScaledProblem.of(p, sc).arrays()
+    O.set_threads(1)
+    for k in ("c", "lc", "uc", "lv", "uv", "row_scale", "col_scale"):
+        assert np.array_equal(one[k], many[k]), k
+    assert np.array_equal(one["a"].values, many["a"].values)
+    assert np.array_equal(one["at"].values, many["at"].values)
+    g = ScaledProblem.new(p, default_params(pock_chambolle_alpha=alpha))
+    assert np.array_equal(g.a.values, many["a"].values) and np.array_equal(g.col_scale, many["col_scale"])
+
+
 def test_host_scaling_config0_and_options():
     p = generate.config0()
     for kw in ({}, {"bound_objective_rescaling": 0}, {"has_pock_chambolle": 0, "ruiz_iterations": 3}):

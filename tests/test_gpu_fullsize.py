@@ -4,6 +4,9 @@
   oracle's, both capped at 12,800 iterations (200 checks, 11 restarts), end
   with the same status, iteration and restart counts, and iterates that agree
   far inside the solver's tolerance;
+* config 2 at full size (4M × 10M, ~100M nonzeros, Zipf(1.5) row lengths):
+  the same comparison capped at 1,280 iterations (20 checks, restarts), and the
+  mixed-precision solve at the same cap beside FP64 (objective gap and KKT);
 * SpMV at config 1 and at a quarter of config 2 (25M nonzeros, rows of up to
   ~25k nonzeros): adjointness ⟨A'x, y⟩ = ⟨x, A'ᵀy⟩, agreement with scipy's
   serial CSR product (each row summed left to right, as the CPU backend
@@ -53,6 +56,52 @@ def test_config1_capped_solve_matches_the_oracle(gpu, config1):
     assert g.iterations == c.iterations == CAP and g.restarts == c.restarts
     assert dx < 1e-8 and dy < 1e-8
     assert abs(g.objective - c.objective) <= 1e-9 * (1 + abs(c.objective))
+
+
+CAP2 = 1280
+
+
+def test_config2_capped_solve_matches_the_oracle(gpu):
+    """Config 2 at full size against the oracle (crates/saddlepoint-cuda/tests/
+    netlib.rs:31-50 at this size): same status, iteration and restart counts,
+    iterates far inside the tolerance. Mixed precision at the same cap is
+    reported beside FP64: its objective gap and relative KKT error at the
+    same iterate count."""
+    import time
+
+    t = time.perf_counter()
+    p = generate.config2()
+    print(f"config2: m={p.nrows} n={p.ncols} nnz={p.a.nnz} generated in {time.perf_counter() - t:.1f}s", flush=True)
+    threads = min(os.cpu_count() or 1, 16)
+    O.set_threads(threads)
+    t = time.perf_counter()
+    c, cx, cy = O.solve(p, O.default_params(max_iterations=CAP2, log_every_checks=1))
+    t_oracle = time.perf_counter() - t
+    t = time.perf_counter()
+    g = solve(gpu, p, default_params(max_iterations=CAP2))
+    t_gpu = time.perf_counter() - t
+    mx = solve(gpu, p, default_params(max_iterations=CAP2, precision=1))
+    dx = np.max(np.abs(g.x - cx)) / (1 + np.max(np.abs(cx)))
+    dy = np.max(np.abs(g.y - cy)) / (1 + np.max(np.abs(cy)))
+    gap = abs(mx.objective - g.objective) / (1 + abs(g.objective))
+    bn, cn = O.finite_bound_norm(p.lc, p.uc), O.norm2(p.c)
+
+    def rel(r):  # the relative KKT error, as the termination test measures it (solver.rs:142-152)
+        return max(r.primal_residual / (1 + bn), r.dual_residual / (1 + cn),
+                   r.gap / (1 + abs(r.objective) + abs(r.dual_objective)))
+
+    print(f"config2 capped at {CAP2}: gpu {g.status} {g.iterations} it {g.restarts} restarts {t_gpu:.2f}s; oracle "
+          f"{O.STATUS[c.status]} {c.iterations} it {c.restarts} restarts {t_oracle:.1f}s ({threads} threads); "
+          f"x diff {dx:.2e}, y diff {dy:.2e}; objective gap {abs(g.objective - c.objective) / (1 + abs(c.objective)):.2e}",
+          flush=True)
+    print(f"config2 mixed at {CAP2}: {mx.status} {mx.iterations} it {mx.restarts} restarts; objective gap vs FP64 "
+          f"{gap:.2e}; KKT (max of primal, dual, gap) {rel(mx):.3e} vs FP64 {rel(g):.3e}", flush=True)
+    assert g.status == O.STATUS[c.status] == "IterationLimit"
+    assert g.iterations == c.iterations == CAP2 and g.restarts == c.restarts
+    assert dx < 1e-8 and dy < 1e-8
+    assert abs(g.objective - c.objective) <= 1e-9 * (1 + abs(c.objective))
+    assert mx.status == "IterationLimit" and mx.iterations == CAP2
+    assert np.all(np.isfinite(mx.x)) and np.all(np.isfinite(mx.y))
 
 
 def spmv_properties(gpu, p, name):

@@ -130,3 +130,61 @@ def test_graphs_and_eager_launches_agree(gpu):
         s.close()
     for a, b in zip(*res):
         assert a.iterations == b.iterations and np.array_equal(a.x, b.x)
+
+
+def relative_measures(p, r):
+    """The solver's own KKT figures of a result as the checker's relative
+    measures (check/src/lib.rs:107-117)."""
+    b = O.finite_bound_norm(p.lc, p.uc)
+    c = O.norm2(p.c)
+    return (r.primal_residual / (1 + b), r.dual_residual / (1 + c),
+            r.gap / (1 + abs(r.objective) + abs(r.dual_objective)))
+
+
+def test_config4_large_lp_served_in_mixed_precision_with_autotuned_plans(gpu):
+    """Config 4's setting: a large LP of the stream (instance 1: 234k rows,
+    Zipf(1.5) row lengths, 9M nonzeros) served with small ones, every LP in
+    mixed precision with the autotuned SpMV plans. Each served LP is bitwise
+    its single solve under the same parameters, and the oracle's checker on
+    the served large solution gives the KKT measures the solver reported."""
+    large = generate.serving_stream(4, 2)[1]
+    assert 200_000 <= large.nrows <= 1_000_000 and large.a.nnz > 5_000_000
+    items = [("large", large), ("afiro", netlib("afiro"))]
+    items += [(f"small-{i}", generate.serving_lp(generate.seed_of(4, 5000 + i), m_lo=1000, m_hi=5000))
+              for i in range(3)]
+    params = default_params(tolerance=1e-6, precision=1, spmv_autotune=1, max_iterations=1280)
+    s = Server(gpu)
+    ids = [s.submit(p, params) for _, p in items]
+    s.drain(ids)
+    for (name, p), i in zip(items, ids):
+        r = s.result(i).result
+        g = solve(gpu, p, params)
+        print(f"{name:>8}: m={p.nrows} nnz={p.a.nnz} served {r.status} {r.iterations} it "
+              f"({r.mixed_fp64_iterations} FP64); single {g.status} {g.iterations} it", flush=True)
+        assert r.status == g.status and r.iterations == g.iterations and r.restarts == g.restarts
+        assert np.array_equal(r.x, g.x) and np.array_equal(r.y, g.y)
+        if name == "large":
+            k = O.check(p, r.x, r.y)
+            own = relative_measures(p, r)
+            theirs = (k.relative_primal, k.relative_dual, k.relative_gap)
+            print(f"   checker {theirs}; solver {own}", flush=True)
+            for a, b in zip(own, theirs):
+                assert abs(a - b) <= 1e-6 * max(a, b) + 1e-12, (own, theirs)
+        elif r.status == "Optimal":
+            O.check(p, r.x, r.y).verify(2e-6)
+    s.close()
+
+
+def test_no_iteration_limit_is_served_like_a_single_solve(gpu):
+    """max_iterations = UINT64_MAX (the natural "no limit") is rounded to the
+    largest multiple of 64, not past it: the served LP runs to its optimum
+    with its single solve's iterations."""
+    p = netlib("israel")
+    params = default_params(max_iterations=2 ** 64 - 1)
+    s = Server(gpu)
+    i = s.submit(p, params)
+    s.drain([i])
+    r = s.result(i).result
+    g = solve(gpu, p, params)
+    assert r.status == g.status == "Optimal" and r.iterations == g.iterations > 64
+    s.close()

@@ -350,3 +350,67 @@ def test_config0_matches_highs():
     O.check(p, x, y).verify(2e-6)
     assert abs(s.objective - ref["objective"]) / (1 + abs(ref["objective"])) < 1e-5
     assert ref["m"] == p.nrows and ref["n"] == p.ncols and ref["nnz"] == p.a.nnz
+
+
+# ---- unbounded and maximization instances (solver.rs:291-309, 424-430) ------------------
+
+def _highs(p):
+    """scipy's bundled HiGHS on the LP as stated (c and c0 as stored): (status, optimum)."""
+    from scipy.optimize import linprog
+
+    A = p.a.to_dense()
+    aub, bub, aeq, beq = [], [], [], []
+    for i in range(p.nrows):
+        if p.lc[i] == p.uc[i]:
+            aeq.append(A[i]), beq.append(p.lc[i])
+            continue
+        if p.uc[i] < INF:
+            aub.append(A[i]), bub.append(p.uc[i])
+        if p.lc[i] > -INF:
+            aub.append(-A[i]), bub.append(-p.lc[i])
+    bounds = list(zip([None if v == -INF else v for v in p.lv], [None if v == INF else v for v in p.uv]))
+    r = linprog(p.c, A_ub=np.array(aub) if aub else None, b_ub=bub or None, A_eq=np.array(aeq) if aeq else None,
+                b_eq=beq or None, bounds=bounds, method="highs")
+    return r.status, (r.fun + p.c0 if r.status == 0 else None)
+
+
+@pytest.mark.parametrize("name", ["afiro", "israel"])
+def test_unbounded_construction_is_dual_infeasible(name):
+    """generate.unbounded adds column pairs along which the objective falls
+    without bound: HiGHS calls it unbounded (status 3) and the oracle certifies
+    DualInfeasible, while the LP it started from is optimal for both."""
+    base = netlib(name)
+    p = generate.unbounded(base)
+    assert _highs(base)[0] == 0 and _highs(p)[0] == 3
+    s, _, _ = O.solve(p, tolerance=1e-6)
+    assert O.STATUS[s.status] == "DualInfeasible"
+
+
+def test_maximize_reports_the_maximum():
+    """A max problem (c and c0 stored negated) is solved as the min problem and
+    its objective reported with the sign flipped back (solver.rs:424-430):
+    the maximum is minus the minimum, HiGHS's maximum of the stated
+    objective, and the objective of the returned x (problem.rs:198-201)."""
+    base = netlib("afiro")
+    p = generate.maximized(base)
+    s, x, y = O.solve(p, tolerance=1e-8)
+    smin, _, _ = O.solve(base, tolerance=1e-8)
+    assert O.STATUS[s.status] == "Optimal"
+    assert s.objective == -smin.objective
+    hs, hopt = _highs(base)
+    assert hs == 0 and abs(s.objective + hopt) <= 1e-6 * (1 + abs(hopt))
+    assert abs(s.objective - p.objective(x)) <= 1e-6 * (1 + abs(s.objective))
+    O.check(p, x, y).verify(1e-6)
+
+
+def test_mps_sample_is_infeasible():
+    """The reader's OBJSENSE MAX sample (crates/saddlepoint-io/src/lib.rs:328-375)
+    has no feasible point (R4 caps X3 at 2, MYEQN then forces X2 ≤ -5, and the
+    range of LIM1 needs X1 + X2 ≥ 1.5 with X1 ≤ 4): HiGHS and the oracle agree."""
+    from saddlepoint_amd import mps
+    from test_cpu_abi import SAMPLE
+
+    p = mps.read_mps(SAMPLE)
+    assert p.maximize and _highs(p)[0] == 2
+    s, _, _ = O.solve(p, tolerance=1e-6)
+    assert O.STATUS[s.status] == "PrimalInfeasible"
