@@ -119,6 +119,19 @@ int sp_primal_stats(sp_backend *b, const sp_problem *p, const sp_vec *x, const s
                     sp_primal_stats_t *out);                                               /* :106 */
 int sp_dual_stats(sp_backend *b, const sp_problem *p, const sp_vec *y, const sp_vec *aty,
                   sp_dual_stats_t *out);                                                   /* :108 */
+/* No counterpart in the reference. sp_dual_stats for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same kernels and result, bitwise, but
+ * the per-workgroup partials and totals live in a static device array of
+ * 8,200 doubles (64 KiB) that each GPU holds from the library's load until
+ * the process ends, instead of the backend's buffers; the call allocates
+ * nothing. Its reductions use at most 1,024 workgroups × 3 sums, so every
+ * length fits; a launch shape needing more scratch is rejected with
+ * SP_ERR_TOO_LARGE. All calls on one GPU share that array: calls that overlap
+ * on one GPU (from several threads or streams) race. SP_ERR_UNAVAILABLE if
+ * the array's address could not be taken on the backend's device. */
+int sp_dual_stats_noalloc(sp_backend *b, const sp_problem *p, const sp_vec *y, const sp_vec *aty,
+                          sp_dual_stats_t *out);
 int sp_step_dual_residual_sq(sp_backend *b, const sp_problem *p, const sp_vec *x, const sp_vec *x_t,
                              const sp_vec *aty, const sp_vec *aty_t, double tau, double *out); /* :113-121 */
 int sp_primal_ray_stats(sp_backend *b, const sp_problem *p, const sp_vec *dx, const sp_vec *adx,

@@ -128,6 +128,7 @@ SIGNATURES = {
     "sp_scale": (C.c_int, [_vp, _vp, _d]),
     "sp_primal_stats": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(PrimalStats)]),
     "sp_dual_stats": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(DualStats)]),
+    "sp_dual_stats_noalloc": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(DualStats)]),
     "sp_step_dual_residual_sq": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _d, _pd]),
     "sp_primal_ray_stats": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(RayStats)]),
     "sp_dual_ray_stats": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(RayStats)]),

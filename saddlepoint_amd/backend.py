@@ -324,6 +324,12 @@ class HipBackend:
         self._ok(lib().sp_dual_stats(self.h, p.h, y.h, aty.h, C.byref(out)))
         return out
 
+    def dual_stats_noalloc(self, p: Problem, y: Vector, aty: Vector) -> DualStats:
+        """dual_stats through sp_dual_stats_noalloc (no device allocation)."""
+        out = DualStats()
+        self._ok(lib().sp_dual_stats_noalloc(self.h, p.h, y.h, aty.h, C.byref(out)))
+        return out
+
     def step_dual_residual_sq(self, p: Problem, x: Vector, x_t: Vector, aty: Vector, aty_t: Vector,
                               tau: float) -> float:
         return self._scalar(lib().sp_step_dual_residual_sq, p.h, x.h, x_t.h, aty.h, aty_t.h, C.c_double(tau))

@@ -1135,6 +1135,33 @@ sp_dual_stats_t Hip::dual_stats(const sp_problem* p, const sp_vec* y, const sp_v
     return dual_from(read_result(3, 0));
 }
 
+sp_dual_stats_t Hip::dual_stats_in(const sp_problem* p, const sp_vec* y, const sp_vec* aty, double* ws,
+                                   int64_t cap) {
+    need_len(y, p->m, "dual_stats y");
+    need_len(aty, p->n, "dual_stats aty");
+    constexpr int K = 3;
+    const int grid = reduce_grid(p->m + p->n);
+    // Partials first, the K totals in the last 8 doubles.
+    if (!ws || (int64_t)grid * K > cap - 8)
+        throw Error(SP_ERR_TOO_LARGE, "dual_stats: the reduction needs " + std::to_string((int64_t)grid * K + 8) +
+                                          " doubles of scratch, more than the " + std::to_string(cap) + " it has");
+    double* partials = ws;
+    double* result = ws + cap - 8;
+    auto go = [&](hipFunction_t f, unsigned g, auto... args) {
+        void* params[] = {static_cast<void*>(&args)...};
+        hip_check(hipModuleLaunchKernel(f, g, 1, 1, kWG, 1, 1, 0, b->stream, params, nullptr),
+                  "hipModuleLaunchKernel");
+    };
+    go(b->k_dual_stats, (unsigned)grid, (const double*)y->d, (const double*)p->lc, (const double*)p->uc,
+       (long long)p->m, (const double*)p->c, (const double*)aty->d, (const double*)p->lv, (const double*)p->uv,
+       (const double*)p->df, (long long)p->n, partials, result);
+    if (grid > 1) go(b->k_red_finish, 1u, (const double*)partials, grid, K, result);
+    double r[K];
+    hip_check(hipMemcpyAsync(r, result, sizeof r, hipMemcpyDeviceToHost, b->stream), "hipMemcpyAsync D2H");
+    hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    return dual_from(r);
+}
+
 double Hip::step_dual_residual_sq(const sp_problem* p, const sp_vec* x, const sp_vec* x_t, const sp_vec* aty,
                                   const sp_vec* aty_t, double tau) {
     step_dual_residual_sq_into(p, x, x_t, aty, aty_t, tau, b->red_result);
@@ -1283,6 +1310,13 @@ sp_backend* sp_backend_open(int ordinal) {
         get(b->k_step_dual, "sp_red_step_dual_f64");
         get(b->k_primal_ray, "sp_red_primal_ray_f64");
         get(b->k_dual_ray, "sp_red_dual_ray_f64");
+        // The no-allocation entry point's scratch: this device's copy of
+        // the static array, its address taken now with the device current.
+        try {
+            b->scratch = sp::scratch_on_device(ordinal);
+        } catch (const sp::Error&) {
+            b->scratch = nullptr;  // sp_dual_stats_noalloc then reports SP_ERR_UNAVAILABLE
+        }
         b->red_partials = sp::dmalloc<double>((size_t)sp::Hip::kMaxReduceBlocks * sp::Hip::kMaxSlots);
         b->red_result = sp::dmalloc<double>(sp::Hip::kMaxSlots);
         b->slots = sp::dmalloc<double>(sp::Hip::kSlots);

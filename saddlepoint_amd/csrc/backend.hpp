@@ -101,6 +101,9 @@ struct sp_backend {
     // (one transfer per check iteration in the fused driver).
     double* slots = nullptr;
     double* host_slots = nullptr;
+    // This device's copy of the no-allocation entry point's static scratch
+    // array (noalloc.hip); null if its address could not be taken.
+    double* scratch = nullptr;
 
     // Kernel timing (sp_backend_profile): while enabled, every launch is
     // bracketed by two events on the backend's stream.
@@ -146,6 +149,12 @@ void hip_check(hipError_t e, const char* what);
 void set_error(const sp_backend* b, const std::string& msg);
 const char* last_error(const sp_backend* b);
 void forget_errors(const sp_backend* b);
+
+// The device address of `device`'s copy of the static scratch array of the
+// no-allocation entry point (taken once per device, with `device` current),
+// and its length in doubles (noalloc.hip).
+double* scratch_on_device(int device);
+int64_t scratch_doubles();
 
 // The Backend methods. Each throws sp::Error on failure.
 struct Hip {
@@ -196,6 +205,10 @@ struct Hip {
     void scale(sp_vec* v, double alpha);
     sp_primal_stats_t primal_stats(const sp_problem* p, const sp_vec* x, const sp_vec* ax);
     sp_dual_stats_t dual_stats(const sp_problem* p, const sp_vec* y, const sp_vec* aty);
+    // dual_stats with its partials and totals in caller-provided device memory
+    // (`cap` doubles) instead of the backend's buffers; launches directly
+    // (never through the profiler's events) and allocates nothing.
+    sp_dual_stats_t dual_stats_in(const sp_problem* p, const sp_vec* y, const sp_vec* aty, double* ws, int64_t cap);
     double step_dual_residual_sq(const sp_problem* p, const sp_vec* x, const sp_vec* x_t, const sp_vec* aty,
                                  const sp_vec* aty_t, double tau);
     sp_ray_stats_t primal_ray_stats(const sp_problem* p, const sp_vec* dx, const sp_vec* adx);
