@@ -281,10 +281,90 @@ def serving_bench(args) -> dict:
     return out
 
 
+def stream_bench(args) -> dict:
+    """Steady state of a serving stream: each mode takes LPs from the same
+    config-3 sequence (instance 0, 1, 2, ...), keeps `--stream-backlog` of
+    them submitted and waiting at all times (a saturated server: work is
+    always there), and runs for `--stream` seconds. Completions per second are
+    counted after the first quarter of the window (warm-up) and latencies
+    (completion - submission) taken of the LPs completed in the rest.
+    One at a time solves the same sequence back to back in plain FP64."""
+    from saddlepoint_amd import generate
+    from saddlepoint_amd.backend import HipBackend, default_params, solve
+    from saddlepoint_amd.serving import DONE, FAILED, Server
+
+    T, K = args.stream, args.stream_backlog
+    t0 = time.time()
+    pool = generate.serving_stream(3, args.stream_pool)
+    tols = [1e-4 if i % 2 == 0 else 1e-6 for i in range(len(pool))]
+    log(f"stream: {len(pool)} config-3 LPs generated in {time.time() - t0:.1f}s; window {T}s, backlog {K}")
+    b = HipBackend(0)
+    out = {"window_s": T, "warmup_s": T / 4, "backlog": K, "pool": len(pool),
+           "workload": "config-3 stream (rows log-uniform 1k-200k, columns 1.5-3x rows, 4-30 nonzeros per column, "
+                       "tolerance 50/50 1e-4 / 1e-6), instances taken in order"}
+
+    def summary(done, lat, taken):
+        rate = done / (0.75 * T)
+        return {"lps_per_s": rate, "completed_in_window": done, "taken": taken, "p50_s": pct(lat, 50),
+                "p99_s": pct(lat, 99)}
+
+    # One at a time. LP i joins the backlog when LP i - K completes (at the
+    # start for the first K), so its latency counts its wait in the backlog.
+    start = time.perf_counter()
+    done, lat, i, ends = 0, [], 0, []
+    while time.perf_counter() - start < T and i < len(pool):
+        solve(b, pool[i], default_params(tolerance=tols[i]))
+        s1 = time.perf_counter()
+        ends.append(s1)
+        if s1 - start >= T / 4 and s1 - start < T:
+            done += 1
+            lat.append(s1 - (ends[i - K] if i >= K else start))
+        i += 1
+    out["one_at_a_time_fp64"] = summary(done, lat, i)
+    log(f"  one at a time: {out['one_at_a_time_fp64']['lps_per_s']:.3f} LPs/s ({i} taken)")
+    for prec in ("fp64", "mixed"):
+        srv = Server(b)
+        pr = int(prec == "mixed")
+        sub, ids, i = {}, [], 0
+        start = srv.clock()
+        done, lat, last_report = 0, [], time.time()
+        pending = set()
+        while srv.clock() - start < T:
+            _, waiting, _ = srv.run_chunk() if pending else (0, 0, 0)
+            while waiting < K and i < len(pool):
+                j = srv.submit(pool[i], default_params(tolerance=tols[i], precision=pr))
+                sub[j] = srv.clock()
+                pending.add(j)
+                waiting += 1
+                i += 1
+            for j in [j for j in pending if srv.state(j) in (DONE, FAILED)]:
+                pending.discard(j)
+                r = srv.result(j, with_solution=False)
+                t_rel = r.completed - start
+                if T / 4 <= t_rel < T:
+                    done += 1
+                    lat.append(r.completed - sub[j])
+                srv.forget(j)
+            if time.time() - last_report > 30:
+                log(f"  served {prec}: {srv.clock() - start:.0f}s, {i} taken, {len(pending)} in the server")
+                last_report = time.time()
+        srv.close()
+        out["served_" + prec] = summary(done, lat, i)
+        out["served_" + prec]["speedup_vs_one_at_a_time"] = (out["served_" + prec]["lps_per_s"] /
+                                                            max(out["one_at_a_time_fp64"]["lps_per_s"], 1e-12))
+        log(f"  served {prec}: {out['served_' + prec]['lps_per_s']:.3f} LPs/s ({i} taken)")
+    b.close()
+    return out
+
+
 def large_bench(args) -> dict:
     """The large single instance (config 2), plain FP64 against the mixed
-    precision mode at 1e-4 and 1e-6: iterations, solve time, and each mixed
-    result's objective gap and KKT report (device checker) beside FP64's."""
+    precision mode. Both run the same iteration cap (`--max-iterations`, no
+    time limit unless `--time-limit` > 0) at 1e-4, so a mode that does not
+    converge stops at the same iteration count as the other and the two are
+    compared at equal work: objective gap, distance of the iterates, and each
+    one's KKT report from the device checker. A mode that reaches 1e-4 within
+    the cap is run again at 1e-6 with the same cap."""
     from saddlepoint_amd import generate
     from saddlepoint_amd.backend import HipBackend, check, default_params, solve
 
@@ -292,30 +372,38 @@ def large_bench(args) -> dict:
     p = generate.config2(scale=args.scale)
     log(f"generated config2 m={p.nrows} n={p.ncols} nnz={p.a.nnz} in {time.time() - t0:.1f}s")
     b = HipBackend(0)
-    out = {"m": p.nrows, "n": p.ncols, "nnz": p.a.nnz, "scale": args.scale, "runs": []}
-    fp64 = {}
+    out = {"m": p.nrows, "n": p.ncols, "nnz": p.a.nnz, "scale": args.scale, "max_iterations": args.max_iterations,
+           "runs": []}
+    limit = args.time_limit if args.time_limit > 0 else -1.0
+    tols = [1e-4]
     for tol in (1e-4, 1e-6):
+        if tol not in tols:
+            break
+        res = {}
         for prec in ("fp64", "mixed"):
-            prm = default_params(tolerance=tol, precision=int(prec == "mixed"), gpu_scaling=1,
-                                 max_iterations=args.max_iterations, time_limit_s=args.time_limit)
+            prm = default_params(tolerance=tol, precision=int(prec == "mixed"), max_iterations=args.max_iterations,
+                                 time_limit_s=limit)
             t = time.perf_counter()
             r = solve(b, p, prm)
             wall = time.perf_counter() - t
             k = check(b, p, r.x, r.y)
+            res[prec] = r
             e = {"tolerance": tol, "precision": prec, "status": r.status, "iterations": r.iterations,
                  "restarts": r.restarts, "setup_s": r.setup_time, "solve_s": r.solve_time, "wall_s": wall,
                  "iters_per_s": r.iterations / max(r.solve_time, 1e-9), "objective": r.objective,
                  "kkt_relative_primal": k.relative_primal, "kkt_relative_dual": k.relative_dual,
                  "kkt_relative_gap": k.relative_gap, "fp64_iterations_in_mixed": r.mixed_fp64_iterations}
-            if prec == "fp64":
-                fp64[tol] = r
-            else:
-                f = fp64[tol]
+            if prec == "mixed":
+                f = res["fp64"]
+                e["equal_iterations"] = r.iterations == f.iterations
                 e["objective_gap_vs_fp64"] = abs(r.objective - f.objective) / (1 + abs(f.objective))
                 e["x_rel_diff_vs_fp64"] = float(np.max(np.abs(r.x - f.x)) / (1 + np.max(np.abs(f.x))))
+                e["y_rel_diff_vs_fp64"] = float(np.max(np.abs(r.y - f.y)) / (1 + np.max(np.abs(f.y))))
             log(f"  tol {tol:g} {prec}: {r.status} {r.iterations} it, solve {r.solve_time:.2f}s "
                 f"({e['iters_per_s']:.0f} it/s), setup {r.setup_time:.2f}s, kkt max {k.max_relative():.2e}")
             out["runs"].append(e)
+        if any(r.status == "Optimal" for r in res.values()):
+            tols.append(1e-6)
     # Host scaling of the same LP, for the setup comparison.
     t = time.perf_counter()
     from saddlepoint_amd.backend import ScaledProblem
@@ -435,7 +523,8 @@ def full_solve_bench(args) -> dict:
     p = generate.config1(scale=args.scale)
     log(f"generated config1 m={p.nrows} n={p.ncols} nnz={p.a.nnz}; progress lines every 200 checks (solver.rs:367-378)")
     b = HipBackend(0)
-    prm = default_params(tolerance=1e-6, max_iterations=args.max_iterations, time_limit_s=args.time_limit,
+    prm = default_params(tolerance=1e-6, max_iterations=args.max_iterations,
+                         time_limit_s=args.time_limit if args.time_limit > 0 else -1.0,
                          log_every_checks=200)
     t = time.perf_counter()
     r = solve(b, p, prm)
@@ -471,13 +560,17 @@ def main() -> int:
     ap.add_argument("--large", action="store_true",
                     help="config 2 instead: FP64 vs mixed precision at 1e-4 and 1e-6 (use --scale to shrink)")
     ap.add_argument("--max-iterations", type=int, default=200_000)
-    ap.add_argument("--time-limit", type=float, default=150.0, help="--large: seconds per solve")
+    ap.add_argument("--time-limit", type=float, default=0.0, help="--large, --full-solve: seconds per solve (0: none)")
     ap.add_argument("--full-solve", action="store_true",
                     help="config 1 solved to 1e-6 on the GPU (KKT from the device checker)")
     ap.add_argument("--reference-shapes", action="store_true",
                     help="us/iteration on random LPs of the reference's published instance sizes, beside its figures")
     ap.add_argument("--share", type=int, default=0,
                     help="config 2's SpMVs on one GPU's share of a P-way row-block split (P = this value)")
+    ap.add_argument("--stream", type=float, default=0.0,
+                    help="steady-state serving over a window of this many seconds (saturated, per mode)")
+    ap.add_argument("--stream-backlog", type=int, default=64, help="--stream: LPs kept submitted and waiting")
+    ap.add_argument("--stream-pool", type=int, default=600, help="--stream: LPs of the sequence generated up front")
     ap.add_argument("--serving", action="store_true",
                     help="serving stream instead: LPs/s and latency, served (FP64 and mixed) vs one at a time")
     ap.add_argument("--config", type=int, default=3, choices=[3, 4], help="serving stream (3, or 4 with large LPs)")
@@ -525,6 +618,11 @@ def main() -> int:
     if args.share:
         r = share_bench(args)
         print(json.dumps({"metric": METRIC, "share": r}), flush=True)
+        return 0
+
+    if args.stream:
+        r = stream_bench(args)
+        print(json.dumps({"metric": METRIC, "stream": r}), flush=True)
         return 0
 
     if args.serving:
@@ -631,6 +729,12 @@ def iteration_bench(args) -> dict:
                                 "total_ms": round(ms, 3)}
     sides = {0: "sp_spmv_t_primal_mixed" if mixed else "sp_spmv_t_primal_halpern_f64",
              1: "sp_spmv_dual_mixed" if mixed else "sp_spmv_dual_halpern_f64"}
+    if any(k.startswith("sp_spmv_panel") for k in per_kernel):
+        # Column panels: the A side is the panel SpMV, the short rows' SpMV and
+        # the long rows' finishing pass, timed together.
+        label = f"A side ({'sp_spmv_panel_f32' if mixed else 'sp_spmv_panel_f64'} + {sides[1]} + finishing)"
+        kb[label] = kb[sides[1]]
+        sides[1] = label
     # Each side's main kernel timed without per-launch events: two events
     # around 20 back-to-back launches on copies of the vectors it writes.
     for side, name in sides.items():

@@ -180,6 +180,11 @@ typedef struct {
                              original order. 0: the LP's own column order */
     int64_t mixed_stall_checks; /* precision = 1: checks without halving the relative KKT error after
                                    which the solve continues in FP64 (0, the default: no such switch) */
+    int column_panels; /* A' rows with more than 64 nonzeros gather x̄ panel by panel, so that the
+                          workgroups in flight share an L2-sized slice of it (sp_problem_set_panels):
+                          k in [8, 30]: panels of 2^k columns; 0: off; -1 (default): k = 18 when A' has
+                          at least 2^21 columns (a gathered vector of 16 MB or more), else off. Changes
+                          the order of the long rows' sums only. The serving loop runs without panels */
 } sp_params;
 
 /* Status (solver.rs:95-102). */
@@ -250,7 +255,8 @@ int sp_probe(sp_backend *b, int kind, int64_t n, int64_t table_len, int reps, do
 /* No counterpart in the reference: the ceiling a matrix's own access pattern
  * puts on its SpMV's gathers. Average milliseconds of one launch that loads,
  * for every nonzero of A' (transpose = 0) or A'ᵀ (1) of a loaded problem and
- * in the SpMV's order and tiling, the table entry its column index names
+ * in its SpMV plan's order and work items (with column panels: the panel
+ * matrix's, then the short rows'), the table entry its column index names
  * (a double, or a float with fp32 = 1) and nothing else. */
 int sp_gather_replay_time(sp_backend *b, const sp_problem *p, int transpose, int fp32, int reps, double *ms);
 /* Waits for all work queued on the backend's stream. */
@@ -319,6 +325,15 @@ int sp_problem_load_tuned(sp_backend *b, const sp_scaled_view *p, const sp_spmv_
 /* Average milliseconds of one spmv (transpose = 0) or spmv_t (1) of a loaded
  * problem over `reps` launches, timed with HIP events on the backend's stream. */
 int sp_spmv_time(sp_backend *b, const sp_problem *p, int transpose, int reps, double *ms);
+/* No counterpart in the reference. Column panels for A' of a loaded problem:
+ * the rows with more than 64 nonzeros are also kept as a panel matrix whose
+ * rows are their parts in consecutive column ranges of 2^shift columns, stored
+ * and run panel by panel (so the gathers of the workgroups in flight hit one
+ * L2-sized slice of the input), and each row adds its panel sums in panel
+ * order; the other rows run as before. Every A' product of the problem uses it
+ * from then on (the same sums in every method). shift in [8, 30]; 0 removes
+ * the panels. A matrix without such rows is left as it is. */
+int sp_problem_set_panels(sp_backend *b, sp_problem *p, int shift);
 
 /* ---- serving: continuous batching of many LPs on one GPU ---------------------- */
 

@@ -69,7 +69,8 @@ class Params(C.Structure):
                 ("has_pock_chambolle", C.c_int), ("pock_chambolle_alpha", _d),
                 ("bound_objective_rescaling", C.c_int), ("termination", C.c_int), ("fused", C.c_int),
                 ("log_every_checks", C.c_int), ("precision", C.c_int), ("spmv_autotune", C.c_int),
-                ("gpu_scaling", C.c_int), ("reorder_columns", C.c_int), ("mixed_stall_checks", _i64)]
+                ("gpu_scaling", C.c_int), ("reorder_columns", C.c_int), ("mixed_stall_checks", _i64),
+                ("column_panels", C.c_int)]
 
 
 class Solution(C.Structure):
@@ -149,6 +150,7 @@ SIGNATURES = {
     "sp_spmv_tuning_for": (None, [C.POINTER(_i64), _i64, C.c_void_p]),
     "sp_problem_load_tuned": (C.c_int, [_vp, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(_vp)]),
     "sp_spmv_time": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _pd]),
+    "sp_problem_set_panels": (C.c_int, [_vp, _vp, C.c_int]),
     "sp_default_server_params": (None, [C.POINTER(ServerParams)]),
     "sp_server_create": (C.c_int, [_vp, C.POINTER(ServerParams), C.POINTER(_vp)]),
     "sp_server_free": (None, [_vp]),

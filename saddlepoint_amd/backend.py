@@ -216,6 +216,10 @@ class HipBackend:
             self._ok(lib().sp_problem_load_tuned(self.h, C.byref(v), ta, tt, C.byref(h)))
         return Problem(self, h, s.a.nrows, s.a.ncols)
 
+    def set_panels(self, p: Problem, shift: int) -> None:
+        """Column panels of 2^shift columns for A's long rows (0: none)."""
+        self._ok(lib().sp_problem_set_panels(self.h, p.h, int(shift)))
+
     def spmv_time(self, p: Problem, transpose: bool = False, reps: int = 20) -> float:
         """Average milliseconds of one spmv (or spmv_t) of a loaded problem."""
         ms = C.c_double()

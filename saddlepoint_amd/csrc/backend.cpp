@@ -110,6 +110,25 @@ unsigned ew_grid(int64_t n) {
 template <class... E>
 void spmv_launch(sp_backend* b, hipFunction_t k, hipFunction_t kfin, const DevCsr& a, const void* va,
                  const void* xin, E... epi) {
+    if (a.pmat) {
+        // Column panels: the panel matrix's sums (panel by panel), the short
+        // rows, then each long row's panel sums added in panel order with the
+        // epilogue. FP32 values in, FP32 panel sums out.
+        const DevCsr& B = *a.pmat;
+        const bool f32 = a.va32 && va == (const void*)a.va32;
+        const void* bva = f32 ? (const void*)B.va32 : (const void*)B.va;
+        hipFunction_t kp = f32 ? b->k_spmv_panel32 : b->k_spmv_panel;
+        hipFunction_t kpf = f32 ? b->k_fin_panel32 : b->k_fin_panel;
+        if (B.nblocks)
+            launch(b, kp, (unsigned)B.nblocks, B.desc, B.rp, B.ci, bva, xin, (void*)B.partials, (void*)a.psum, a.plong,
+                   a.npanels);
+        if (B.nlong)
+            launch(b, kpf, (unsigned)((B.nlong + 3) / 4), (const LongRow*)B.longs, B.nlong, (const void*)B.partials,
+                   (void*)a.psum, a.plong, a.npanels);
+        if (a.npdesc) launch(b, k, (unsigned)a.npdesc, a.pdesc, a.rp, a.ci, va, xin, (void*)a.partials, epi...);
+        launch(b, kfin, (unsigned)((a.plong + 3) / 4), (const LongRow*)a.plongs, a.plong, (const void*)a.psum, epi...);
+        return;
+    }
     if (a.nblocks) launch(b, k, (unsigned)a.nblocks, a.desc, a.rp, a.ci, va, xin, (void*)a.partials, epi...);
     if (a.nlong)
         launch(b, kfin, (unsigned)((a.nlong + 3) / 4), (const LongRow*)a.longs, a.nlong, (const void*)a.partials,
@@ -159,7 +178,20 @@ void plan_csr(sp_backend* b, DevCsr& d, const int64_t* row_ptr, const SpmvTuning
     hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
 }
 
+void free_csr(DevCsr& d);
+
+void free_panels(DevCsr& d) {
+    if (d.pmat) {
+        free_csr(*d.pmat);
+        delete d.pmat;
+    }
+    dfree(d.psum), dfree(d.plongs), dfree(d.pdesc);
+    d.pmat = nullptr, d.psum = nullptr, d.plongs = nullptr, d.pdesc = nullptr;
+    d.plong = d.npanels = d.pshift = d.npdesc = 0;
+}
+
 void free_csr(DevCsr& d) {
+    free_panels(d);
     dfree(d.rp), dfree(d.ci), dfree(d.va), dfree(d.va32), dfree(d.desc), dfree(d.partials), dfree(d.longs);
     d = DevCsr{};
 }
@@ -392,6 +424,103 @@ sp_problem* Hip::load_scaled(const LpView& p_in, const ScalingParams& sp, bool a
     return prob;
 }
 
+namespace {
+void scan_ints(sp_backend* b, int* a, int64_t len, std::vector<int*>& scratch);
+}  // namespace
+
+void Hip::set_panels(sp_problem* prob, int shift) {
+    if (!prob) throw Error(SP_ERR_INVALID, "set_panels: null problem");
+    DevCsr& A = prob->a;
+    hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    free_panels(A);
+    if (shift == 0) return;
+    if (shift < 8 || shift > 30) throw Error(SP_ERR_INVALID, "set_panels: panel width 2^shift needs shift in [8, 30]");
+    const int m = A.nrows, n = A.ncols;
+    std::vector<int> rp32((size_t)m + 1);
+    hip_check(hipMemcpyAsync(rp32.data(), A.rp, sizeof(int) * rp32.size(), hipMemcpyDeviceToHost, b->stream),
+              "hipMemcpyAsync D2H");
+    hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    // The panel rows are the rows the short rows' plan below leaves out.
+    SpmvTuning ts = A.tuning;
+    ts.long_row = std::min(ts.block_nnz, kPanelMinRow);
+    std::vector<int> rows;
+    for (int r = 0; r < m; ++r)
+        if (rp32[(size_t)r + 1] - rp32[(size_t)r] > ts.long_row) rows.push_back(r);
+    if (rows.empty()) return;
+    const int L = (int)rows.size();
+    const int64_t P64 = ((int64_t)n + (int64_t(1) << shift) - 1) >> shift;
+    const int64_t nq = (int64_t)L * P64;
+    if (nq + 1 > INT_MAX / 2) throw Error(SP_ERR_TOO_LARGE, "set_panels: too many (panel, long row) pairs");
+    const int P = (int)P64;
+    std::vector<int*> scratch;
+    auto* B = new DevCsr;
+    // The panel parts of a row are contiguous only where its columns are
+    // sorted, which relabeling by popularity does not keep. A'ᵀ transposed
+    // back has A's rows with their columns in increasing order (its rows are
+    // scanned in column order); the panel matrix takes its nonzeros from it.
+    DevCsr S;
+    try {
+        std::vector<int64_t> srp;
+        transpose(prob->at, S, false, srp);
+        int* drows = dupload(b->stream, rows.data(), rows.size());
+        scratch.push_back(drows);
+        int* start = dmalloc<int>((size_t)nq);
+        scratch.push_back(start);
+        B->rp = dmalloc<int>((size_t)nq + 1);  // the counts, then their exclusive scan
+        launch(b, b->k_panel_count, (unsigned)((nq + kWG - 1) / kWG), (const int*)S.rp, (const int*)S.ci,
+               (const int*)drows, L, P, shift, start, B->rp);
+        hip_check(hipMemsetAsync(B->rp + nq, 0, sizeof(int), b->stream), "hipMemsetAsync");
+        scan_ints(b, B->rp, nq + 1, scratch);
+        std::vector<int> brp32((size_t)nq + 1);
+        hip_check(hipMemcpyAsync(brp32.data(), B->rp, sizeof(int) * brp32.size(), hipMemcpyDeviceToHost, b->stream),
+                  "hipMemcpyAsync D2H");
+        hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        B->nrows = (int)nq;
+        B->ncols = n;
+        B->nnz = brp32[(size_t)nq];
+        B->ci = dmalloc<int>((size_t)std::max<int64_t>(B->nnz, 1));
+        B->va = dmalloc<double>((size_t)std::max<int64_t>(B->nnz, 1));
+        launch(b, b->k_panel_gather, (unsigned)((nq * 64 + kWG - 1) / kWG), (const int*)start, (const int*)B->rp,
+               (long long)nq, (const int*)S.ci, (const double*)S.va, B->ci, B->va);
+        if (A.va32) {
+            B->va32 = dmalloc<float>((size_t)std::max<int64_t>(B->nnz, 1));
+            if (B->nnz) launch(b, b->k_to_f32, ew_grid(B->nnz), (const double*)B->va, B->va32, (long long)B->nnz);
+        }
+        const std::vector<int64_t> brp(brp32.begin(), brp32.end());
+        plan_csr(b, *B, brp.data(), SpmvTuning{});  // panel-major stream blocks (synchronizes)
+        // Each long row's P panel sums, added in panel order by the finishing kernels.
+        std::vector<LongRow> lr((size_t)L);
+        for (int l = 0; l < L; ++l) lr[(size_t)l] = LongRow{rows[(size_t)l], l * P, P, 0};
+        A.plongs = dupload(b->stream, lr.data(), lr.size());
+        A.psum = dmalloc<double>((size_t)nq);
+        // The short rows: the stream blocks of a plan whose long rows are the panel rows.
+        const std::vector<int64_t> rp64(rp32.begin(), rp32.end());
+        SpmvPlan sp = plan_spmv(rp64.data(), m, ts);
+        std::vector<Desc> shorts;
+        for (const Desc& d : sp.desc)
+            if (d.r1 >= 0) shorts.push_back(d);
+        A.npdesc = (int)shorts.size();
+        A.pdesc = shorts.empty() ? dmalloc<Desc>(1) : dupload(b->stream, shorts.data(), shorts.size());
+        A.pmat = B;
+        A.plong = L;
+        A.npanels = P;
+        A.pshift = shift;
+        hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    } catch (...) {
+        (void)hipStreamSynchronize(b->stream);
+        for (int* q : scratch) dfree(q);
+        free_csr(S);
+        if (A.pmat != B) {
+            free_csr(*B);
+            delete B;
+        }
+        free_panels(A);
+        throw;
+    }
+    for (int* q : scratch) dfree(q);
+    free_csr(S);
+}
+
 void Hip::upload(DevCsr& d, const int64_t* row_ptr, const int64_t* col_idx, const double* values, int64_t nrows,
                  int64_t ncols, int64_t nnz, const SpmvTuning& tuning) {
     upload_csr(b, d, row_ptr, col_idx, values, nrows, ncols, nnz, tuning);
@@ -614,7 +743,8 @@ void Hip::copy(const sp_vec* src, sp_vec* dst) {
 
 void Hip::ensure_fp32(sp_problem* p) {
     if (p->has_fp32) return;
-    for (DevCsr* a : {&p->a, &p->at}) {
+    for (DevCsr* a : {&p->a, &p->at, p->a.pmat}) {
+        if (!a) continue;
         a->va32 = dmalloc<float>((size_t)a->nnz);
         if (a->nnz) launch(b, b->k_to_f32, ew_grid(a->nnz), (const double*)a->va, a->va32, (long long)a->nnz);
     }
@@ -816,11 +946,19 @@ double Hip::gather_replay_time(const sp_problem* p, bool transpose, bool fp32, i
     const DevCsr& M = transpose ? p->at : p->a;
     std::unique_ptr<sp_vec, void (*)(sp_vec*)> table(zeros(std::max(M.ncols, 1)), free_vec);
     double* sink = dmalloc<double>(1);
-    const long long nnz = (long long)M.nnz;
-    const unsigned grid = (unsigned)std::max<int64_t>(1, (M.nnz + kStreamNnzF64 - 1) / kStreamNnzF64);
     hipFunction_t k = fp32 ? b->k_replay32 : b->k_replay64;
+    // The work items of the matrix's own plan (with column panels: the panel
+    // matrix's, then the short rows'), each loading its nonzeros' table entries.
     auto once = [&] {
-        if (nnz) launch(b, k, grid, (const int*)M.ci, nnz, (const void*)table->d, sink);
+        if (M.pmat) {
+            const DevCsr& B = *M.pmat;
+            if (B.nblocks) launch(b, k, (unsigned)B.nblocks, (const Desc*)B.desc, (const int*)B.ci,
+                                  (const void*)table->d, sink);
+            if (M.npdesc) launch(b, k, (unsigned)M.npdesc, (const Desc*)M.pdesc, (const int*)M.ci,
+                                 (const void*)table->d, sink);
+        } else if (M.nblocks) {
+            launch(b, k, (unsigned)M.nblocks, (const Desc*)M.desc, (const int*)M.ci, (const void*)table->d, sink);
+        }
     };
     double ms = 0.0;
     try {
@@ -840,6 +978,9 @@ double Hip::plain_kernel_time(const sp_problem* p, int side, bool mixed, const P
     if (mixed && !p->has_fp32) throw Error(SP_ERR_INVALID, "plain_kernel_time: FP32 values not prepared");
     const DevCsr& M = side == 0 ? p->at : p->a;
     if (!M.nblocks) return 0.0;
+    // With column panels a side is several launches (panel sums, short rows,
+    // finishing); they are timed together. Otherwise the main kernel alone.
+    const bool whole = M.pmat != nullptr;
     // Copies of what the kernel writes: the iterate it updates in place (and
     // x̄ on the A'ᵀ side).
     std::vector<void*> scratch;
@@ -856,18 +997,28 @@ double Hip::plain_kernel_time(const sp_problem* p, int side, bool mixed, const P
             double* x = (double*)copy_of(a.x->d, 8 * n);
             double* xb = (double*)copy_of(a.x_bar->d, 8 * n);
             auto once = [&] {
-                launch(b, b->k_spmv_t_primal_halpern, (unsigned)M.nblocks, M.desc, M.rp, M.ci, (const void*)M.va,
-                       (const void*)a.y->d, (void*)M.partials, x, (const double*)p->c, (const double*)p->lv,
-                       (const double*)p->uv, (const double*)a.x_anchor->d, a.tau, a.w, a.rho, xb);
+                if (whole)
+                    spmv_launch(b, b->k_spmv_t_primal_halpern, b->k_fin_t_primal_halpern, M, M.va, a.y->d, x,
+                                (const double*)p->c, (const double*)p->lv, (const double*)p->uv,
+                                (const double*)a.x_anchor->d, a.tau, a.w, a.rho, xb);
+                else
+                    launch(b, b->k_spmv_t_primal_halpern, (unsigned)M.nblocks, M.desc, M.rp, M.ci, (const void*)M.va,
+                           (const void*)a.y->d, (void*)M.partials, x, (const double*)p->c, (const double*)p->lv,
+                           (const double*)p->uv, (const double*)a.x_anchor->d, a.tau, a.w, a.rho, xb);
             };
             once();
             ms = time_reps(b, reps, once) / reps;
         } else if (!mixed) {
             double* y = (double*)copy_of(a.y->d, 8 * m);
             auto once = [&] {
-                launch(b, b->k_spmv_dual_halpern, (unsigned)M.nblocks, M.desc, M.rp, M.ci, (const void*)M.va,
-                       (const void*)a.x_bar->d, (void*)M.partials, y, (const double*)p->lc, (const double*)p->uc,
-                       (const double*)a.y_anchor->d, a.sigma, a.w, a.rho);
+                if (whole)
+                    spmv_launch(b, b->k_spmv_dual_halpern, b->k_fin_dual_halpern, M, M.va, a.x_bar->d, y,
+                                (const double*)p->lc, (const double*)p->uc, (const double*)a.y_anchor->d, a.sigma,
+                                a.w, a.rho);
+                else
+                    launch(b, b->k_spmv_dual_halpern, (unsigned)M.nblocks, M.desc, M.rp, M.ci, (const void*)M.va,
+                           (const void*)a.x_bar->d, (void*)M.partials, y, (const double*)p->lc, (const double*)p->uc,
+                           (const double*)a.y_anchor->d, a.sigma, a.w, a.rho);
             };
             once();
             ms = time_reps(b, reps, once) / reps;
@@ -875,19 +1026,31 @@ double Hip::plain_kernel_time(const sp_problem* p, int side, bool mixed, const P
             float* dx = (float*)copy_of(a.dx->d, 4 * n);
             float* dxb = (float*)copy_of(a.dx_bar->d, 4 * n);
             auto once = [&] {
-                launch(b, b->k_spmv_t_primal_mixed, (unsigned)M.nblocks, M.desc, M.rp, M.ci, (const void*)M.va32,
-                       (const void*)a.dy->d, (void*)M.partials, dx, (const double*)a.x->d, (const double*)p->c,
-                       (const double*)p->lv, (const double*)p->uv, (const double*)a.atya->d, (const float*)a.ex->d,
-                       a.tau, a.w, a.rho, dxb);
+                if (whole)
+                    spmv_launch(b, b->k_spmv_t_primal_mixed, b->k_fin_t_primal_mixed, M, M.va32, a.dy->d, dx,
+                                (const double*)a.x->d, (const double*)p->c, (const double*)p->lv,
+                                (const double*)p->uv, (const double*)a.atya->d, (const float*)a.ex->d, a.tau, a.w,
+                                a.rho, dxb);
+                else
+                    launch(b, b->k_spmv_t_primal_mixed, (unsigned)M.nblocks, M.desc, M.rp, M.ci, (const void*)M.va32,
+                           (const void*)a.dy->d, (void*)M.partials, dx, (const double*)a.x->d, (const double*)p->c,
+                           (const double*)p->lv, (const double*)p->uv, (const double*)a.atya->d,
+                           (const float*)a.ex->d, a.tau, a.w, a.rho, dxb);
             };
             once();
             ms = time_reps(b, reps, once) / reps;
         } else {
             float* dy = (float*)copy_of(a.dy->d, 4 * m);
             auto once = [&] {
-                launch(b, b->k_spmv_dual_mixed, (unsigned)M.nblocks, M.desc, M.rp, M.ci, (const void*)M.va32,
-                       (const void*)a.dx_bar->d, (void*)M.partials, dy, (const double*)a.y->d, (const double*)p->lc,
-                       (const double*)p->uc, (const double*)a.axa->d, (const float*)a.ey->d, a.sigma, a.w, a.rho);
+                if (whole)
+                    spmv_launch(b, b->k_spmv_dual_mixed, b->k_fin_dual_mixed, M, M.va32, a.dx_bar->d, dy,
+                                (const double*)a.y->d, (const double*)p->lc, (const double*)p->uc,
+                                (const double*)a.axa->d, (const float*)a.ey->d, a.sigma, a.w, a.rho);
+                else
+                    launch(b, b->k_spmv_dual_mixed, (unsigned)M.nblocks, M.desc, M.rp, M.ci, (const void*)M.va32,
+                           (const void*)a.dx_bar->d, (void*)M.partials, dy, (const double*)a.y->d,
+                           (const double*)p->lc, (const double*)p->uc, (const double*)a.axa->d,
+                           (const float*)a.ey->d, a.sigma, a.w, a.rho);
             };
             once();
             ms = time_reps(b, reps, once) / reps;
@@ -1297,6 +1460,12 @@ sp_backend* sp_backend_open(int ordinal) {
         get(b->k_probe_copy32, "sp_probe_copy_f32");
         get(b->k_probe_triad32, "sp_probe_triad_f32");
         get(b->k_replay64, "sp_probe_replay_f64");
+        get(b->k_panel_count, "sp_panel_count");
+        get(b->k_panel_gather, "sp_panel_gather");
+        get(b->k_spmv_panel, "sp_spmv_panel_f64");
+        get(b->k_fin_panel, "sp_finish_panel_f64");
+        get(b->k_spmv_panel32, "sp_spmv_panel_f32");
+        get(b->k_fin_panel32, "sp_finish_panel_f32");
         get(b->k_replay32, "sp_probe_replay_f32");
         get(b->k_spmv_t_primal_mixed, "sp_spmv_t_primal_mixed");
         get(b->k_spmv_dual_mixed, "sp_spmv_dual_mixed");

@@ -50,7 +50,22 @@ struct DevCsr {
     std::vector<Desc> host_desc;       // the plan, kept for batch tables
     std::vector<LongRow> host_longs;
     SpmvTuning tuning;
+    // Column panels (Hip::set_panels): the long rows (more than kPanelMinRow
+    // nonzeros) are also kept as a panel matrix whose row q = p·L + l holds
+    // long row l's nonzeros in column panel p (columns [p·2^pshift,
+    // (p+1)·2^pshift)), stored and dispatched panel by panel; its sums land in
+    // psum[l·P + p] and the finishing table `plongs` adds each long row's P
+    // sums in panel order. The short rows run as the stream blocks `pdesc`.
+    DevCsr* pmat = nullptr;
+    int plong = 0, npanels = 0, pshift = 0;
+    double* psum = nullptr;     // L·P panel sums (floats in mixed precision)
+    LongRow* plongs = nullptr;  // {row, l·P, P}
+    Desc* pdesc = nullptr;      // the short rows' stream blocks
+    int npdesc = 0;
 };
+
+// Rows longer than this are gathered by column panel when panels are on.
+constexpr int kPanelMinRow = 64;
 
 // Host-side planner: cuts rows into stream blocks and long-row chunks.
 struct SpmvPlan {
@@ -135,6 +150,7 @@ struct sp_backend {
     hipFunction_t k_probe_copy, k_probe_triad, k_probe_gather, k_probe_copy32, k_probe_triad32, k_replay64,
         k_replay32;
     hipFunction_t k_spmv32, k_spmv_t_primal_mixed, k_spmv_dual_mixed, k_to_f32, k_delta_from, k_delta_to;
+    hipFunction_t k_panel_count, k_panel_gather, k_spmv_panel, k_fin_panel, k_spmv_panel32, k_fin_panel32;
     hipFunction_t k_dot, k_dist, k_primal_stats, k_dual_stats, k_step_dual, k_primal_ray, k_dual_ray;
 };
 
@@ -180,6 +196,11 @@ struct Hip {
     // autotuned rule's or the default); its row pointers are also returned
     // in `host_rp`.
     void transpose(const DevCsr& A, DevCsr& T, bool autotune, std::vector<int64_t>& host_rp);
+    // Column panels for A' of a loaded problem: from now on every A' product
+    // gathers the long rows' nonzeros panel by panel, 2^shift columns per
+    // panel (shift in [8, 30]); shift 0 drops them. A matrix without long rows
+    // keeps its plain plan.
+    void set_panels(sp_problem* p, int shift);
     // Uploads a CSR matrix (int32 indices) and plans it, as load does.
     void upload(DevCsr& d, const int64_t* row_ptr, const int64_t* col_idx, const double* values, int64_t nrows,
                 int64_t ncols, int64_t nnz, const SpmvTuning& tuning);

@@ -277,6 +277,7 @@ void sp_default_params(sp_params* p) {
     p->spmv_autotune = 0;
     p->gpu_scaling = 1;
     p->reorder_columns = 1;
+    p->column_panels = -1;
 }
 
 int sp_solve(sp_backend* b, const sp_lp_view* lp, const sp_params* params, sp_solution* out, double* x_out,
@@ -432,6 +433,13 @@ int sp_spmv_time(sp_backend* b, const sp_problem* p, int transpose, int reps, do
     return guarded(b, [&] {
         if (!p || !ms) throw sp::Error(SP_ERR_INVALID, "spmv_time: null argument");
         *ms = sp::Hip{b}.spmv_time(p, transpose != 0, reps);
+    });
+}
+
+int sp_problem_set_panels(sp_backend* b, sp_problem* p, int shift) {
+    return guarded(b, [&] {
+        check_problem(b, p);
+        sp::Hip{b}.set_panels(p, shift);
     });
 }
 

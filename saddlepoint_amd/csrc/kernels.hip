@@ -297,6 +297,18 @@ struct EpiStore {
     __device__ void apply(int r, A s, const Pre&) const { st(out + r, s); }
 };
 
+// The panel matrix's sums (below): its row q = p·L + l is long row l's
+// share in column panel p, and its sum goes to out[l·P + p], so that each
+// long row's panel sums sit together, in panel order, for the finishing pass.
+template <class A>
+struct EpiStorePanel {
+    A* out;
+    int L, P;
+    struct Pre {};
+    __device__ Pre load(int) const { return {}; }
+    __device__ void apply(int q, A s, const Pre&) const { out[(i64)(q % L) * P + q / L] = s; }
+};
+
 // spmv_t + primal_step: aty = A'ᵀy; x_t = clamp(x - τ(c - aty), lv, uv);
 // x_bar = 2 x_t - x (cpu/src/lib.rs:147-160). Stores aty, x_t and x_bar.
 struct EpiPrimal {
@@ -445,6 +457,74 @@ extern "C" __global__ __launch_bounds__(WG) void sp_finish_dual_halpern_f64(
     finish_body<double>(longs, nlong, partials, EpiDualHalpern{y, lc, uc, anchor, sigma, w, rho});
 }
 
+// ---- column panels: the long rows of a wide matrix ------------------------------
+//
+// A row's gathers land anywhere in x, and when x is far larger than an XCD's
+// L2 (config 2: 80 MB against 4 MB) almost every gather misses it. The long
+// rows of such a matrix (more than 64 nonzeros; on config 2 they hold 94% of
+// the nonzeros) are therefore also kept as a panel matrix: its row
+// q = p·L + l holds long row l's nonzeros whose columns fall in panel p
+// (columns [p·W, (p+1)·W), a contiguous part of the row since its columns are
+// sorted), and its rows are stored and dispatched panel by panel. The
+// workgroups in flight then all gather from one panel of x, a slice that fits
+// in L2. The panel matrix is an ordinary CSR matrix with its own SpMV plan;
+// its sums land in a buffer row by row (EpiStorePanel), and the long-row
+// finishing kernels add each long row's P panel sums in panel order and
+// apply the epilogue. The short rows run as stream blocks of the original
+// matrix. Every A' product uses this one decomposition, so the summation
+// order is a function of the matrix and its plan, as before.
+
+// First position k in [a, b) with ci[k] >= v (ci sorted on [a, b)).
+__device__ __forceinline__ int lower_bound_col(const int* __restrict__ ci, int a, int b, i64 v) {
+    while (a < b) {
+        const int mid = a + ((b - a) >> 1);
+        if ((i64)ci[mid] < v) a = mid + 1;
+        else b = mid;
+    }
+    return a;
+}
+
+// For each panel-matrix row q = p·L + l: where long row rows[l]'s panel-p
+// nonzeros start in A and how many there are.
+extern "C" __global__ __launch_bounds__(WG) void sp_panel_count(const int* __restrict__ rp, const int* __restrict__ ci,
+                                                                const int* __restrict__ rows, int L, int P, int shift,
+                                                                int* __restrict__ start, int* __restrict__ count) {
+    const i64 q = (i64)blockIdx.x * WG + threadIdx.x;
+    if (q >= (i64)P * L) return;
+    const int p = (int)(q / L), r = rows[q % L];
+    const int lo = lower_bound_col(ci, rp[r], rp[r + 1], (i64)p << shift);
+    const int hi = lower_bound_col(ci, lo, rp[r + 1], (i64)(p + 1) << shift);
+    start[q] = lo;
+    count[q] = hi - lo;
+}
+
+// The panel matrix's column indices and values (one wave per row).
+extern "C" __global__ __launch_bounds__(WG) void sp_panel_gather(const int* __restrict__ start,
+                                                                 const int* __restrict__ brp, i64 nq,
+                                                                 const int* __restrict__ ci,
+                                                                 const double* __restrict__ va,
+                                                                 int* __restrict__ bci, double* __restrict__ bva) {
+    const i64 q = ((i64)blockIdx.x * WG + threadIdx.x) / WAVE;
+    const int lane = threadIdx.x % WAVE;
+    if (q >= nq) return;  // uniform per wave
+    const int s = start[q], d0 = brp[q], len = brp[q + 1] - d0;
+    for (int t = lane; t < len; t += WAVE) {
+        bci[d0 + t] = ci[s + t];
+        bva[d0 + t] = va[s + t];
+    }
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_spmv_panel_f64(SPMV_ARGS, double* __restrict__ out, int L,
+                                                                   int P) {
+    spmv_body<SP_NNZ_F64, double, double, double>(desc, rp, ci, va, xin, partials,
+                                                  EpiStorePanel<double>{out, L, P});
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_finish_panel_f64(FIN_ARGS(double), double* __restrict__ out,
+                                                                     int L, int P) {
+    finish_body<double>(longs, nlong, partials, EpiStorePanel<double>{out, L, P});
+}
+
 // ---- mixed precision: FP32 matrix and displacements, FP64 anchors ------------
 //
 // Between checks the iterates are held as z = z_base + dz: z_base is the FP64
@@ -526,6 +606,16 @@ extern "C" __global__ __launch_bounds__(WG) void sp_spmv_dual_mixed(
 
 extern "C" __global__ __launch_bounds__(WG) void sp_finish_f32(FIN_ARGS(float), float* __restrict__ out) {
     finish_body<float>(longs, nlong, partials, EpiStore<float>{out});
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_spmv_panel_f32(SPMV32_ARGS, float* __restrict__ out, int L,
+                                                                   int P) {
+    spmv_body<SP_NNZ_F64, float, float, float>(desc, rp, ci, va, xin, partials, EpiStorePanel<float>{out, L, P});
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_finish_panel_f32(FIN_ARGS(float), float* __restrict__ out, int L,
+                                                                     int P) {
+    finish_body<float>(longs, nlong, partials, EpiStorePanel<float>{out, L, P});
 }
 
 extern "C" __global__ __launch_bounds__(WG) void sp_finish_t_primal_mixed(
@@ -1177,44 +1267,48 @@ extern "C" __global__ __launch_bounds__(WG) void sp_probe_triad_f32(const float*
     GRID_LOOP(i, n) { c[i] = __builtin_nontemporal_load(a + i) + s * __builtin_nontemporal_load(b + i); }
 }
 
-// The gathers of an SpMV and nothing else: workgroup b takes nonzeros
-// [b·SP_NNZ_F64, (b+1)·SP_NNZ_F64) of a matrix's column indices with the
-// stream blocks' straight-line loads (the same lanes, the same order, all
-// in flight) and loads table[col] for each. Values, row pointers and the
-// epilogue are left out, so its rate is the ceiling that this matrix's
-// access pattern puts on the SpMV's gathers. The sum is stored only if it
-// is NaN, which a table of finite values never gives.
+// The gathers of an SpMV and nothing else: workgroup b takes work item
+// desc[b] of a matrix's SpMV plan (a stream block's nonzeros [k0, k1), or a
+// long-row chunk's) and, in batches of SP_NNZ_F64 with the stream blocks'
+// straight-line loads (the same lanes, the same order, all in flight), loads
+// each nonzero's column index and table[col]. Values, row pointers, the
+// reduction and the epilogue are left out, so its rate is the ceiling that
+// this matrix's access pattern puts on the SpMV's gathers. The sum is stored
+// only if it is NaN, which a table of finite values never gives.
 template <class T>
-__device__ __forceinline__ void replay_body(const int* __restrict__ ci, i64 nnz, const T* __restrict__ table,
-                                            double* __restrict__ sink) {
+__device__ __forceinline__ void replay_body(const Desc* __restrict__ desc, const int* __restrict__ ci,
+                                            const T* __restrict__ table, double* __restrict__ sink) {
     constexpr int PER = SP_NNZ_F64 / WG;
-    const i64 k0 = (i64)blockIdx.x * SP_NNZ_F64;
-    const int n = (int)min((i64)SP_NNZ_F64, nnz - k0);
-    const int last = n - 1;
+    const Desc d = desc[blockIdx.x];
     const int tid = threadIdx.x;
-    int c[PER];
-#pragma unroll
-    for (int u = 0; u < PER; ++u) c[u] = __builtin_nontemporal_load(ci + k0 + min(tid + u * WG, last));
-    T v[PER];
-#pragma unroll
-    for (int u = 0; u < PER; ++u) v[u] = table[c[u]];
     double s = 0.0;
+    for (int base = d.k0; base < d.k1; base += SP_NNZ_F64) {
+        const int n = min(SP_NNZ_F64, d.k1 - base), last = n - 1;
+        int c[PER];
 #pragma unroll
-    for (int u = 0; u < PER; ++u)
-        if (tid + u * WG < n) s += (double)v[u];
+        for (int u = 0; u < PER; ++u) c[u] = __builtin_nontemporal_load(ci + base + min(tid + u * WG, last));
+        T v[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) v[u] = table[c[u]];
+#pragma unroll
+        for (int u = 0; u < PER; ++u)
+            if (tid + u * WG < n) s += (double)v[u];
+    }
     if (s != s) sink[0] = s;
 }
 
-extern "C" __global__ __launch_bounds__(WG) void sp_probe_replay_f64(const int* __restrict__ ci, i64 nnz,
+extern "C" __global__ __launch_bounds__(WG) void sp_probe_replay_f64(const Desc* __restrict__ desc,
+                                                                     const int* __restrict__ ci,
                                                                      const double* __restrict__ table,
                                                                      double* __restrict__ sink) {
-    replay_body<double>(ci, nnz, table, sink);
+    replay_body<double>(desc, ci, table, sink);
 }
 
-extern "C" __global__ __launch_bounds__(WG) void sp_probe_replay_f32(const int* __restrict__ ci, i64 nnz,
+extern "C" __global__ __launch_bounds__(WG) void sp_probe_replay_f32(const Desc* __restrict__ desc,
+                                                                     const int* __restrict__ ci,
                                                                      const float* __restrict__ table,
                                                                      double* __restrict__ sink) {
-    replay_body<float>(ci, nnz, table, sink);
+    replay_body<float>(desc, ci, table, sink);
 }
 
 // ---- batched plain iterations for the serving loop --------------------------------

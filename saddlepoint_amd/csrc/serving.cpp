@@ -192,6 +192,9 @@ class Server {
         // iteration limit is rounded up to a whole number of chunks.
         if (params.check_every != kChunk) throw Error(SP_ERR_INVALID, "serving needs check_every = 64");
         job->params.fused = 1;
+        // The batched kernels run each LP's plain plan, so its single-solve
+        // counterpart is the one without column panels.
+        job->params.column_panels = 0;
         // Limits past the largest multiple of 64 (u64::MAX, "no limit") become it.
         const uint64_t last = UINT64_MAX / kChunk * kChunk;
         job->params.max_iterations =

@@ -108,6 +108,10 @@ class Solver {
             }
             dev_ = b_.load(scaled_, params.spmv_autotune != 0);
         }
+        // Column panels for the long rows of a wide A' (the gathered x̄ far
+        // beyond an XCD's L2).
+        const int shift = params.column_panels >= 0 ? params.column_panels : (n_ >= (int64_t(1) << 21) ? 18 : 0);
+        if (shift) b_.set_panels(dev_.get(), shift);
         // The scaled matrices live on the device now; the host keeps the factors.
         scaled_.a = Csr{};
         scaled_.at = Csr{};

@@ -55,11 +55,13 @@ def scaled_pair(p):
     return so, sg
 
 
-def compare(name, p, gpu, tuning=None):
-    print(f"ops parity: {name} m={p.nrows} n={p.ncols} nnz={p.a.nnz} plan={tuning}", flush=True)
+def compare(name, p, gpu, tuning=None, panels=0):
+    print(f"ops parity: {name} m={p.nrows} n={p.ncols} nnz={p.a.nnz} plan={tuning} panels={panels}", flush=True)
     cpu, sp = scaled_pair(p)
     m, n = p.nrows, p.ncols
     pg = gpu.load(sp) if tuning is None else gpu.load(sp, tuning_a=tuning, tuning_at=tuning)
+    if panels:
+        gpu.set_panels(pg, panels)
     up, down = gpu.upload, gpu.download
     x, y, xa, ya = vector(n, 1), vector(m, 2), vector(n, 3), vector(m, 4)
     gx, gy, gxa, gya = up(x), up(y), up(xa), up(ya)
@@ -147,6 +149,12 @@ def test_backend_ops_match_cpu_power_law(gpu):
 def test_backend_ops_match_cpu_very_long_rows(gpu):
     # Two rows far longer than one workgroup's chunk (4,096 nonzeros), so they are
     # split across workgroups and combined by the finishing kernel.
+    p = very_long_rows()
+    assert np.diff(p.a.row_ptr).max() > 4096
+    compare("very-long-rows", p, gpu)
+
+
+def very_long_rows():
     rng = np.random.default_rng(21)
     m, n = 3000, 150_000
     cols = np.repeat(np.arange(n), 3)
@@ -156,9 +164,23 @@ def test_backend_ops_match_cpu_very_long_rows(gpu):
     r = np.concatenate([rows, long_rows])
     c = np.concatenate([cols, long_cols])
     a = generate._csr_from_rows_cols(m, n, r, c, generate._values(rng, r.size, -2.0, 2.0))
-    p = generate._finish(rng, a, eq_frac=0.6)
-    assert np.diff(p.a.row_ptr).max() > 4096
-    compare("very-long-rows", p, gpu)
+    return generate._finish(rng, a, eq_frac=0.6)
+
+
+@pytest.mark.parametrize("shift", [8, 12])
+def test_backend_ops_match_cpu_with_column_panels(gpu, shift):
+    # Column panels: the rows with more than 64 nonzeros gathered panel by
+    # panel (256 or 4,096 columns per panel) and their panel sums added in
+    # panel order; every A' product and step still matches the CPU backend.
+    compare("config1@0.1", generate.config1(scale=0.1), gpu, panels=shift)
+    compare("config2@0.02", generate.config2(scale=0.02), gpu, panels=shift)
+
+
+def test_backend_ops_match_cpu_with_wide_column_panels(gpu):
+    # Panels of 65,536 columns over rows of 140k and 70k nonzeros: a row's
+    # share of one panel spans several workgroups of the panel matrix, which
+    # its own finishing pass adds before the panel sums are added.
+    compare("very-long-rows", very_long_rows(), gpu, panels=16)
 
 
 @pytest.mark.parametrize("tuning", [(256, 256, 256), (512, 64, 1024), (1024, 1024, 37), (2048, 100, 1024)])

@@ -244,3 +244,27 @@ def test_fused_path_is_bitwise_the_method_path_with_long_rows(gpu):
     b = solve(gpu, p, default_params(fused=1, max_iterations=640))
     assert a.iterations == b.iterations == 640 and a.restarts == b.restarts
     assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
+
+
+def test_column_panels_solve_like_the_oracle(gpu):
+    """Column panels (forced on a small wide LP: 1,024 columns per panel) keep
+    the solve the oracle's: same status, iteration and restart counts over a
+    capped run, iterates within 1e-8; the fused path with panels is bitwise
+    the method path with panels (every A' product uses the one decomposition),
+    and mixed precision runs on the FP32 panel matrix."""
+    p = generate.config2(scale=0.02)
+    cap = 1280
+    c, cx, cy = O.solve(p, O.default_params(max_iterations=cap))
+    a = solve(gpu, p, default_params(max_iterations=cap, fused=0, column_panels=10))
+    b = solve(gpu, p, default_params(max_iterations=cap, fused=1, column_panels=10))
+    mx = solve(gpu, p, default_params(max_iterations=cap, precision=1, column_panels=10))
+    dx = np.max(np.abs(b.x - cx)) / (1 + np.max(np.abs(cx)))
+    dy = np.max(np.abs(b.y - cy)) / (1 + np.max(np.abs(cy)))
+    print(f"config2@0.02 panels: oracle {O.STATUS[c.status]} {c.iterations} it {c.restarts} restarts; gpu {b.status} "
+          f"{b.iterations} it {b.restarts} restarts; x diff {dx:.2e} y diff {dy:.2e}; mixed {mx.status} "
+          f"{mx.iterations} it", flush=True)
+    assert a.status == b.status and a.iterations == b.iterations and a.restarts == b.restarts
+    assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
+    assert b.status == O.STATUS[c.status] and b.iterations == c.iterations and b.restarts == c.restarts
+    assert dx < 1e-8 and dy < 1e-8
+    assert mx.iterations == cap and np.all(np.isfinite(mx.x))
