@@ -550,6 +550,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="config1",
                     help="the LP the iteration rate is measured on (config1: the headline)")
+    ap.add_argument("--panels", type=int, default=-1,
+                    help="sp_params.column_panels for the iteration bench (-1: the library's choice; 0: off)")
     ap.add_argument("--serving-sample", type=int, default=16,
                     help="LPs of the config-3 stream served beside the headline (0: none)")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
@@ -694,7 +696,7 @@ def iteration_bench(args) -> dict:
     b = HipBackend(0)
     log(f"backend: {b.name()}")
     t0 = time.time()
-    s = Solver(b, p, default_params(tolerance=1e-6, fused=1, precision=int(mixed)))
+    s = Solver(b, p, default_params(tolerance=1e-6, fused=1, precision=int(mixed), column_panels=args.panels))
     log(f"solver ready (scale + load + power iteration) in {time.time() - t0:.1f}s")
     warmup, steps = whole_windows(args.warmup), whole_windows(args.steps)
     s.advance(warmup)
@@ -787,6 +789,7 @@ def iteration_bench(args) -> dict:
         "dtype": "f32+f64" if mixed else "f64",
         "data": "synthetic (seeded generator; random instance of the named shape)",
         "config": {"workload": WORKLOADS[args.workload] + ("; mixed precision" if mixed else ""), "m": m, "n": n,
+                   "column_panels": args.panels,
                    "nnz": nnz, "scale": args.scale, "timed_window": f"iterations {it0 + 1}-{it0 + ran}, "
                    f"{ran // CHECK} whole check windows (checks included)", "parallelism": "single GPU"},
         "iteration_gbs": round(b_iter * ran / elapsed / 1e9, 1),

@@ -182,9 +182,12 @@ typedef struct {
                                    which the solve continues in FP64 (0, the default: no such switch) */
     int column_panels; /* A' rows with more than 64 nonzeros gather x̄ panel by panel, so that the
                           workgroups in flight share an L2-sized slice of it (sp_problem_set_panels):
-                          k in [8, 30]: panels of 2^k columns; 0: off; -1 (default): k = 18 when A' has
-                          at least 2^21 columns (a gathered vector of 16 MB or more), else off. Changes
-                          the order of the long rows' sums only. The serving loop runs without panels */
+                          k in [8, 30]: panels of 2^k columns; 0: off; -1 (default): k = 16 when A' has
+                          at least 2^21 columns (a gathered vector of 16 MB or more), else off. With
+                          panels the driver also relabels the rows, those long rows first and the
+                          others by their first column (y comes back in the LP's order). Changes the
+                          order of the long rows' sums and of sums over rows only. The serving loop
+                          runs without panels */
 } sp_params;
 
 /* Status (solver.rs:95-102). */

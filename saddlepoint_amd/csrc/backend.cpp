@@ -119,12 +119,10 @@ void spmv_launch(sp_backend* b, hipFunction_t k, hipFunction_t kfin, const DevCs
         const void* bva = f32 ? (const void*)B.va32 : (const void*)B.va;
         hipFunction_t kp = f32 ? b->k_spmv_panel32 : b->k_spmv_panel;
         hipFunction_t kpf = f32 ? b->k_fin_panel32 : b->k_fin_panel;
-        if (B.nblocks)
-            launch(b, kp, (unsigned)B.nblocks, B.desc, B.rp, B.ci, bva, xin, (void*)B.partials, (void*)a.psum, a.plong,
-                   a.npanels);
+        if (B.nblocks) launch(b, kp, (unsigned)B.nblocks, B.desc, B.rp, B.ci, bva, xin, (void*)B.partials, (void*)a.psum);
         if (B.nlong)
             launch(b, kpf, (unsigned)((B.nlong + 3) / 4), (const LongRow*)B.longs, B.nlong, (const void*)B.partials,
-                   (void*)a.psum, a.plong, a.npanels);
+                   (void*)a.psum);
         if (a.npdesc) launch(b, k, (unsigned)a.npdesc, a.pdesc, a.rp, a.ci, va, xin, (void*)a.partials, epi...);
         launch(b, kfin, (unsigned)((a.plong + 3) / 4), (const LongRow*)a.plongs, a.plong, (const void*)a.psum, epi...);
         return;
@@ -490,7 +488,7 @@ void Hip::set_panels(sp_problem* prob, int shift) {
         plan_csr(b, *B, brp.data(), SpmvTuning{});  // panel-major stream blocks (synchronizes)
         // Each long row's P panel sums, added in panel order by the finishing kernels.
         std::vector<LongRow> lr((size_t)L);
-        for (int l = 0; l < L; ++l) lr[(size_t)l] = LongRow{rows[(size_t)l], l * P, P, 0};
+        for (int l = 0; l < L; ++l) lr[(size_t)l] = LongRow{rows[(size_t)l], l, P, L};
         A.plongs = dupload(b->stream, lr.data(), lr.size());
         A.psum = dmalloc<double>((size_t)nq);
         // The short rows: the stream blocks of a plan whose long rows are the panel rows.
@@ -519,6 +517,66 @@ void Hip::set_panels(sp_problem* prob, int shift) {
     }
     for (int* q : scratch) dfree(q);
     free_csr(S);
+}
+
+void Hip::permute_rows(sp_problem* prob, const std::vector<int64_t>& order) {
+    if (!prob) throw Error(SP_ERR_INVALID, "permute_rows: null problem");
+    DevCsr& A = prob->a;
+    const int m = A.nrows;
+    if ((int64_t)order.size() != m) throw Error(SP_ERR_SHAPE, "permute_rows: order has the wrong length");
+    if (prob->has_fp32 || A.pmat) throw Error(SP_ERR_INVALID, "permute_rows: after FP32 copies or panels");
+    std::vector<int> ord32((size_t)m), rank((size_t)m, -1);
+    for (int i = 0; i < m; ++i) {
+        const int64_t o = order[(size_t)i];
+        if (o < 0 || o >= m || rank[(size_t)o] >= 0) throw Error(SP_ERR_INVALID, "permute_rows: not a permutation");
+        ord32[(size_t)i] = (int)o;
+        rank[(size_t)o] = i;
+    }
+    std::vector<int> rp_old((size_t)m + 1);
+    hip_check(hipMemcpyAsync(rp_old.data(), A.rp, sizeof(int) * rp_old.size(), hipMemcpyDeviceToHost, b->stream),
+              "hipMemcpyAsync D2H");
+    hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    std::vector<int64_t> rp_new((size_t)m + 1, 0);
+    for (int i = 0; i < m; ++i)
+        rp_new[(size_t)i + 1] = rp_new[(size_t)i] + rp_old[(size_t)ord32[(size_t)i] + 1] - rp_old[(size_t)ord32[(size_t)i]];
+    std::vector<int> rp_new32(rp_new.begin(), rp_new.end());
+    std::vector<void*> scratch;
+    int *drp = nullptr, *dci = nullptr;
+    double* dva = nullptr;
+    try {
+        int* dord = dupload(b->stream, ord32.data(), std::max<size_t>(ord32.size(), 1));
+        scratch.push_back(dord);
+        int* drank = dupload(b->stream, rank.data(), std::max<size_t>(rank.size(), 1));
+        scratch.push_back(drank);
+        drp = dupload(b->stream, rp_new32.data(), rp_new32.size());
+        dci = dmalloc<int>((size_t)std::max<int64_t>(A.nnz, 1));
+        dva = dmalloc<double>((size_t)std::max<int64_t>(A.nnz, 1));
+        if (m)
+            launch(b, b->k_rows_gather, (unsigned)(((int64_t)m * 64 + kWG - 1) / kWG), (const int*)A.rp,
+                   (const int*)A.ci, (const double*)A.va, (const int*)drp, (const int*)dord, m, dci, dva);
+        // The row-indexed vectors, and A'ᵀ's column indices (its rows keep their order).
+        for (double** v : {&prob->lc, &prob->uc, &prob->pf}) {
+            double* nv = dmalloc<double>((size_t)std::max(m, 1));
+            if (m) launch(b, b->k_gather, ew_grid(m), (const double*)*v, (const int*)dord, nv, (long long)m);
+            hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+            dfree(*v);
+            *v = nv;
+        }
+        if (prob->at.nnz) launch(b, b->k_remap, ew_grid(prob->at.nnz), prob->at.ci, (const int*)drank, (long long)prob->at.nnz);
+        hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    } catch (...) {
+        (void)hipStreamSynchronize(b->stream);
+        for (void* q : scratch) dfree(q);
+        dfree(drp), dfree(dci), dfree(dva);
+        throw;
+    }
+    for (void* q : scratch) dfree(q);
+    // A' takes the new arrays and a plan for its new row order.
+    const SpmvTuning t = A.tuning;
+    dfree(A.rp), dfree(A.ci), dfree(A.va), dfree(A.desc), dfree(A.partials), dfree(A.longs);
+    A.rp = drp, A.ci = dci, A.va = dva;
+    A.desc = nullptr, A.partials = nullptr, A.longs = nullptr;
+    plan_csr(b, A, rp_new.data(), t);
 }
 
 void Hip::upload(DevCsr& d, const int64_t* row_ptr, const int64_t* col_idx, const double* values, int64_t nrows,
@@ -1466,6 +1524,8 @@ sp_backend* sp_backend_open(int ordinal) {
         get(b->k_fin_panel, "sp_finish_panel_f64");
         get(b->k_spmv_panel32, "sp_spmv_panel_f32");
         get(b->k_fin_panel32, "sp_finish_panel_f32");
+        get(b->k_rows_gather, "sp_rows_gather");
+        get(b->k_gather, "sp_gather_f64");
         get(b->k_replay32, "sp_probe_replay_f32");
         get(b->k_spmv_t_primal_mixed, "sp_spmv_t_primal_mixed");
         get(b->k_spmv_dual_mixed, "sp_spmv_dual_mixed");

@@ -20,9 +20,10 @@ struct Desc {
 };
 
 // A row split into several chunks: the finishing kernel adds its chunk sums
-// partials[pbase .. pbase + nch). Mirrors `struct LongRow` in kernels.hip.
+// partials[pbase + q·stride], q < nch (stride 0 reads as 1). Mirrors
+// `struct LongRow` in kernels.hip.
 struct LongRow {
-    int row, pbase, nch, pad;
+    int row, pbase, nch, stride;
 };
 
 // How an SpMV is cut into work items (see plan_spmv). The autotuner searches
@@ -54,12 +55,13 @@ struct DevCsr {
     // nonzeros) are also kept as a panel matrix whose row q = p·L + l holds
     // long row l's nonzeros in column panel p (columns [p·2^pshift,
     // (p+1)·2^pshift)), stored and dispatched panel by panel; its sums land in
-    // psum[l·P + p] and the finishing table `plongs` adds each long row's P
-    // sums in panel order. The short rows run as the stream blocks `pdesc`.
+    // psum[q] and the finishing table `plongs` ({row, l, P, stride L}) adds
+    // each long row's P sums in panel order. The short rows run as the
+    // stream blocks `pdesc`.
     DevCsr* pmat = nullptr;
     int plong = 0, npanels = 0, pshift = 0;
     double* psum = nullptr;     // L·P panel sums (floats in mixed precision)
-    LongRow* plongs = nullptr;  // {row, l·P, P}
+    LongRow* plongs = nullptr;  // {row, l, P, L}
     Desc* pdesc = nullptr;      // the short rows' stream blocks
     int npdesc = 0;
 };
@@ -151,6 +153,7 @@ struct sp_backend {
         k_replay32;
     hipFunction_t k_spmv32, k_spmv_t_primal_mixed, k_spmv_dual_mixed, k_to_f32, k_delta_from, k_delta_to;
     hipFunction_t k_panel_count, k_panel_gather, k_spmv_panel, k_fin_panel, k_spmv_panel32, k_fin_panel32;
+    hipFunction_t k_rows_gather, k_gather;
     hipFunction_t k_dot, k_dist, k_primal_stats, k_dual_stats, k_step_dual, k_primal_ray, k_dual_ray;
 };
 
@@ -201,6 +204,10 @@ struct Hip {
     // panel (shift in [8, 30]); shift 0 drops them. A matrix without long rows
     // keeps its plain plan.
     void set_panels(sp_problem* p, int shift);
+    // Relabels the rows of a loaded problem (before its FP32 copy or panels
+    // exist): row i of A' becomes row order[i], the row-indexed vectors
+    // follow, and A'ᵀ's column indices are renamed in place.
+    void permute_rows(sp_problem* p, const std::vector<int64_t>& order);
     // Uploads a CSR matrix (int32 indices) and plans it, as load does.
     void upload(DevCsr& d, const int64_t* row_ptr, const int64_t* col_idx, const double* values, int64_t nrows,
                 int64_t ncols, int64_t nnz, const SpmvTuning& tuning);

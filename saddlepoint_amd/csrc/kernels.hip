@@ -265,8 +265,10 @@ __device__ __forceinline__ void spmv_body(const Desc* __restrict__ desc, const i
 // row's chunk sums in a fixed order and applies the epilogue. (Combining in
 // the SpMV itself would need an agent-scope release fence per workgroup, and
 // on this part, with one L2 per XCD, that is an L2 write-back each time.)
+// A row whose sum is split into nch partial sums: they are
+// partials[pbase + q·stride] for q < nch (stride 0 reads as 1).
 struct LongRow {
-    int row, pbase, nch, pad;
+    int row, pbase, nch, stride;
 };
 
 template <class A, class Epi>
@@ -275,8 +277,9 @@ __device__ __forceinline__ void finish_body(const LongRow* __restrict__ longs, i
     const int lane = threadIdx.x % WAVE;
     if (w >= nlong) return;  // uniform per wave
     const LongRow L = longs[w];
+    const i64 stride = L.stride ? L.stride : 1;
     A s = 0;
-    for (int q = lane; q < L.nch; q += WAVE) s += partials[L.pbase + q];
+    for (int q = lane; q < L.nch; q += WAVE) s += partials[L.pbase + q * stride];
     s = wave_sum(s);
     if (lane == 0) epi.apply(L.row, s, epi.load(L.row));
 }
@@ -295,18 +298,6 @@ struct EpiStore {
     struct Pre {};
     __device__ Pre load(int) const { return {}; }
     __device__ void apply(int r, A s, const Pre&) const { st(out + r, s); }
-};
-
-// The panel matrix's sums (below): its row q = p·L + l is long row l's
-// share in column panel p, and its sum goes to out[l·P + p], so that each
-// long row's panel sums sit together, in panel order, for the finishing pass.
-template <class A>
-struct EpiStorePanel {
-    A* out;
-    int L, P;
-    struct Pre {};
-    __device__ Pre load(int) const { return {}; }
-    __device__ void apply(int q, A s, const Pre&) const { out[(i64)(q % L) * P + q / L] = s; }
 };
 
 // spmv_t + primal_step: aty = A'ᵀy; x_t = clamp(x - τ(c - aty), lv, uv);
@@ -468,9 +459,9 @@ extern "C" __global__ __launch_bounds__(WG) void sp_finish_dual_halpern_f64(
 // sorted), and its rows are stored and dispatched panel by panel. The
 // workgroups in flight then all gather from one panel of x, a slice that fits
 // in L2. The panel matrix is an ordinary CSR matrix with its own SpMV plan;
-// its sums land in a buffer row by row (EpiStorePanel), and the long-row
-// finishing kernels add each long row's P panel sums in panel order and
-// apply the epilogue. The short rows run as stream blocks of the original
+// its row sums are stored in row order, so long row l's P panel sums sit at
+// l, l + L, ..., and the long-row finishing kernels (stride L) add them in
+// panel order and apply the epilogue. The short rows run as stream blocks of the original
 // matrix. Every A' product uses this one decomposition, so the summation
 // order is a function of the matrix and its plan, as before.
 
@@ -514,15 +505,37 @@ extern "C" __global__ __launch_bounds__(WG) void sp_panel_gather(const int* __re
     }
 }
 
-extern "C" __global__ __launch_bounds__(WG) void sp_spmv_panel_f64(SPMV_ARGS, double* __restrict__ out, int L,
-                                                                   int P) {
-    spmv_body<SP_NNZ_F64, double, double, double>(desc, rp, ci, va, xin, partials,
-                                                  EpiStorePanel<double>{out, L, P});
+extern "C" __global__ __launch_bounds__(WG) void sp_spmv_panel_f64(SPMV_ARGS, double* __restrict__ out) {
+    spmv_body<SP_NNZ_F64, double, double, double>(desc, rp, ci, va, xin, partials, EpiStore<double>{out});
 }
 
-extern "C" __global__ __launch_bounds__(WG) void sp_finish_panel_f64(FIN_ARGS(double), double* __restrict__ out,
-                                                                     int L, int P) {
-    finish_body<double>(longs, nlong, partials, EpiStorePanel<double>{out, L, P});
+extern "C" __global__ __launch_bounds__(WG) void sp_finish_panel_f64(FIN_ARGS(double), double* __restrict__ out) {
+    finish_body<double>(longs, nlong, partials, EpiStore<double>{out});
+}
+
+// Row relabeling: new row i is old row order[i], its nonzeros in their order
+// (one wave per row).
+extern "C" __global__ __launch_bounds__(WG) void sp_rows_gather(const int* __restrict__ rp_old,
+                                                                const int* __restrict__ ci_old,
+                                                                const double* __restrict__ va_old,
+                                                                const int* __restrict__ rp_new,
+                                                                const int* __restrict__ order, int m,
+                                                                int* __restrict__ ci_new,
+                                                                double* __restrict__ va_new) {
+    const int i = (int)(((i64)blockIdx.x * WG + threadIdx.x) / WAVE), lane = threadIdx.x % WAVE;
+    if (i >= m) return;  // uniform per wave
+    const int src = order[i], s0 = rp_old[src], d0 = rp_new[i], len = rp_new[i + 1] - d0;
+    for (int t = lane; t < len; t += WAVE) {
+        ci_new[d0 + t] = ci_old[s0 + t];
+        va_new[d0 + t] = va_old[s0 + t];
+    }
+}
+
+// out[i] = in[idx[i]].
+extern "C" __global__ __launch_bounds__(WG) void sp_gather_f64(const double* __restrict__ in,
+                                                               const int* __restrict__ idx,
+                                                               double* __restrict__ out, i64 n) {
+    for (i64 i = (i64)blockIdx.x * WG + threadIdx.x; i < n; i += (i64)gridDim.x * WG) out[i] = in[idx[i]];
 }
 
 // ---- mixed precision: FP32 matrix and displacements, FP64 anchors ------------
@@ -608,14 +621,12 @@ extern "C" __global__ __launch_bounds__(WG) void sp_finish_f32(FIN_ARGS(float), 
     finish_body<float>(longs, nlong, partials, EpiStore<float>{out});
 }
 
-extern "C" __global__ __launch_bounds__(WG) void sp_spmv_panel_f32(SPMV32_ARGS, float* __restrict__ out, int L,
-                                                                   int P) {
-    spmv_body<SP_NNZ_F64, float, float, float>(desc, rp, ci, va, xin, partials, EpiStorePanel<float>{out, L, P});
+extern "C" __global__ __launch_bounds__(WG) void sp_spmv_panel_f32(SPMV32_ARGS, float* __restrict__ out) {
+    spmv_body<SP_NNZ_F64, float, float, float>(desc, rp, ci, va, xin, partials, EpiStore<float>{out});
 }
 
-extern "C" __global__ __launch_bounds__(WG) void sp_finish_panel_f32(FIN_ARGS(float), float* __restrict__ out, int L,
-                                                                     int P) {
-    finish_body<float>(longs, nlong, partials, EpiStorePanel<float>{out, L, P});
+extern "C" __global__ __launch_bounds__(WG) void sp_finish_panel_f32(FIN_ARGS(float), float* __restrict__ out) {
+    finish_body<float>(longs, nlong, partials, EpiStore<float>{out});
 }
 
 extern "C" __global__ __launch_bounds__(WG) void sp_finish_t_primal_mixed(

@@ -226,6 +226,35 @@ ColumnOrder popularity_order(const LpView& p) {
     return popularity_order_of_counts(count);
 }
 
+RowOrder long_rows_first(const LpView& p, const ColumnOrder& cols, int64_t long_row) {
+    RowOrder o;
+    const int64_t m = p.m, n = p.n;
+    o.order.reserve((size_t)m);
+    // The short rows by key (smallest relabeled column; n for an empty row),
+    // a counting sort that keeps the index order among equal keys.
+    std::vector<int64_t> key((size_t)m, -1), start((size_t)n + 2, 0);
+    for (int64_t i = 0; i < m; ++i) {
+        const int64_t k0 = p.row_ptr[i], k1 = p.row_ptr[i + 1];
+        if (k1 - k0 > long_row) {
+            o.order.push_back(i);
+            continue;
+        }
+        int64_t kmin = n;
+        for (int64_t k = k0; k < k1; ++k) {
+            const int64_t j = cols.empty() ? p.col_idx[k] : cols.rank[(size_t)p.col_idx[k]];
+            kmin = std::min(kmin, j);
+        }
+        key[(size_t)i] = kmin;
+        start[(size_t)kmin + 1] += 1;
+    }
+    for (size_t b = 1; b < start.size(); ++b) start[b] += start[b - 1];
+    const size_t base = o.order.size();
+    o.order.resize((size_t)m);
+    for (int64_t i = 0; i < m; ++i)
+        if (key[(size_t)i] >= 0) o.order[base + (size_t)start[(size_t)key[(size_t)i]]++] = i;
+    return o;
+}
+
 ColumnOrder popularity_order_of_counts(const std::vector<int64_t>& count) {
     ColumnOrder o;
     const int64_t n = (int64_t)count.size();

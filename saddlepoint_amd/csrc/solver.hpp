@@ -109,9 +109,13 @@ class Solver {
             dev_ = b_.load(scaled_, params.spmv_autotune != 0);
         }
         // Column panels for the long rows of a wide A' (the gathered x̄ far
-        // beyond an XCD's L2).
-        const int shift = params.column_panels >= 0 ? params.column_panels : (n_ >= (int64_t(1) << 21) ? 18 : 0);
-        if (shift) b_.set_panels(dev_.get(), shift);
+        // beyond an XCD's L2), with the rows relabeled long rows first.
+        const int shift = params.column_panels >= 0 ? params.column_panels : (n_ >= (int64_t(1) << 21) ? 16 : 0);
+        if (shift) {
+            rows_ = long_rows_first(problem, cols_, kPanelMinRow);
+            b_.permute_rows(dev_.get(), rows_.order);
+            b_.set_panels(dev_.get(), shift);
+        }
         // The scaled matrices live on the device now; the host keeps the factors.
         scaled_.a = Csr{};
         scaled_.at = Csr{};
@@ -539,6 +543,11 @@ class Solver {
             std::vector<double> xs((size_t)n_), ys((size_t)m_);
             b_.download(x_t_.get(), xs.data(), n_);
             b_.download(y_t_.get(), ys.data(), m_);
+            if (!rows_.empty()) {
+                std::vector<double> yo((size_t)m_);
+                for (int64_t i = 0; i < m_; ++i) yo[(size_t)rows_.order[(size_t)i]] = ys[(size_t)i];
+                ys.swap(yo);
+            }
             const std::vector<double> x = scaled_.unscale_primal(xs);
             const std::vector<double> y = scaled_.unscale_dual(ys);
             if (x_out && cols_.empty()) std::copy(x.begin(), x.end(), x_out);
@@ -892,6 +901,7 @@ class Solver {
     const sp_params p_;
     ScaledProblem scaled_;
     ColumnOrder cols_;  // empty unless params.reorder_columns
+    RowOrder rows_;     // empty unless column panels are on
     int64_t m_, n_, nnz_;
     double c0_;
     bool maximize_;
