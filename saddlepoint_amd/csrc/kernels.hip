@@ -42,12 +42,12 @@ __device__ __forceinline__ T clampv(T v, T lo, T hi) {
     if (v > hi) v = hi;
     return v;
 }
-// Epilogue outputs are written with non-temporal stores: they stream out
-// during the kernel instead of sitting dirty in the XCD's L2 until the
-// kernel-end write-back.
+// Epilogue stores. Plain stores, like the plain loads of the matrix streams
+// below: on gfx950 the non-temporal hint on either made the config-1 SpMVs
+// slower (A side 51 -> 60 us, A'ᵀ side 55 -> 58 us) and config 2 no faster.
 template <class T>
 __device__ __forceinline__ void st(T* p, T v) {
-    __builtin_nontemporal_store(v, p);
+    *p = v;
 }
 
 __device__ __forceinline__ double excess(double v, double lo, double hi) {
@@ -144,9 +144,9 @@ __device__ __forceinline__ void spmv_block(const Desc d, const int* __restrict__
         V v[PER];
         if (n > 0) {
 #pragma unroll
-            for (int u = 0; u < PER; ++u) c[u] = __builtin_nontemporal_load(ci + kb + min(tid + u * WG, last));
+            for (int u = 0; u < PER; ++u) c[u] = ci[kb + min(tid + u * WG, last)];
 #pragma unroll
-            for (int u = 0; u < PER; ++u) v[u] = __builtin_nontemporal_load(va + kb + min(tid + u * WG, last));
+            for (int u = 0; u < PER; ++u) v[u] = va[kb + min(tid + u * WG, last)];
         }
         int re[SP_RPT];
         typename Epi::Pre pre[SP_RPT];
@@ -231,9 +231,9 @@ __device__ __forceinline__ void spmv_block(const Desc d, const int* __restrict__
             int c[PERL];
             V v[PERL];
 #pragma unroll
-            for (int u = 0; u < PERL; ++u) c[u] = __builtin_nontemporal_load(ci + base + min(tid + u * WG, lastl));
+            for (int u = 0; u < PERL; ++u) c[u] = ci[base + min(tid + u * WG, lastl)];
 #pragma unroll
-            for (int u = 0; u < PERL; ++u) v[u] = __builtin_nontemporal_load(va + base + min(tid + u * WG, lastl));
+            for (int u = 0; u < PERL; ++u) v[u] = va[base + min(tid + u * WG, lastl)];
             X xv[PERL];
 #pragma unroll
             for (int u = 0; u < PERL; ++u) xv[u] = x[c[u]];
@@ -1297,7 +1297,7 @@ __device__ __forceinline__ void replay_body(const Desc* __restrict__ desc, const
         const int n = min(SP_NNZ_F64, d.k1 - base), last = n - 1;
         int c[PER];
 #pragma unroll
-        for (int u = 0; u < PER; ++u) c[u] = __builtin_nontemporal_load(ci + base + min(tid + u * WG, last));
+        for (int u = 0; u < PER; ++u) c[u] = ci[base + min(tid + u * WG, last)];
         T v[PER];
 #pragma unroll
         for (int u = 0; u < PER; ++u) v[u] = table[c[u]];
