@@ -295,7 +295,10 @@ def stream_bench(args) -> dict:
 
     T, K = args.stream, args.stream_backlog
     t0 = time.time()
-    pool = generate.serving_stream(3, args.stream_pool)
+    pool = []
+    for start_i in range(0, args.stream_pool, 50):
+        pool += generate.serving_stream(3, min(50, args.stream_pool - start_i), start=start_i)
+        log(f"  generated {len(pool)} LPs")
     tols = [1e-4 if i % 2 == 0 else 1e-6 for i in range(len(pool))]
     log(f"stream: {len(pool)} config-3 LPs generated in {time.time() - t0:.1f}s; window {T}s, backlog {K}")
     b = HipBackend(0)
@@ -311,10 +314,13 @@ def stream_bench(args) -> dict:
     # One at a time. LP i joins the backlog when LP i - K completes (at the
     # start for the first K), so its latency counts its wait in the backlog.
     start = time.perf_counter()
-    done, lat, i, ends = 0, [], 0, []
+    done, lat, i, ends, last_report = 0, [], 0, [], time.time()
     while time.perf_counter() - start < T and i < len(pool):
         solve(b, pool[i], default_params(tolerance=tols[i]))
         s1 = time.perf_counter()
+        if time.time() - last_report > 30:
+            log(f"  one at a time: {s1 - start:.0f}s, {i + 1} solved")
+            last_report = time.time()
         ends.append(s1)
         if s1 - start >= T / 4 and s1 - start < T:
             done += 1

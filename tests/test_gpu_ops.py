@@ -254,3 +254,30 @@ def test_int32_limit_is_reported(gpu):
     rc = lib().sp_problem_load(gpu.h, C.byref(v), C.byref(h))
     assert rc == SP_ERR_TOO_LARGE and not h.value
     assert "32-bit" in lib().sp_last_error(gpu.h).decode()
+
+
+@pytest.mark.parametrize("kind,per", [("copy", 16), ("triad", 24), ("copy_f32", 8), ("triad_f32", 12)])
+def test_bandwidth_probes_report_plausible_rates(gpu, kind, per):
+    """The copy and triad probes in f64 and f32 (the counterpart of the
+    reference's probe.rs:38-41) time a kernel that moves `per` bytes per
+    element: a finite rate above zero and below the 8 TB/s HBM peak."""
+    n = 1 << 26
+    ms = gpu.probe(kind, n, reps=5)
+    gbs = per * n / (ms * 1e-3) / 1e9
+    print(f"{kind}: {ms:.4f} ms per launch, {gbs:.0f} GB/s", flush=True)
+    assert np.isfinite(ms) and ms > 0.0
+    assert 0.0 < gbs <= 8000.0
+
+
+def test_gather_replay_is_no_slower_than_the_spmv(gpu):
+    """The gather replay (the matrix's own index stream, gathers only) is the
+    ceiling of its SpMV's gathers: it takes no longer than the SpMV itself."""
+    from saddlepoint_amd.autotune import structural
+
+    pg = gpu.load(structural(generate.config1(scale=0.25)))
+    for transpose in (False, True):
+        replay = gpu.gather_replay_time(pg, transpose=transpose, reps=10)
+        spmv = gpu.spmv_time(pg, transpose=transpose, reps=10)
+        print(f"transpose={transpose}: replay {replay * 1e3:.1f} us, spmv {spmv * 1e3:.1f} us", flush=True)
+        assert 0.0 < replay <= spmv
+    pg.free()
