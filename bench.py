@@ -283,12 +283,14 @@ def serving_bench(args) -> dict:
 
 def stream_bench(args) -> dict:
     """Steady state of a serving stream: each mode takes LPs from the same
-    config-3 sequence (instance 0, 1, 2, ...), keeps `--stream-backlog` of
-    them submitted and waiting at all times (a saturated server: work is
-    always there), and runs for `--stream` seconds. Completions per second are
-    counted after the first quarter of the window (warm-up) and latencies
-    (completion - submission) taken of the LPs completed in the rest.
-    One at a time solves the same sequence back to back in plain FP64."""
+    config-3 sequence (instance 0, 1, 2, ...) and keeps `--stream-backlog` of
+    them in the system at all times (a closed loop: the next LP enters when
+    one completes, so work is always waiting), for `--stream` seconds.
+    Completions per second are counted after the first quarter of the window
+    (warm-up) and latencies (completion - entry) taken of the LPs completed in
+    the rest. One at a time solves the same sequence back to back in plain
+    FP64 with the same number in the system (LP i enters when LP i - K
+    completes)."""
     from saddlepoint_amd import generate
     from saddlepoint_amd.backend import HipBackend, default_params, solve
     from saddlepoint_amd.serving import DONE, FAILED, Server
@@ -336,13 +338,12 @@ def stream_bench(args) -> dict:
         done, lat, last_report = 0, [], time.time()
         pending = set()
         while srv.clock() - start < T:
-            _, waiting, _ = srv.run_chunk() if pending else (0, 0, 0)
-            while waiting < K and i < len(pool):
+            while len(pending) < K and i < len(pool):
                 j = srv.submit(pool[i], default_params(tolerance=tols[i], precision=pr))
                 sub[j] = srv.clock()
                 pending.add(j)
-                waiting += 1
                 i += 1
+            srv.run_chunk()
             for j in [j for j in pending if srv.state(j) in (DONE, FAILED)]:
                 pending.discard(j)
                 r = srv.result(j, with_solution=False)
@@ -577,7 +578,7 @@ def main() -> int:
                     help="config 2's SpMVs on one GPU's share of a P-way row-block split (P = this value)")
     ap.add_argument("--stream", type=float, default=0.0,
                     help="steady-state serving over a window of this many seconds (saturated, per mode)")
-    ap.add_argument("--stream-backlog", type=int, default=64, help="--stream: LPs kept submitted and waiting")
+    ap.add_argument("--stream-backlog", type=int, default=64, help="--stream: LPs kept in the system (closed loop)")
     ap.add_argument("--stream-pool", type=int, default=600, help="--stream: LPs of the sequence generated up front")
     ap.add_argument("--serving", action="store_true",
                     help="serving stream instead: LPs/s and latency, served (FP64 and mixed) vs one at a time")

@@ -74,6 +74,8 @@ typedef struct {
 /* CudaBackend::new(ordinal) (cuda/src/backend.rs:129-140). Loads the gfx950
  * code object that sits next to this library. */
 int sp_backend_create(int ordinal, sp_backend **out);
+/* Free every solver and server created on `b` first: they use its stream and
+ * kernels until they are freed. */
 void sp_backend_destroy(sp_backend *b);
 /* Backend::name (backend.rs:48). The string lives as long as the backend. */
 const char *sp_backend_name(const sp_backend *b);

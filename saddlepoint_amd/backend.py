@@ -9,6 +9,7 @@ Errors raise :class:`BackendError`; there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -184,9 +185,18 @@ class HipBackend:
         if rc != SP_OK:
             raise BackendError(rc, lib().sp_last_error(None).decode())
         self.h = h
+        # Servers and solvers hold the C backend's pointer (its stream, its
+        # kernels): they are closed before it, also when one outlives a
+        # failed caller (a traceback keeping it alive past close()).
+        self._users = weakref.WeakSet()
+
+    def _adopt(self, user) -> None:
+        self._users.add(user)
 
     def close(self):
         if getattr(self, "h", None):
+            for u in list(getattr(self, "_users", ())):
+                u.close()
             lib().sp_backend_destroy(self.h)
             self.h = None
 
@@ -427,6 +437,7 @@ class Solver:
         h = C.c_void_p()
         backend._ok(lib().sp_solver_create(backend.h, C.byref(v), C.byref(self.params), C.byref(h)))
         self.h = h
+        backend._adopt(self)
 
     def advance(self, iterations: int) -> bool:
         done = C.c_int()

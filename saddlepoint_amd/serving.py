@@ -51,6 +51,7 @@ class Server:
         backend._ok(lib().sp_server_create(backend.h, C.byref(p), C.byref(h)))
         self.h = h
         self.shapes = {}
+        backend._adopt(self)
 
     def close(self):
         if getattr(self, "h", None):
@@ -58,6 +59,12 @@ class Server:
             self.h = None
 
     def __del__(self):
+        self.close()
+
+    def __enter__(self) -> "Server":
+        return self
+
+    def __exit__(self, *exc) -> None:
         self.close()
 
     def clock(self) -> float:
