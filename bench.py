@@ -282,84 +282,114 @@ def serving_bench(args) -> dict:
 
 
 def stream_bench(args) -> dict:
-    """Steady state of a serving stream: each mode takes LPs from the same
-    config-3 sequence (instance 0, 1, 2, ...) and keeps `--stream-backlog` of
-    them in the system at all times (a closed loop: the next LP enters when
-    one completes, so work is always waiting), for `--stream` seconds.
-    Completions per second are counted after the first quarter of the window
-    (warm-up) and latencies (completion - entry) taken of the LPs completed in
-    the rest. One at a time solves the same sequence back to back in plain
-    FP64 with the same number in the system (LP i enters when LP i - K
-    completes)."""
+    """Steady state of a serving stream over a window of `--stream` seconds
+    per mode, on the same config-3 sequence (instance 0, 1, 2, ...).
+
+    Arrivals are either a closed loop (`--stream-rate 0`: `--stream-backlog`
+    LPs are in the system at all times, the next entering when one completes,
+    so work is always waiting: the mode's capacity) or Poisson at
+    `--stream-rate` LPs/s (seeded exponential gaps, the same arrival times in
+    every mode). Completions per second are counted over the last three
+    quarters of the window (the first quarter is warm-up) and latencies
+    (completion - arrival) taken of the LPs completed in them; LPs that have
+    arrived but not completed at the window's end are reported, since an
+    overloaded mode's latencies of completed LPs leave them out. One at a time
+    solves the sequence back to back in plain FP64, in arrival order."""
     from saddlepoint_amd import generate
     from saddlepoint_amd.backend import HipBackend, default_params, solve
     from saddlepoint_amd.serving import DONE, FAILED, Server
 
-    T, K = args.stream, args.stream_backlog
+    T, K, rate = args.stream, args.stream_backlog, args.stream_rate
     t0 = time.time()
     pool = []
     for start_i in range(0, args.stream_pool, 50):
         pool += generate.serving_stream(3, min(50, args.stream_pool - start_i), start=start_i)
         log(f"  generated {len(pool)} LPs")
     tols = [1e-4 if i % 2 == 0 else 1e-6 for i in range(len(pool))]
-    log(f"stream: {len(pool)} config-3 LPs generated in {time.time() - t0:.1f}s; window {T}s, backlog {K}")
+    arrivals = (np.cumsum(np.random.default_rng(2024).exponential(1.0 / rate, len(pool))) if rate > 0 else None)
+    log(f"stream: {len(pool)} config-3 LPs generated in {time.time() - t0:.1f}s; window {T}s, "
+        + (f"Poisson arrivals at {rate} LPs/s" if rate > 0 else f"closed loop of {K}"))
     b = HipBackend(0)
-    out = {"window_s": T, "warmup_s": T / 4, "backlog": K, "pool": len(pool),
+    out = {"window_s": T, "warmup_s": T / 4, "pool": len(pool),
+           "arrivals": f"poisson {rate} LPs/s (seed 2024)" if rate > 0 else f"closed loop, {K} in the system",
            "workload": "config-3 stream (rows log-uniform 1k-200k, columns 1.5-3x rows, 4-30 nonzeros per column, "
                        "tolerance 50/50 1e-4 / 1e-6), instances taken in order"}
 
-    def summary(done, lat, taken):
-        rate = done / (0.75 * T)
-        return {"lps_per_s": rate, "completed_in_window": done, "taken": taken, "p50_s": pct(lat, 50),
-                "p99_s": pct(lat, 99)}
+    def summary(done, lat, taken, arrived, open_at_end):
+        return {"lps_per_s": done / (0.75 * T), "completed_in_window": done, "taken": taken,
+                "arrived_by_end": arrived, "in_system_at_end": open_at_end, "pool_exhausted": taken >= len(pool),
+                "p50_s": pct(lat, 50), "p99_s": pct(lat, 99)}
 
-    # One at a time. LP i joins the backlog when LP i - K completes (at the
-    # start for the first K), so its latency counts its wait in the backlog.
+    def arrived_by(t):
+        return int(np.searchsorted(arrivals, t, side="right")) if rate > 0 else None
+
+    # One at a time. Closed loop: LP i enters when LP i - K completes (at the
+    # start for the first K). Poisson: LP i enters at its arrival time; the
+    # GPU idles while nothing has arrived.
     start = time.perf_counter()
     done, lat, i, ends, last_report = 0, [], 0, [], time.time()
-    while time.perf_counter() - start < T and i < len(pool):
+    while i < len(pool):
+        now = time.perf_counter() - start
+        if now >= T:
+            break
+        if rate > 0 and arrivals[i] > now:
+            time.sleep(min(arrivals[i], T) - now)
+            continue
         solve(b, pool[i], default_params(tolerance=tols[i]))
-        s1 = time.perf_counter()
+        s1 = time.perf_counter() - start
         if time.time() - last_report > 30:
-            log(f"  one at a time: {s1 - start:.0f}s, {i + 1} solved")
+            log(f"  one at a time: {s1:.0f}s, {i + 1} solved")
             last_report = time.time()
         ends.append(s1)
-        if s1 - start >= T / 4 and s1 - start < T:
+        if T / 4 <= s1 < T:
             done += 1
-            lat.append(s1 - (ends[i - K] if i >= K else start))
+            entered = arrivals[i] if rate > 0 else (ends[i - K] if i >= K else 0.0)
+            lat.append(s1 - entered)
         i += 1
-    out["one_at_a_time_fp64"] = summary(done, lat, i)
+    arrived = arrived_by(T) if rate > 0 else i + K
+    out["one_at_a_time_fp64"] = summary(done, lat, i, arrived, max(arrived - i, 0) if rate > 0 else K)
     log(f"  one at a time: {out['one_at_a_time_fp64']['lps_per_s']:.3f} LPs/s ({i} taken)")
     for prec in ("fp64", "mixed"):
         srv = Server(b)
         pr = int(prec == "mixed")
-        sub, ids, i = {}, [], 0
+        entered, i = {}, 0
         start = srv.clock()
         done, lat, last_report = 0, [], time.time()
         pending = set()
-        while srv.clock() - start < T:
-            while len(pending) < K and i < len(pool):
+        while True:
+            now = srv.clock() - start
+            if now >= T:
+                break
+            while i < len(pool) and (len(pending) < K if rate <= 0 else arrivals[i] <= now):
                 j = srv.submit(pool[i], default_params(tolerance=tols[i], precision=pr))
-                sub[j] = srv.clock()
+                entered[j] = start + (arrivals[i] if rate > 0 else now)
                 pending.add(j)
                 i += 1
-            srv.run_chunk()
+            if pending:
+                srv.run_chunk()
+            elif i < len(pool) and rate > 0:
+                time.sleep(max(min(arrivals[i], T) - now, 0.0))
+                continue
+            else:
+                break
             for j in [j for j in pending if srv.state(j) in (DONE, FAILED)]:
                 pending.discard(j)
                 r = srv.result(j, with_solution=False)
                 t_rel = r.completed - start
                 if T / 4 <= t_rel < T:
                     done += 1
-                    lat.append(r.completed - sub[j])
+                    lat.append(r.completed - entered[j])
                 srv.forget(j)
             if time.time() - last_report > 30:
                 log(f"  served {prec}: {srv.clock() - start:.0f}s, {i} taken, {len(pending)} in the server")
                 last_report = time.time()
         srv.close()
-        out["served_" + prec] = summary(done, lat, i)
-        out["served_" + prec]["speedup_vs_one_at_a_time"] = (out["served_" + prec]["lps_per_s"] /
-                                                            max(out["one_at_a_time_fp64"]["lps_per_s"], 1e-12))
-        log(f"  served {prec}: {out['served_' + prec]['lps_per_s']:.3f} LPs/s ({i} taken)")
+        arrived = arrived_by(T) if rate > 0 else i
+        key = "served_" + prec
+        out[key] = summary(done, lat, i, arrived, len(pending) + (arrived - i if rate > 0 else 0))
+        out[key]["speedup_vs_one_at_a_time"] = out[key]["lps_per_s"] / max(out["one_at_a_time_fp64"]["lps_per_s"],
+                                                                          1e-12)
+        log(f"  served {prec}: {out[key]['lps_per_s']:.3f} LPs/s ({i} taken)")
     b.close()
     return out
 
@@ -579,6 +609,8 @@ def main() -> int:
     ap.add_argument("--stream", type=float, default=0.0,
                     help="steady-state serving over a window of this many seconds (saturated, per mode)")
     ap.add_argument("--stream-backlog", type=int, default=64, help="--stream: LPs kept in the system (closed loop)")
+    ap.add_argument("--stream-rate", type=float, default=0.0,
+                    help="--stream: Poisson arrivals at this many LPs/s instead of the closed loop")
     ap.add_argument("--stream-pool", type=int, default=600, help="--stream: LPs of the sequence generated up front")
     ap.add_argument("--serving", action="store_true",
                     help="serving stream instead: LPs/s and latency, served (FP64 and mixed) vs one at a time")

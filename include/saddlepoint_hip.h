@@ -396,6 +396,14 @@ int sp_scale_problem(const sp_lp_view *lp, const sp_params *params, double *a_va
 int sp_scale_problem_gpu(sp_backend *b, const sp_lp_view *lp, const sp_params *params, double *a_values,
                          double *at_values, double *c, double *lc, double *uc, double *lv, double *uv,
                          double *row_scale, double *col_scale, double *bound_scale, double *obj_scale);
+/* ScaledProblem::unscale_primal (scaling.rs:114-116) on the device:
+ * out = xs · col_scale / bound_scale entry by entry (the product, then a
+ * correctly rounded division: bitwise the host's). All three vectors have the
+ * LP's n entries. sp_solve and the solvers unscale this way from 2^20 entries
+ * of x and y together. */
+int sp_unscale_primal(sp_backend *b, const sp_vec *xs, const sp_vec *col_scale, double bound_scale, sp_vec *out);
+/* ScaledProblem::unscale_dual (scaling.rs:119-121): out = ys · row_scale / obj_scale. */
+int sp_unscale_dual(sp_backend *b, const sp_vec *ys, const sp_vec *row_scale, double obj_scale, sp_vec *out);
 
 #ifdef __cplusplus
 }

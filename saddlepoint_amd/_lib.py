@@ -127,6 +127,8 @@ SIGNATURES = {
     "sp_norm_sq": (C.c_int, [_vp, _vp, _pd]),
     "sp_dot": (C.c_int, [_vp, _vp, _vp, _pd]),
     "sp_scale": (C.c_int, [_vp, _vp, _d]),
+    "sp_unscale_primal": (C.c_int, [_vp, _vp, _vp, _d, _vp]),
+    "sp_unscale_dual": (C.c_int, [_vp, _vp, _vp, _d, _vp]),
     "sp_primal_stats": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(PrimalStats)]),
     "sp_dual_stats": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(DualStats)]),
     "sp_dual_stats_noalloc": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(DualStats)]),

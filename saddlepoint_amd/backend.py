@@ -328,6 +328,14 @@ class HipBackend:
     def scale(self, v: Vector, alpha: float) -> None:
         self._ok(lib().sp_scale(self.h, v.h, float(alpha)))
 
+    def unscale_primal(self, xs: Vector, col_scale: Vector, bound_scale: float, out: Vector) -> None:
+        """``ScaledProblem::unscale_primal`` (scaling.rs:114-116) on the device."""
+        self._ok(lib().sp_unscale_primal(self.h, xs.h, col_scale.h, float(bound_scale), out.h))
+
+    def unscale_dual(self, ys: Vector, row_scale: Vector, obj_scale: float, out: Vector) -> None:
+        """``ScaledProblem::unscale_dual`` (scaling.rs:119-121) on the device."""
+        self._ok(lib().sp_unscale_dual(self.h, ys.h, row_scale.h, float(obj_scale), out.h))
+
     def primal_stats(self, p: Problem, x: Vector, ax: Vector) -> PrimalStats:
         out = PrimalStats()
         self._ok(lib().sp_primal_stats(self.h, p.h, x.h, ax.h, C.byref(out)))

@@ -1197,6 +1197,53 @@ void Hip::sub(const sp_vec* a, const sp_vec* bv, sp_vec* out) {
         launch(b, b->k_sub, ew_grid(a->len), (const double*)a->d, (const double*)bv->d, out->d, (long long)a->len);
 }
 
+void Hip::unscale(const sp_vec* v, const double* f, int64_t flen, double s, const int* to, bool dual, sp_vec* out) {
+    if (!v || (v->len && !f)) throw Error(SP_ERR_INVALID, "unscale: null argument");
+    if (flen != v->len) throw Error(SP_ERR_SHAPE, "unscale: factors of the wrong length");
+    need_len(out, v->len, "unscale out");
+    if (v->len)
+        launch(b, dual ? b->k_unscale_dual : b->k_unscale_primal, ew_grid(v->len), (const double*)v->d, f, s, to,
+               out->d, (long long)v->len);
+}
+
+void Hip::unscale_to_host(const sp_vec* v, const std::vector<double>& f, double s, const std::vector<int64_t>* order,
+                          bool dual, double* out) {
+    if (!v) throw Error(SP_ERR_INVALID, "unscale: null vector");
+    const int64_t n = v->len;
+    if ((int64_t)f.size() != n) throw Error(SP_ERR_SHAPE, "unscale: factors of the wrong length");
+    const bool relabeled = order && !order->empty();
+    if (relabeled && (int64_t)order->size() != n) throw Error(SP_ERR_SHAPE, "unscale: relabeling of the wrong length");
+    if (n == 0) return;
+    std::vector<int> to32;
+    if (relabeled) {
+        to32.resize((size_t)n);
+        for (int64_t q = 0; q < n; ++q) {
+            const int64_t t = (*order)[(size_t)q];
+            if (t < 0 || t >= n) throw Error(SP_ERR_INVALID, "unscale: relabeling out of range");
+            to32[(size_t)q] = (int)t;
+        }
+    }
+    double* fd = nullptr;
+    int* td = nullptr;
+    sp_vec* o = nullptr;
+    try {
+        fd = dupload(b->stream, f.data(), f.size());
+        if (relabeled) td = dupload(b->stream, to32.data(), to32.size());
+        o = zeros(n);
+        unscale(v, fd, n, s, td, dual, o);
+        download(o, out, n);
+    } catch (...) {
+        (void)hipStreamSynchronize(b->stream);
+        dfree(fd);
+        dfree(td);
+        free_vec(o);
+        throw;
+    }
+    dfree(fd);
+    dfree(td);
+    free_vec(o);
+}
+
 void Hip::scale_inv_sqrt(sp_vec* v, const double* ns) {
     if (!v || !ns) throw Error(SP_ERR_INVALID, "scale_inv_sqrt: null argument");
     if (v->len) launch(b, b->k_scale_inv_sqrt, ew_grid(v->len), v->d, ns, (long long)v->len);
@@ -1511,6 +1558,8 @@ sp_backend* sp_backend_open(int ordinal) {
         get(b->k_sub, "sp_sub_f64");
         get(b->k_scale, "sp_scale_f64");
         get(b->k_fill, "sp_fill_f64");
+        get(b->k_unscale_primal, "sp_unscale_primal_f64");
+        get(b->k_unscale_dual, "sp_unscale_dual_f64");
         get(b->k_spmv32, "sp_spmv_f32");
         get(b->k_probe_copy, "sp_probe_copy");
         get(b->k_probe_triad, "sp_probe_triad");

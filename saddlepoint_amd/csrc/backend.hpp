@@ -148,7 +148,7 @@ struct sp_backend {
     hipFunction_t k_tr_rows, k_tr_hist, k_scan_tiles, k_scan_add, k_tr_scatter, k_tr_ptr, k_tr_gather, k_tr_iota,
         k_remap, k_col_count, k_scale_inv_sqrt;
     hipFunction_t k_spmv, k_spmv_t_primal, k_spmv_t_primal_halpern, k_spmv_dual, k_spmv_dual_halpern;
-    hipFunction_t k_primal_step, k_dual_step, k_halpern, k_sub, k_scale, k_fill;
+    hipFunction_t k_primal_step, k_dual_step, k_halpern, k_sub, k_scale, k_fill, k_unscale_primal, k_unscale_dual;
     hipFunction_t k_probe_copy, k_probe_triad, k_probe_gather, k_probe_copy32, k_probe_triad32, k_replay64,
         k_replay32;
     hipFunction_t k_spmv32, k_spmv_t_primal_mixed, k_spmv_dual_mixed, k_to_f32, k_delta_from, k_delta_to;
@@ -228,6 +228,15 @@ struct Hip {
     void halpern_step(sp_vec* z, const sp_vec* z_t, const sp_vec* anchor, double w, double rho);
     double distance_sq(const sp_vec* a, const sp_vec* bv);
     void sub(const sp_vec* a, const sp_vec* bv, sp_vec* out);
+    // ScaledProblem::unscale_primal / unscale_dual (scaling.rs:114-121) on the
+    // device: out = v · f / s entry by entry. With `to` (device int32, v's
+    // length) entry q goes to out[to[q]], and the dual one takes f[to[q]].
+    void unscale(const sp_vec* v, const double* f, int64_t flen, double s, const int* to, bool dual, sp_vec* out);
+    // The same into host memory, for a vector in a relabeled order: the
+    // factors and the relabeling (`order`, may be null or empty) are uploaded,
+    // and the result comes back in the LP's order.
+    void unscale_to_host(const sp_vec* v, const std::vector<double>& f, double s, const std::vector<int64_t>* order,
+                         bool dual, double* out);
     double norm_sq(const sp_vec* v);
     double dot(const sp_vec* a, const sp_vec* bv);
     void scale(sp_vec* v, double alpha);

@@ -187,6 +187,20 @@ int sp_sub(sp_backend* b, const sp_vec* a, const sp_vec* bv, sp_vec* out) {
     return guarded(b, [&] { sp::Hip{b}.sub(a, bv, out); });
 }
 
+int sp_unscale_primal(sp_backend* b, const sp_vec* xs, const sp_vec* col_scale, double bound_scale, sp_vec* out) {
+    return guarded(b, [&] {
+        if (!xs || !col_scale) throw sp::Error(SP_ERR_INVALID, "unscale_primal: null vector");
+        sp::Hip{b}.unscale(xs, col_scale->d, col_scale->len, bound_scale, nullptr, false, out);
+    });
+}
+
+int sp_unscale_dual(sp_backend* b, const sp_vec* ys, const sp_vec* row_scale, double obj_scale, sp_vec* out) {
+    return guarded(b, [&] {
+        if (!ys || !row_scale) throw sp::Error(SP_ERR_INVALID, "unscale_dual: null vector");
+        sp::Hip{b}.unscale(ys, row_scale->d, row_scale->len, obj_scale, nullptr, true, out);
+    });
+}
+
 int sp_norm_sq(sp_backend* b, const sp_vec* v, double* out) {
     return guarded(b, [&] {
         if (!out) throw sp::Error(SP_ERR_INVALID, "null output");

@@ -703,6 +703,30 @@ extern "C" __global__ __launch_bounds__(WG) void sp_fill_f64(double* __restrict_
     GRID_LOOP(i, n) { v[i] = value; }
 }
 
+// ---- unscaling (scaling.rs:114-121) -----------------------------------------
+//
+// x = x_s · col_scale / bound_scale and y = y_s · row_scale / obj_scale, the
+// product first and then the (correctly rounded) division, as on the host.
+// With `to`, the result lands in the LP's own order: entry q of a relabeled
+// vector goes to to[q]. The primal factors are in the relabeled order (they
+// were computed on the relabeled matrix), the dual factors in the LP's own.
+extern "C" __global__ __launch_bounds__(WG) void sp_unscale_primal_f64(const double* __restrict__ v,
+                                                                       const double* __restrict__ f, double s,
+                                                                       const int* __restrict__ to,
+                                                                       double* __restrict__ out, i64 n) {
+    GRID_LOOP(q, n) { out[to ? to[q] : q] = v[q] * f[q] / s; }
+}
+
+extern "C" __global__ __launch_bounds__(WG) void sp_unscale_dual_f64(const double* __restrict__ v,
+                                                                     const double* __restrict__ f, double s,
+                                                                     const int* __restrict__ to,
+                                                                     double* __restrict__ out, i64 m) {
+    GRID_LOOP(i, m) {
+        const i64 d = to ? to[i] : i;
+        out[d] = v[i] * f[d] / s;
+    }
+}
+
 // Precision conversions (round to nearest; ±inf and NaN carry over).
 extern "C" __global__ __launch_bounds__(WG) void sp_f64_to_f32(const double* __restrict__ a, float* __restrict__ b,
                                                                i64 n) {

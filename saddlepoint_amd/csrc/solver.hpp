@@ -539,7 +539,12 @@ class Solver {
     // solver.rs:422-437. Before termination, status is -1 and the figures are
     // those of the last check.
     sp_solution finish(double* x_out, double* y_out) {
-        if (x_out || y_out) {
+        if ((x_out || y_out) && n_ + m_ >= kDeviceUnscaleFrom) {
+            // Unscaled on the device (scaling.rs:114-121), the relabelings
+            // undone by the same pass; bitwise the host's arithmetic below.
+            if (x_out) b_.unscale_to_host(x_t_.get(), scaled_.col_scale, scaled_.bound_scale, &cols_.order, false, x_out);
+            if (y_out) b_.unscale_to_host(y_t_.get(), scaled_.row_scale, scaled_.obj_scale, &rows_.order, true, y_out);
+        } else if (x_out || y_out) {
             std::vector<double> xs((size_t)n_), ys((size_t)m_);
             b_.download(x_t_.get(), xs.data(), n_);
             b_.download(y_t_.get(), ys.data(), m_);
@@ -592,6 +597,8 @@ class Solver {
         kSlotsAllocated = 39
     };
     static constexpr int kPowerBlock = 8;
+    // From this many entries of x and y together, finish() unscales on the device.
+    static constexpr int64_t kDeviceUnscaleFrom = int64_t(1) << 20;
 
     void enqueue_fixed_point(int base) {
         b_.sub(x_t_.get(), x_.get(), tmp_n_.get());

@@ -281,3 +281,39 @@ def test_gather_replay_is_no_slower_than_the_spmv(gpu):
         print(f"transpose={transpose}: replay {replay * 1e3:.1f} us, spmv {spmv * 1e3:.1f} us", flush=True)
         assert 0.0 < replay <= spmv
     pg.free()
+
+
+@pytest.mark.parametrize("n", [0, 1, 1000, (1 << 20) + 7])
+def test_device_unscale_is_the_host_unscale(gpu, n):
+    """sp_unscale_primal / sp_unscale_dual (scaling.rs:114-121) are bitwise
+    x_s · col_scale / bound_scale and y_s · row_scale / obj_scale as the host
+    computes them (the product, then the division), over signs, zeros and
+    factors from 2^-40 to 2^40 (subnormal and overflowing products included)."""
+    rng = np.random.default_rng(n + 11)
+    v = rng.standard_normal(n) * np.exp2(rng.integers(-60, 60, n))
+    f = np.exp2(rng.uniform(-40.0, 40.0, n))
+    if n:
+        v[0] = 0.0
+        v[-1] = -v[-1]
+    if n > 3:
+        v[1], f[1] = 1e-300, 1e-20    # a subnormal product
+        v[2], f[2] = 1e300, 1e20      # an overflowing product: +inf
+    for s in (1.0, 3.7, 2.0 ** -30):
+        with np.errstate(over="ignore", under="ignore"):
+            want = v * f / s
+        dv, df, out = gpu.upload(v), gpu.upload(f), gpu.zeros(n)
+        gpu.unscale_primal(dv, df, s, out)
+        got_x = gpu.download(out)
+        gpu.unscale_dual(dv, df, s, out)
+        got_y = gpu.download(out)
+        assert np.array_equal(got_x, want) and np.array_equal(got_y, want)
+        assert np.array_equal(np.signbit(got_x), np.signbit(want))
+
+
+def test_device_unscale_rejects_mismatched_lengths(gpu):
+    from saddlepoint_amd._lib import BackendError
+
+    with pytest.raises(BackendError, match="wrong length"):
+        gpu.unscale_primal(gpu.upload(np.ones(5)), gpu.upload(np.ones(4)), 1.0, gpu.zeros(5))
+    with pytest.raises(BackendError):
+        gpu.unscale_dual(gpu.upload(np.ones(5)), gpu.upload(np.ones(5)), 1.0, gpu.zeros(6))
