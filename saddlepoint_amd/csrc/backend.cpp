@@ -1381,6 +1381,115 @@ void Hip::dual_ray_stats_into(const sp_problem* p, const sp_vec* dy, const sp_ve
                 (const double*)p->uv, (long long)p->n);
 }
 
+// ---- an OpList on the device --------------------------------------------------------
+
+struct DevOpList {
+    // Host copies of the tables (alive while their asynchronous uploads may
+    // still read them) and the device tables.
+    std::vector<COp> h_ops;
+    std::vector<CBlk> h_sub, h_red, h_hal;
+    std::vector<int> h_fin;
+    COp* ops = nullptr;
+    CBlk *sub = nullptr, *red = nullptr, *hal = nullptr;
+    int* fin = nullptr;
+    double* partials = nullptr;
+};
+
+namespace {
+int red_sums(int kind) {
+    switch (kind) {
+        case kRedDot:
+        case kRedDist:
+        case kRedStep:
+            return 1;
+        case kRedPrimal:
+            return 2;
+        case kRedDual:
+            return 3;
+        case kRedPrimalRay:
+            return 4;
+        case kRedDualRay:
+            return 5;
+    }
+    throw Error(SP_ERR_INVALID, "upload_ops: unknown reduction kind");
+}
+// Reductions over rows and columns run over m + n elements, the others over n.
+int64_t red_len(const COp& o) {
+    return (o.kind == kRedPrimal || o.kind == kRedDual || o.kind == kRedPrimalRay || o.kind == kRedDualRay)
+               ? o.m + o.n
+               : o.n;
+}
+}  // namespace
+
+DevOpList* Hip::upload_ops(const OpList& l) {
+    auto d = std::make_unique<DevOpList>();
+    auto tiles = [](std::vector<CBlk>& blk, const COp& o, int i) {
+        for (long long t = 0; t * kCTile < o.n; ++t) blk.push_back(CBlk{i, (int)t});
+    };
+    for (const COp& o : l.subs) {
+        tiles(d->h_sub, o, (int)d->h_ops.size());
+        d->h_ops.push_back(o);
+    }
+    int pb = 0;
+    for (COp o : l.reds) {
+        // The workgroup count of the method's own launch (reduce_into).
+        o.K = red_sums(o.kind);
+        o.G = reduce_grid(red_len(o));
+        o.pbase = pb;
+        pb += o.G * o.K;
+        const int i = (int)d->h_ops.size();
+        for (int t = 0; t < o.G; ++t) d->h_red.push_back(CBlk{i, t});
+        if (o.G > 1) d->h_fin.push_back(i);
+        d->h_ops.push_back(o);
+    }
+    for (const COp& o : l.halperns) {
+        tiles(d->h_hal, o, (int)d->h_ops.size());
+        d->h_ops.push_back(o);
+    }
+    hipStream_t st = b->stream;
+    try {
+        d->ops = dupload(st, d->h_ops.data(), d->h_ops.size());
+        d->sub = dupload(st, d->h_sub.data(), d->h_sub.size());
+        d->red = dupload(st, d->h_red.data(), d->h_red.size());
+        d->hal = dupload(st, d->h_hal.data(), d->h_hal.size());
+        d->fin = dupload(st, d->h_fin.data(), d->h_fin.size());
+        d->partials = dmalloc<double>((size_t)std::max(pb, 1));
+    } catch (...) {
+        free_ops(d.release());
+        throw;
+    }
+    return d.release();
+}
+
+void free_ops(DevOpList* d) {
+    if (!d) return;
+    dfree(d->ops);
+    dfree(d->sub);
+    dfree(d->red);
+    dfree(d->hal);
+    dfree(d->fin);
+    dfree(d->partials);
+    delete d;
+}
+
+void Hip::run_subs(const DevOpList* d) {
+    if (!d->h_sub.empty()) launch(b, b->k_csub, (unsigned)d->h_sub.size(), (const CBlk*)d->sub, (const COp*)d->ops);
+}
+
+void Hip::run_reductions(const DevOpList* d, double inv_tau) {
+    if (d->h_red.empty()) return;
+    launch(b, b->k_cred_any, (unsigned)d->h_red.size(), (const CBlk*)d->red, (const COp*)d->ops, d->partials,
+           inv_tau);
+    if (!d->h_fin.empty())
+        launch(b, b->k_cred_finish_any, (unsigned)d->h_fin.size(), (const int*)d->fin, (const COp*)d->ops,
+               (const double*)d->partials);
+}
+
+void Hip::run_halperns(const DevOpList* d, double w, double rho) {
+    if (!d->h_hal.empty())
+        launch(b, b->k_chalpern_wr, (unsigned)d->h_hal.size(), (const CBlk*)d->hal, (const COp*)d->ops, w, rho);
+}
+
 double Hip::dot(const sp_vec* a, const sp_vec* bv) {
     dot_into(a, bv, b->red_result);
     return read_result(1, 0)[0];
@@ -1529,6 +1638,9 @@ sp_backend* sp_backend_open(int ordinal) {
         get(b->k_cred_pray, "sp_cred_primal_ray_f64");
         get(b->k_cred_dray, "sp_cred_dual_ray_f64");
         get(b->k_cred_finish, "sp_cred_finish");
+        get(b->k_cred_any, "sp_cred_any_f64");
+        get(b->k_cred_finish_any, "sp_cred_finish_any");
+        get(b->k_chalpern_wr, "sp_chalpern_wr_f64");
         get(b->k_cscale_inv_sqrt, "sp_cscale_inv_sqrt_f64");
         get(b->k_sc_minmax, "sp_sc_minmax");
         get(b->k_sc_powsum, "sp_sc_powsum");

@@ -69,6 +69,46 @@ struct DevCsr {
 // Rows longer than this are gathered by column panel when panels are on.
 constexpr int kPanelMinRow = 64;
 
+// Host mirror of `struct COp` in kernels.hip (same layout): one entry of a
+// batched operation's table (the operands of an SpMV, element-wise pass or
+// reduction; which fields an operation reads is listed at its kernel).
+struct COp {
+    const int *rp = nullptr, *ci = nullptr;
+    const double* va = nullptr;
+    double* part = nullptr;
+    const double* in = nullptr;
+    double *o0 = nullptr, *o1 = nullptr, *o2 = nullptr;
+    const double *a0 = nullptr, *a1 = nullptr, *a2 = nullptr, *a3 = nullptr, *a4 = nullptr, *a5 = nullptr,
+                 *a6 = nullptr, *a7 = nullptr;
+    float* f0 = nullptr;
+    double s0 = 0, s1 = 0;
+    long long m = 0, n = 0;
+    double* dst = nullptr;
+    int G = 0, pbase = 0;
+    int kind = 0, K = 0;
+};
+static_assert(sizeof(COp) == 192, "COp layout");
+// One workgroup of a batched element-wise pass or reduction (mirrors `struct CBlk`).
+struct CBlk {
+    int op, b;
+};
+constexpr int kCTile = 4 * 256;  // CTILE: elements per workgroup of an element-wise pass
+
+// Reduction kinds of sp_cred_any_f64 (RK_* in kernels.hip) and their sum counts.
+enum RedKind { kRedDot = 0, kRedDist = 1, kRedPrimal = 2, kRedDual = 3, kRedStep = 4, kRedPrimalRay = 5,
+               kRedDualRay = 6 };
+
+// A fixed list of element-wise differences (o0 = a0 - a1), reductions and
+// Halpern updates (o0 = w((1+ρ)a0 - ρ·o0) + (1-w)a1) over device memory that
+// stays where it is: one solver's vectors and result slots. Uploaded once
+// (Hip::upload_ops); then each group runs as one launch (the reductions as
+// two), and every entry computes what its single-method launch computes, in
+// the same order, so the results are bitwise those of the methods.
+struct OpList {
+    std::vector<COp> subs, reds, halperns;  // reds: kind, operands, m, n, dst
+};
+struct DevOpList;  // the uploaded tables
+
 // Host-side planner: cuts rows into stream blocks and long-row chunks.
 struct SpmvPlan {
     std::vector<Desc> desc;
@@ -144,6 +184,8 @@ struct sp_backend {
     hipFunction_t k_cspmv_store, k_cspmv_primal, k_cspmv_dual, k_cfin_store, k_cfin_primal, k_cfin_dual, k_csub,
         k_chalpern, k_cdelta_to, k_cdelta_from, k_ccopy, k_cred_dot, k_cred_dist, k_cred_primal, k_cred_dual,
         k_cred_step, k_cred_pray, k_cred_dray, k_cred_finish, k_cscale_inv_sqrt;
+    // One solver's check operations in few launches (Hip::upload_ops).
+    hipFunction_t k_cred_any, k_cred_finish_any, k_chalpern_wr;
     hipFunction_t k_sc_minmax, k_sc_powsum, k_sc_factor, k_sc_apply, k_sc_mul, k_red_check, k_red_finish7;
     hipFunction_t k_tr_rows, k_tr_hist, k_scan_tiles, k_scan_add, k_tr_scatter, k_tr_ptr, k_tr_gather, k_tr_iota,
         k_remap, k_col_count, k_scale_inv_sqrt;
@@ -329,6 +371,14 @@ struct Hip {
     // v *= 1 / sqrt(*ns) with *ns on the device (a result slot).
     void scale_inv_sqrt(sp_vec* v, const double* ns);
 
+    // An OpList on the device, and its three groups as launches: the
+    // differences; the reductions (step_dual's entries with the given 1/τ);
+    // the Halpern updates with the given w and ρ.
+    DevOpList* upload_ops(const OpList& l);
+    void run_subs(const DevOpList* d);
+    void run_reductions(const DevOpList* d, double inv_tau);
+    void run_halperns(const DevOpList* d, double w, double rho);
+
     void sync();
 
     // Workgroups of a reduction over n elements: a fixed function of n, so a
@@ -344,6 +394,7 @@ void free_vec(sp_vec* v);
 void free_vec32(sp_vec32* v);
 void free_csr_public(DevCsr& d);  // releases a DevCsr's device arrays
 void free_host(double* h);
+void free_ops(DevOpList* d);
 // Folds finished event pairs into the per-kernel totals (synchronizes).
 void profile_collect(sp_backend* b);
 

@@ -23,25 +23,8 @@
 
 namespace sp {
 
-// Host mirrors of COp / CBlk / BDesc / BLong in kernels.hip (same layout).
-struct COp {
-    const int *rp = nullptr, *ci = nullptr;
-    const double* va = nullptr;
-    double* part = nullptr;
-    const double* in = nullptr;
-    double *o0 = nullptr, *o1 = nullptr, *o2 = nullptr;
-    const double *a0 = nullptr, *a1 = nullptr, *a2 = nullptr, *a3 = nullptr, *a4 = nullptr, *a5 = nullptr,
-                 *a6 = nullptr, *a7 = nullptr;
-    float* f0 = nullptr;
-    double s0 = 0, s1 = 0;
-    long long m = 0, n = 0;
-    double* dst = nullptr;
-    int G = 0, pbase = 0;
-};
-static_assert(sizeof(COp) == 184, "COp layout");
-struct CBlk {
-    int op, b;
-};
+// Host mirrors of BDesc / BLong in kernels.hip (same layout); COp and CBlk
+// are in backend.hpp.
 struct BDesc {
     Desc d;
     int inst, pad0, pad1, pad2;
@@ -51,8 +34,6 @@ struct BLong {
 };
 static_assert(sizeof(BDesc) == 32, "BDesc layout");
 static_assert(sizeof(BLong) == 16, "BLong layout");
-
-constexpr int kCTile = 4 * 256;  // CTILE: elements per workgroup of an element-wise pass
 
 // Pinned host arena + device arena with bump allocation; reset once the
 // stream has drained everything that read from them.

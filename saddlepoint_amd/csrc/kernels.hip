@@ -278,10 +278,13 @@ __device__ __forceinline__ void finish_body(const LongRow* __restrict__ longs, i
     if (w >= nlong) return;  // uniform per wave
     const LongRow L = longs[w];
     const i64 stride = L.stride ? L.stride : 1;
+    // The row's epilogue operands are loaded beside its partial sums (every
+    // lane, one address), not after the sum: one memory round trip less.
+    const typename Epi::Pre pre = epi.load(L.row);
     A s = 0;
     for (int q = lane; q < L.nch; q += WAVE) s += partials[L.pbase + q * stride];
     s = wave_sum(s);
-    if (lane == 0) epi.apply(L.row, s, epi.load(L.row));
+    if (lane == 0) epi.apply(L.row, s, pre);
 }
 
 template <class A, class Epi>
@@ -1502,9 +1505,10 @@ struct COp {
     i64 m, n;                // lengths (element-wise ops use n)
     double* dst;             // reductions: where the K sums go
     int G, pbase;            // reductions: workgroups, offset of its partials
+    int kind, K;             // sp_cred_any_f64: which reduction, its sum count
 };
 
-static_assert(sizeof(COp) == 184, "COp layout is shared with the host");
+static_assert(sizeof(COp) == 192, "COp layout is shared with the host");
 
 // A workgroup of an element-wise pass or a reduction: entry `op`, tile or
 // workgroup index `b`.
@@ -1688,4 +1692,66 @@ extern "C" __global__ __launch_bounds__(WG) void sp_cred_finish(const int* __res
                                                                 const double* __restrict__ partials, int K) {
     const COp& o = ops[which[blockIdx.x]];
     red_finish_body(partials + o.pbase, o.G, K, o.dst);
+}
+
+// ---- one solver's check iteration in few launches ----
+//
+// The differences, reductions and Halpern updates of a check iteration of the
+// fused single-solve driver, each group as one launch over a table built once
+// per solver: entry i's workgroups compute exactly what its single-method
+// launch computes (same workgroup count G, same order, same finishing code),
+// so every result is bitwise that of the method. Kinds of sp_cred_any_f64,
+// with the operands of the sp_cred_* kernels above:
+enum { RK_DOT = 0, RK_DIST = 1, RK_PRIMAL = 2, RK_DUAL = 3, RK_STEP = 4, RK_PRAY = 5, RK_DRAY = 6 };
+
+// Reductions of several kinds; step_dual's 1/τ is the argument (the table is
+// fixed while τ changes at restarts).
+extern "C" __global__ __launch_bounds__(WG) void sp_cred_any_f64(const CBlk* __restrict__ blk,
+                                                                 const COp* __restrict__ ops,
+                                                                 double* __restrict__ partials, double inv_tau) {
+    CRED_HEAD
+    double* part = partials + o.pbase;
+    switch (o.kind) {  // uniform across the workgroup
+        case RK_DOT:
+            reduce_part<1>(o.n, FDot{o.a0, o.a1}, k.b, o.G, part, o.dst);
+            break;
+        case RK_DIST:
+            reduce_part<1>(o.n, FDist{o.a0, o.a1}, k.b, o.G, part, o.dst);
+            break;
+        case RK_PRIMAL:
+            reduce_part<2>(o.m + o.n, FPrimal{o.a0, o.a1, o.a2, o.a3, o.a4, o.a5, o.m}, k.b, o.G, part, o.dst);
+            break;
+        case RK_DUAL:
+            reduce_part<3>(o.m + o.n, FDual{o.a0, o.a1, o.a2, o.a3, o.a4, o.a5, o.a6, o.a7, o.m}, k.b, o.G, part,
+                           o.dst);
+            break;
+        case RK_STEP:
+            reduce_part<1>(o.n, FStep{o.a0, o.a1, o.a2, o.a3, o.a4, inv_tau}, k.b, o.G, part, o.dst);
+            break;
+        case RK_PRAY:
+            reduce_part<4>(o.m + o.n, FPrimalRay{o.a0, o.a1, o.a2, o.a3, o.a4, o.a5, o.a6, o.m}, k.b, o.G, part,
+                           o.dst);
+            break;
+        default:
+            reduce_part<5>(o.m + o.n, FDualRay{o.a0, o.a1, o.a2, o.a3, o.a4, o.a5, o.m}, k.b, o.G, part, o.dst);
+            break;
+    }
+}
+
+// The finishing pass of sp_cred_any_f64: one workgroup per listed entry (those
+// with G > 1), its K sums from the entry.
+extern "C" __global__ __launch_bounds__(WG) void sp_cred_finish_any(const int* __restrict__ which,
+                                                                    const COp* __restrict__ ops,
+                                                                    const double* __restrict__ partials) {
+    const COp& o = ops[which[blockIdx.x]];
+    red_finish_body(partials + o.pbase, o.G, o.K, o.dst);
+}
+
+// sp_chalpern_f64 with w and ρ as arguments (the same for every entry).
+extern "C" __global__ __launch_bounds__(WG) void sp_chalpern_wr_f64(const CBlk* __restrict__ blk,
+                                                                    const COp* __restrict__ ops, double w,
+                                                                    double rho) {
+    CELEM_HEAD
+    for (i64 i = i0 + threadIdx.x; i < i1; i += WG)
+        o.o0[i] = w * ((1.0 + rho) * o.a0[i] - rho * o.o0[i]) + (1.0 - w) * o.a1[i];
 }

@@ -637,48 +637,101 @@ class Solver {
         b_.spmv_dual(dev_.get(), x_bar_.get(), y_.get(), sigma, ax_.get(), y_t_.get());
         total_ = next_total;
         const bool k_was_zero = k_ == 0;
-        if (check) {
-            enqueue_fixed_point(kFix);
-        } else if (k_was_zero) {
-            // r(z_anchor), needed at the next check (solver.rs:341-346).
-            enqueue_fixed_point(kAnchorFix);
-            anchor_pending_ = true;
-            anchor_omega_ = omega_;
-        }
-        if (check) {
-            b_.spmv(dev_.get(), x_t_.get(), tmp_m_.get());
-            b_.primal_stats_into(dev_.get(), x_t_.get(), tmp_m_.get(), slot(kPrimal));
-            b_.spmv_t(dev_.get(), y_t_.get(), tmp_n_.get());
-            b_.dual_stats_into(dev_.get(), y_t_.get(), tmp_n_.get(), slot(kDual));
-            b_.step_dual_residual_sq_into(dev_.get(), x_.get(), x_t_.get(), aty_.get(), tmp_n_.get(), tau,
-                                          slot(kStep));
-            if (p.termination == 1) {
-                b_.dual_stats_into(dev_.get(), y_t_.get(), c_.get(), slot(kStepRows));
-                b_.dot_into(c_.get(), x_t_.get(), slot(kStepDots));
-                b_.dot_into(aty_.get(), x_t_.get(), slot(kStepDots + 1));
-                b_.dot_into(x_t_.get(), x_t_.get(), slot(kStepDots + 2));
-                b_.dot_into(x_.get(), x_t_.get(), slot(kStepDots + 3));
-            }
-        }
         const double w = (double)(k_ + 1) / (double)(k_ + 2);
-        b_.halpern_step(x_.get(), x_t_.get(), x_anchor_.get(), w, p.reflection);
-        b_.halpern_step(y_.get(), y_t_.get(), y_anchor_.get(), w, p.reflection);
+        if (check) {
+            enqueue_check(tau, w);
+        } else {
+            if (k_was_zero) {
+                // r(z_anchor), needed at the next check (solver.rs:341-346).
+                enqueue_fixed_point(kAnchorFix);
+                anchor_pending_ = true;
+                anchor_omega_ = omega_;
+            }
+            b_.halpern_step(x_.get(), x_t_.get(), x_anchor_.get(), w, p.reflection);
+            b_.halpern_step(y_.get(), y_t_.get(), y_anchor_.get(), w, p.reflection);
+        }
         k_ += 1;
         if (!check) return;
-
-        // Speculative: the infeasibility rays (solver.rs:291-309) and the
-        // restart displacements (solver.rs:403-404).
-        b_.sub(y_t_.get(), y_anchor_.get(), tmp_m_.get());
-        b_.spmv_t(dev_.get(), tmp_m_.get(), tmp_n_.get());
-        b_.dual_ray_stats_into(dev_.get(), tmp_m_.get(), tmp_n_.get(), slot(kDualRay));
-        b_.sub(x_t_.get(), x_anchor_.get(), tmp_n_.get());
-        b_.spmv(dev_.get(), tmp_n_.get(), tmp_m_.get());
-        b_.primal_ray_stats_into(dev_.get(), tmp_n_.get(), tmp_m_.get(), slot(kPrimalRay));
-        b_.distance_sq_into(x_t_.get(), x_anchor_.get(), slot(kDistX));
-        b_.distance_sq_into(y_t_.get(), y_anchor_.get(), slot(kDistY));
         b_.fetch_vec_async(slots_.get(), kSlotCount, host_slots_.get());
         pending_tau_ = tau;
         pending_k_was_zero_ = k_was_zero;
+    }
+
+    // The rest of a check iteration after its two fused SpMVs: the
+    // fixed-point residual's sums (solver.rs:279-287), the KKT sums (kkt,
+    // step_dual_residual_sq and, for step-implied termination, the dual
+    // objective's terms), the speculative infeasibility rays (solver.rs:291-309)
+    // and restart distances (solver.rs:403-404), each into its result slot,
+    // then the Halpern updates. Every reduction reads x before its Halpern
+    // update and nothing the update writes, so the differences run as one
+    // launch, the five products into vectors of their own, and all the
+    // reductions as one launch (plus its finishing pass), each bitwise the
+    // method it stands for (Hip::upload_ops).
+    void enqueue_check(double tau, double w) {
+        if (!check_ops_) build_check_ops();
+        b_.run_subs(check_ops_.get());
+        b_.spmv(dev_.get(), tmp_n_.get(), tmp_m_.get());
+        b_.spmv(dev_.get(), x_t_.get(), ck_axt_.get());
+        b_.spmv_t(dev_.get(), y_t_.get(), ck_atyt_.get());
+        b_.spmv_t(dev_.get(), ck_dya_.get(), ck_atdya_.get());
+        b_.spmv(dev_.get(), ck_dxa_.get(), ck_adxa_.get());
+        b_.run_reductions(check_ops_.get(), 1.0 / tau);  // 1/τ as step_dual_residual_sq forms it
+        b_.run_halperns(check_ops_.get(), w, p_.reflection);
+    }
+
+    // The operand table of enqueue_check, with the vectors only it uses
+    // (made at the first check of the fused path).
+    void build_check_ops() {
+        const int64_t m = m_, n = n_;
+        ck_axt_ = b_.zeros(m), ck_atyt_ = b_.zeros(n), ck_dya_ = b_.zeros(m), ck_atdya_ = b_.zeros(n);
+        ck_dxa_ = b_.zeros(n), ck_adxa_ = b_.zeros(m);
+        const sp_problem* P = dev_.get();
+        OpList l;
+        auto sub = [&](const Vec& a, const Vec& bv, Vec& out) {
+            COp o;
+            o.a0 = a->d, o.a1 = bv->d, o.o0 = out->d, o.n = out->len;
+            l.subs.push_back(o);
+        };
+        sub(x_t_, x_, tmp_n_);           // x_t - x        (fixed point)
+        sub(y_t_, y_, tmp_m2_);          // y_t - y        (fixed point)
+        sub(y_t_, y_anchor_, ck_dya_);   // y_t - y_anchor (dual ray)
+        sub(x_t_, x_anchor_, ck_dxa_);   // x_t - x_anchor (primal ray)
+        // Operands in the order of the kind's reduction functor in kernels.hip.
+        auto red = [&](RedKind kind, std::vector<const double*> a, bool rows_and_cols, int64_t len, int at) {
+            COp o;
+            const double** f[] = {&o.a0, &o.a1, &o.a2, &o.a3, &o.a4, &o.a5, &o.a6, &o.a7};
+            for (size_t i = 0; i < a.size(); ++i) *f[i] = a[i];
+            o.kind = kind;
+            o.m = rows_and_cols ? m : 0;
+            o.n = rows_and_cols ? n : len;
+            o.dst = slot(at);
+            l.reds.push_back(o);
+        };
+        red(kRedDot, {tmp_n_->d, tmp_n_->d}, false, n, kFix);
+        red(kRedDot, {tmp_m2_->d, tmp_m2_->d}, false, m, kFix + 1);
+        red(kRedDot, {tmp_m2_->d, tmp_m_->d}, false, m, kFix + 2);
+        red(kRedPrimal, {ck_axt_->d, P->lc, P->uc, P->pf, P->c, x_t_->d}, true, 0, kPrimal);
+        red(kRedDual, {y_t_->d, P->lc, P->uc, P->c, ck_atyt_->d, P->lv, P->uv, P->df}, true, 0, kDual);
+        red(kRedStep, {x_->d, x_t_->d, aty_->d, ck_atyt_->d, P->df}, false, n, kStep);
+        if (p_.termination == 1) {
+            red(kRedDual, {y_t_->d, P->lc, P->uc, P->c, c_->d, P->lv, P->uv, P->df}, true, 0, kStepRows);
+            red(kRedDot, {c_->d, x_t_->d}, false, n, kStepDots);
+            red(kRedDot, {aty_->d, x_t_->d}, false, n, kStepDots + 1);
+            red(kRedDot, {x_t_->d, x_t_->d}, false, n, kStepDots + 2);
+            red(kRedDot, {x_->d, x_t_->d}, false, n, kStepDots + 3);
+        }
+        red(kRedDualRay, {ck_dya_->d, P->lc, P->uc, ck_atdya_->d, P->lv, P->uv}, true, 0, kDualRay);
+        red(kRedPrimalRay, {ck_adxa_->d, P->lc, P->uc, ck_dxa_->d, P->c, P->lv, P->uv}, true, 0, kPrimalRay);
+        red(kRedDist, {x_t_->d, x_anchor_->d}, false, n, kDistX);
+        red(kRedDist, {y_t_->d, y_anchor_->d}, false, m, kDistY);
+        auto halpern = [&](Vec& z, const Vec& z_t, const Vec& anchor) {
+            COp o;
+            o.o0 = z->d, o.a0 = z_t->d, o.a1 = anchor->d, o.n = z->len;
+            l.halperns.push_back(o);
+        };
+        halpern(x_, x_t_, x_anchor_);
+        halpern(y_, y_t_, y_anchor_);
+        check_ops_ = b_.upload_ops(l);
     }
 
     // The host decisions of a check iteration on the slots read back by
@@ -925,6 +978,11 @@ class Solver {
     Kkt kkt_;
     Prob dev_;
     Vec x_, y_, x_t_, y_t_, x_bar_, x_anchor_, y_anchor_, ax_, aty_, tmp_n_, tmp_m_, tmp_m2_, c_;
+    // Check iterations of the fused path (build_check_ops): A'x_t, A'ᵀy_t,
+    // y_t - y_anchor and its A'ᵀ product, x_t - x_anchor and its A' product,
+    // and the operand table over them.
+    Vec ck_axt_, ck_atyt_, ck_dya_, ck_atdya_, ck_dxa_, ck_adxa_;
+    typename B::Ops check_ops_;
     // Mixed precision: between checks x = x_ + dx32_, y = y_ + dy32_ (x_, y_
     // are the iterates at the last check, the base), with the base's FP64
     // products A'ᵀy_, A'x_ and the anchors' offsets from the base.
