@@ -283,7 +283,9 @@ def serving_bench(args) -> dict:
 
 def stream_bench(args) -> dict:
     """Steady state of a serving stream over a window of `--stream` seconds
-    per mode, on the same config-3 sequence (instance 0, 1, 2, ...).
+    per mode, on the same sequence of the config-3 (or, with `--config 4`,
+    config-4) stream (instance 0, 1, 2, ...), every LP capped at
+    `--lp-max-iterations` in every mode.
 
     Arrivals are either a closed loop (`--stream-rate 0`: `--stream-backlog`
     LPs are in the system at all times, the next entering when one completes,
@@ -300,20 +302,26 @@ def stream_bench(args) -> dict:
     from saddlepoint_amd.serving import DONE, FAILED, Server
 
     T, K, rate = args.stream, args.stream_backlog, args.stream_rate
+    cfg, it_cap = args.config, args.lp_max_iterations
     t0 = time.time()
     pool = []
     for start_i in range(0, args.stream_pool, 50):
-        pool += generate.serving_stream(3, min(50, args.stream_pool - start_i), start=start_i)
+        pool += generate.serving_stream(cfg, min(50, args.stream_pool - start_i), start=start_i)
         log(f"  generated {len(pool)} LPs")
     tols = [1e-4 if i % 2 == 0 else 1e-6 for i in range(len(pool))]
     arrivals = (np.cumsum(np.random.default_rng(2024).exponential(1.0 / rate, len(pool))) if rate > 0 else None)
-    log(f"stream: {len(pool)} config-3 LPs generated in {time.time() - t0:.1f}s; window {T}s, "
+    log(f"stream: {len(pool)} config-{cfg} LPs generated in {time.time() - t0:.1f}s; window {T}s, "
         + (f"Poisson arrivals at {rate} LPs/s" if rate > 0 else f"closed loop of {K}"))
     b = HipBackend(0)
-    out = {"window_s": T, "warmup_s": T / 4, "pool": len(pool),
+    workload = ("config-3 stream (rows log-uniform 1k-200k, columns 1.5-3x rows, 4-30 nonzeros per column, "
+                "tolerance 50/50 1e-4 / 1e-6)")
+    if cfg == 4:
+        workload = ("config-4 stream (the config-3 stream with 10% large LPs: rows log-uniform 200k-1M, "
+                    "Zipf(1.5) row lengths; served mixed precision with autotuned SpMV plans)")
+    out = {"window_s": T, "warmup_s": T / 4, "pool": len(pool), "lp_max_iterations": it_cap,
            "arrivals": f"poisson {rate} LPs/s (seed 2024)" if rate > 0 else f"closed loop, {K} in the system",
-           "workload": "config-3 stream (rows log-uniform 1k-200k, columns 1.5-3x rows, 4-30 nonzeros per column, "
-                       "tolerance 50/50 1e-4 / 1e-6), instances taken in order"}
+           "workload": workload + ", instances taken in order"}
+    autotune = int(cfg == 4)  # config 4: served mixed precision takes each LP's SpMV plans from the autotuned rule
 
     def summary(done, lat, taken, arrived, open_at_end):
         return {"lps_per_s": done / (0.75 * T), "completed_in_window": done, "taken": taken,
@@ -335,7 +343,7 @@ def stream_bench(args) -> dict:
         if rate > 0 and arrivals[i] > now:
             time.sleep(min(arrivals[i], T) - now)
             continue
-        solve(b, pool[i], default_params(tolerance=tols[i]))
+        solve(b, pool[i], default_params(tolerance=tols[i], max_iterations=it_cap))
         s1 = time.perf_counter() - start
         if time.time() - last_report > 30:
             log(f"  one at a time: {s1:.0f}s, {i + 1} solved")
@@ -361,7 +369,8 @@ def stream_bench(args) -> dict:
             if now >= T:
                 break
             while i < len(pool) and (len(pending) < K if rate <= 0 else arrivals[i] <= now):
-                j = srv.submit(pool[i], default_params(tolerance=tols[i], precision=pr))
+                j = srv.submit(pool[i], default_params(tolerance=tols[i], precision=pr, max_iterations=it_cap,
+                                                       spmv_autotune=autotune * pr))
                 entered[j] = start + (arrivals[i] if rate > 0 else now)
                 pending.add(j)
                 i += 1
@@ -593,7 +602,7 @@ def main() -> int:
                     help="LPs of the config-3 stream served beside the headline (0: none)")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--probe", choices=["copy", "triad", "gather", "copy_f32", "triad_f32"],
+    ap.add_argument("--probe", choices=["copy", "triad", "gather", "copy_f32", "triad_f32", "overlap"],
                     help="run only a bandwidth probe (copy: 2^27 doubles, the HBM counter calibration)")
     ap.add_argument("--table", type=int, default=900_000, help="gather probe: table length in doubles")
     ap.add_argument("--large", action="store_true",
@@ -614,7 +623,8 @@ def main() -> int:
     ap.add_argument("--stream-pool", type=int, default=600, help="--stream: LPs of the sequence generated up front")
     ap.add_argument("--serving", action="store_true",
                     help="serving stream instead: LPs/s and latency, served (FP64 and mixed) vs one at a time")
-    ap.add_argument("--config", type=int, default=3, choices=[3, 4], help="serving stream (3, or 4 with large LPs)")
+    ap.add_argument("--config", type=int, default=3, choices=[3, 4],
+                    help="serving and --stream: the config-3 stream, or config 4 (10%% large LPs)")
     ap.add_argument("--lps", type=int, default=64, help="serving: LPs in the sample")
     ap.add_argument("--small", action="store_true", help="serving: small LPs only (rows log-uniform 1k-5k)")
     ap.add_argument("--modes", default="one,fp64,mixed",
@@ -684,6 +694,24 @@ def main() -> int:
                           "serving": r}), flush=True)
         return 0
 
+    if args.probe == "overlap":
+        # Random gathers beside a coalesced stream (8M of each, the size of a
+        # config-1 SpMV): alone, in alternate workgroups, and in every
+        # workgroup one after the other, from tables of 128 KB to 80 MB.
+        b = HipBackend(0)
+        n = 1 << 23
+        for table in (16384, 400_000, 900_000, 10_000_000):
+            t = {k: b.probe("overlap_" + k, n, table, reps=20) * 1e3
+                 for k in ("gather", "stream", "alternate", "both")}
+            print(json.dumps({"probe": "overlap", "elements": n, "table_doubles": table,
+                              "us_gathers_only": round(t["gather"], 2), "us_stream_only": round(t["stream"], 2),
+                              "us_alternate_workgroups": round(t["alternate"], 2),
+                              "us_both_in_every_workgroup": round(t["both"], 2),
+                              "us_sum_of_alone": round(t["gather"] + t["stream"], 2),
+                              "g_gathers_per_s_alone": round(n / t["gather"] / 1e3, 1),
+                              "stream_gbs_alone": round(12 * n / t["stream"] / 1e3, 1)}), flush=True)
+        b.close()
+        return 0
     if args.probe:
         b = HipBackend(0)
         n = 1 << 27

@@ -255,7 +255,14 @@ int sp_backend_profile_read(sp_backend *b, int index, char *name, int name_len, 
  * over n doubles (24 bytes per element), 2 = n random 8-byte gathers from a
  * table of table_len doubles (indices streamed, results written: 12 bytes
  * per element plus the gather), 3 = copy of n floats (8 bytes per element),
- * 4 = triad over n floats (12 bytes per element). */
+ * 4 = triad over n floats (12 bytes per element). Kinds 5-8 have no
+ * counterpart in the reference: how random gathers and a stream share the
+ * memory system, in blocks of 2048 elements (n rounded up to whole blocks):
+ * n 8-byte gathers from a table of table_len doubles (indices streamed) and
+ * a stream of n 12-byte elements (a double and an int, like a matrix's
+ * values and column indices); 5 = gathers only, 6 = stream only, 7 = gather
+ * blocks and stream blocks in alternate workgroups, 8 = every workgroup a
+ * stream block and then a gather block. */
 int sp_probe(sp_backend *b, int kind, int64_t n, int64_t table_len, int reps, double *ms_per_launch);
 /* No counterpart in the reference: the ceiling a matrix's own access pattern
  * puts on its SpMV's gathers. Average milliseconds of one launch that loads,

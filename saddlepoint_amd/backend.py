@@ -296,11 +296,15 @@ class HipBackend:
     def sync(self) -> None:
         self._ok(lib().sp_backend_sync(self.h))
 
-    PROBES = {"copy": 0, "triad": 1, "gather": 2, "copy_f32": 3, "triad_f32": 4}
+    PROBES = {"copy": 0, "triad": 1, "gather": 2, "copy_f32": 3, "triad_f32": 4, "overlap_gather": 5,
+              "overlap_stream": 6, "overlap_alternate": 7, "overlap_both": 8}
 
     def probe(self, kind: str, n: int, table_len: int = 0, reps: int = 20) -> float:
         """Milliseconds per launch of a bandwidth probe: "copy", "triad" (f64),
-        "copy_f32", "triad_f32", or "gather" (random 8-byte gathers)."""
+        "copy_f32", "triad_f32", "gather" (random 8-byte gathers), or one of
+        the "overlap_*" kinds (n gathers beside a stream of n 12-byte
+        elements: gathers only, stream only, alternate workgroups, both in
+        every workgroup)."""
         ms = C.c_double()
         self._ok(lib().sp_probe(self.h, self.PROBES[kind], int(n), int(table_len), int(reps), C.byref(ms)))
         return ms.value

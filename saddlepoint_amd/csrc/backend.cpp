@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <memory>
 #include <climits>
 #include <unordered_map>
@@ -856,7 +857,10 @@ void Hip::plain_iteration_mixed(const sp_problem* p, sp_vec32* dx, sp_vec32* dy,
 // ---- bandwidth probe ---------------------------------------------------------------------
 
 double Hip::probe(int kind, int64_t n, int64_t table_len, int reps, uint64_t seed) {
-    if (n <= 0 || reps <= 0 || (kind == 2 && table_len <= 0)) throw Error(SP_ERR_INVALID, "probe: bad size");
+    if (n <= 0 || reps <= 0 || ((kind == 2 || (kind >= 5 && kind <= 8)) && table_len <= 0))
+        throw Error(SP_ERR_INVALID, "probe: bad size");
+    if (kind < 0 || kind > 8) throw Error(SP_ERR_INVALID, "probe: unknown kind");
+    if (kind >= 5) return probe_overlap(kind - 5, n, table_len, reps, seed);
     if (kind == 3 || kind == 4) {
         // The float copy and triad run on the doubles' buffers, reinterpreted.
         const int64_t half = (n + 1) / 2;
@@ -942,6 +946,64 @@ double Hip::probe(int kind, int64_t n, int64_t table_len, int reps, uint64_t see
     }
     (void)hipStreamSynchronize(b->stream);
     for (void* q : {(void*)a, (void*)bb, (void*)c, (void*)table, (void*)idx}) dfree(q);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return (double)ms / reps;
+}
+
+// sp_probe_overlap in mode `mode` over nb = ceil(n / 2048) blocks of each kind.
+double Hip::probe_overlap(int mode, int64_t n, int64_t table_len, int reps, uint64_t seed) {
+    constexpr int64_t kBlock = 8 * kWG;
+    const int64_t nb = (n + kBlock - 1) / kBlock, len = nb * kBlock;
+    if (nb > (int64_t(1) << 30)) throw Error(SP_ERR_TOO_LARGE, "probe: too many blocks");
+    const int64_t grid = mode == 2 ? 2 * nb : nb;
+    double *table = nullptr, *v = nullptr, *out = nullptr;
+    int *idx = nullptr, *ci = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms = 0.0f;
+    try {
+        table = dmalloc<double>((size_t)table_len);
+        v = dmalloc<double>((size_t)len);
+        out = dmalloc<double>((size_t)(grid * kWG));
+        launch(b, b->k_fill, ew_grid(table_len), table, 3.0, (long long)table_len);
+        launch(b, b->k_fill, ew_grid(len), v, 1.0, (long long)len);
+        // Uniform random indices (xorshift); the stream's ints are a second
+        // draw, so the two kinds of block share no lines.
+        std::vector<int> h((size_t)len);
+        uint64_t st = seed * 0x9E3779B97F4A7C15ull | 1;
+        auto draw = [&] {
+            for (auto& x : h) {
+                st ^= st << 13, st ^= st >> 7, st ^= st << 17;
+                x = (int)(st % (uint64_t)table_len);
+            }
+        };
+        draw();
+        idx = dupload(b->stream, h.data(), h.size());
+        hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        draw();
+        ci = dupload(b->stream, h.data(), h.size());
+        hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        auto once = [&] {
+            launch(b, b->k_probe_overlap, (unsigned)grid, (const int*)idx, (const double*)table, (const double*)v,
+                   (const int*)ci, mode, out);
+        };
+        once();
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        hip_check(hipEventRecord(e0, b->stream), "hipEventRecord");
+        for (int r = 0; r < reps; ++r) once();
+        hip_check(hipEventRecord(e1, b->stream), "hipEventRecord");
+        hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    } catch (...) {
+        (void)hipStreamSynchronize(b->stream);
+        for (void* q : {(void*)table, (void*)v, (void*)out, (void*)idx, (void*)ci}) dfree(q);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipStreamSynchronize(b->stream);
+    for (void* q : {(void*)table, (void*)v, (void*)out, (void*)idx, (void*)ci}) dfree(q);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return (double)ms / reps;
@@ -1393,9 +1455,18 @@ struct DevOpList {
     CBlk *sub = nullptr, *red = nullptr, *hal = nullptr;
     int* fin = nullptr;
     double* partials = nullptr;
+    // step_dual entries: their 1/τ (s0) is rewritten when τ changes, from a
+    // ring of pinned staging slots; a slot is reused once the event recorded
+    // after its copy has completed.
+    static constexpr int kTauSlots = 8;
+    std::vector<int> step_ops;
+    double inv_tau = 0.0;
+    double* h_tau = nullptr;
+    hipEvent_t tau_done[kTauSlots] = {};
+    bool tau_used[kTauSlots] = {};
+    int tau_next = 0;
 };
 
-namespace {
 int red_sums(int kind) {
     switch (kind) {
         case kRedDot:
@@ -1411,15 +1482,8 @@ int red_sums(int kind) {
         case kRedDualRay:
             return 5;
     }
-    throw Error(SP_ERR_INVALID, "upload_ops: unknown reduction kind");
+    throw Error(SP_ERR_INVALID, "unknown reduction kind");
 }
-// Reductions over rows and columns run over m + n elements, the others over n.
-int64_t red_len(const COp& o) {
-    return (o.kind == kRedPrimal || o.kind == kRedDual || o.kind == kRedPrimalRay || o.kind == kRedDualRay)
-               ? o.m + o.n
-               : o.n;
-}
-}  // namespace
 
 DevOpList* Hip::upload_ops(const OpList& l) {
     auto d = std::make_unique<DevOpList>();
@@ -1440,6 +1504,10 @@ DevOpList* Hip::upload_ops(const OpList& l) {
         const int i = (int)d->h_ops.size();
         for (int t = 0; t < o.G; ++t) d->h_red.push_back(CBlk{i, t});
         if (o.G > 1) d->h_fin.push_back(i);
+        if (o.kind == kRedStep) {
+            d->step_ops.push_back(i);
+            d->inv_tau = o.s0;
+        }
         d->h_ops.push_back(o);
     }
     for (const COp& o : l.halperns) {
@@ -1454,6 +1522,10 @@ DevOpList* Hip::upload_ops(const OpList& l) {
         d->hal = dupload(st, d->h_hal.data(), d->h_hal.size());
         d->fin = dupload(st, d->h_fin.data(), d->h_fin.size());
         d->partials = dmalloc<double>((size_t)std::max(pb, 1));
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&d->h_tau), sizeof(double) * DevOpList::kTauSlots,
+                                hipHostMallocDefault),
+                  "hipHostMalloc");
+        for (hipEvent_t& e : d->tau_done) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
     } catch (...) {
         free_ops(d.release());
         throw;
@@ -1469,6 +1541,9 @@ void free_ops(DevOpList* d) {
     dfree(d->hal);
     dfree(d->fin);
     dfree(d->partials);
+    if (d->h_tau) (void)hipHostFree(d->h_tau);
+    for (hipEvent_t e : d->tau_done)
+        if (e) (void)hipEventDestroy(e);
     delete d;
 }
 
@@ -1476,10 +1551,21 @@ void Hip::run_subs(const DevOpList* d) {
     if (!d->h_sub.empty()) launch(b, b->k_csub, (unsigned)d->h_sub.size(), (const CBlk*)d->sub, (const COp*)d->ops);
 }
 
-void Hip::run_reductions(const DevOpList* d, double inv_tau) {
+void Hip::run_reductions(DevOpList* d, double inv_tau) {
     if (d->h_red.empty()) return;
-    launch(b, b->k_cred_any, (unsigned)d->h_red.size(), (const CBlk*)d->red, (const COp*)d->ops, d->partials,
-           inv_tau);
+    if (!d->step_ops.empty() && !(inv_tau == d->inv_tau)) {
+        const int k = d->tau_next++ % DevOpList::kTauSlots;
+        if (d->tau_used[k]) hip_check(hipEventSynchronize(d->tau_done[k]), "hipEventSynchronize");  // its last copy ran
+        d->h_tau[k] = inv_tau;
+        for (int i : d->step_ops)
+            hip_check(hipMemcpyAsync(reinterpret_cast<char*>(d->ops + i) + offsetof(COp, s0), d->h_tau + k,
+                                     sizeof(double), hipMemcpyHostToDevice, b->stream),
+                      "hipMemcpyAsync H2D");
+        hip_check(hipEventRecord(d->tau_done[k], b->stream), "hipEventRecord");
+        d->tau_used[k] = true;
+        d->inv_tau = inv_tau;
+    }
+    launch(b, b->k_cred_any, (unsigned)d->h_red.size(), (const CBlk*)d->red, (const COp*)d->ops, d->partials);
     if (!d->h_fin.empty())
         launch(b, b->k_cred_finish_any, (unsigned)d->h_fin.size(), (const int*)d->fin, (const COp*)d->ops,
                (const double*)d->partials);
@@ -1630,14 +1716,6 @@ sp_backend* sp_backend_open(int ordinal) {
         get(b->k_cdelta_to, "sp_cdelta_to");
         get(b->k_cdelta_from, "sp_cdelta_from");
         get(b->k_ccopy, "sp_ccopy_f64");
-        get(b->k_cred_dot, "sp_cred_dot_f64");
-        get(b->k_cred_dist, "sp_cred_distance_sq_f64");
-        get(b->k_cred_primal, "sp_cred_primal_stats_f64");
-        get(b->k_cred_dual, "sp_cred_dual_stats_f64");
-        get(b->k_cred_step, "sp_cred_step_dual_f64");
-        get(b->k_cred_pray, "sp_cred_primal_ray_f64");
-        get(b->k_cred_dray, "sp_cred_dual_ray_f64");
-        get(b->k_cred_finish, "sp_cred_finish");
         get(b->k_cred_any, "sp_cred_any_f64");
         get(b->k_cred_finish_any, "sp_cred_finish_any");
         get(b->k_chalpern_wr, "sp_chalpern_wr_f64");
@@ -1678,6 +1756,7 @@ sp_backend* sp_backend_open(int ordinal) {
         get(b->k_probe_gather, "sp_probe_gather");
         get(b->k_probe_copy32, "sp_probe_copy_f32");
         get(b->k_probe_triad32, "sp_probe_triad_f32");
+        get(b->k_probe_overlap, "sp_probe_overlap");
         get(b->k_replay64, "sp_probe_replay_f64");
         get(b->k_panel_count, "sp_panel_count");
         get(b->k_panel_gather, "sp_panel_gather");

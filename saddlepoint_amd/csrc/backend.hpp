@@ -94,9 +94,17 @@ struct CBlk {
 };
 constexpr int kCTile = 4 * 256;  // CTILE: elements per workgroup of an element-wise pass
 
-// Reduction kinds of sp_cred_any_f64 (RK_* in kernels.hip) and their sum counts.
+// Reduction kinds of sp_cred_any_f64 (RK_* in kernels.hip).
 enum RedKind { kRedDot = 0, kRedDist = 1, kRedPrimal = 2, kRedDual = 3, kRedStep = 4, kRedPrimalRay = 5,
                kRedDualRay = 6 };
+// A kind's sum count K, and the length of an entry's reduction: rows and
+// columns (m + n) for the stats and rays, n for the others.
+int red_sums(int kind);
+inline int64_t red_len(const COp& o) {
+    return (o.kind == kRedPrimal || o.kind == kRedDual || o.kind == kRedPrimalRay || o.kind == kRedDualRay)
+               ? o.m + o.n
+               : o.n;
+}
 
 // A fixed list of element-wise differences (o0 = a0 - a1), reductions and
 // Halpern updates (o0 = w((1+ρ)a0 - ρ·o0) + (1-w)a1) over device memory that
@@ -182,9 +190,8 @@ struct sp_backend {
         k_bfin_t_mixed, k_bfin_a_mixed;
     // Batched check operations (serving loop).
     hipFunction_t k_cspmv_store, k_cspmv_primal, k_cspmv_dual, k_cfin_store, k_cfin_primal, k_cfin_dual, k_csub,
-        k_chalpern, k_cdelta_to, k_cdelta_from, k_ccopy, k_cred_dot, k_cred_dist, k_cred_primal, k_cred_dual,
-        k_cred_step, k_cred_pray, k_cred_dray, k_cred_finish, k_cscale_inv_sqrt;
-    // One solver's check operations in few launches (Hip::upload_ops).
+        k_chalpern, k_cdelta_to, k_cdelta_from, k_ccopy, k_cscale_inv_sqrt;
+    // Reductions of several kinds in one launch (serving phases, Hip::upload_ops).
     hipFunction_t k_cred_any, k_cred_finish_any, k_chalpern_wr;
     hipFunction_t k_sc_minmax, k_sc_powsum, k_sc_factor, k_sc_apply, k_sc_mul, k_red_check, k_red_finish7;
     hipFunction_t k_tr_rows, k_tr_hist, k_scan_tiles, k_scan_add, k_tr_scatter, k_tr_ptr, k_tr_gather, k_tr_iota,
@@ -192,7 +199,7 @@ struct sp_backend {
     hipFunction_t k_spmv, k_spmv_t_primal, k_spmv_t_primal_halpern, k_spmv_dual, k_spmv_dual_halpern;
     hipFunction_t k_primal_step, k_dual_step, k_halpern, k_sub, k_scale, k_fill, k_unscale_primal, k_unscale_dual;
     hipFunction_t k_probe_copy, k_probe_triad, k_probe_gather, k_probe_copy32, k_probe_triad32, k_replay64,
-        k_replay32;
+        k_replay32, k_probe_overlap;
     hipFunction_t k_spmv32, k_spmv_t_primal_mixed, k_spmv_dual_mixed, k_to_f32, k_delta_from, k_delta_to;
     hipFunction_t k_panel_count, k_panel_gather, k_spmv_panel, k_fin_panel, k_spmv_panel32, k_fin_panel32;
     hipFunction_t k_rows_gather, k_gather;
@@ -322,7 +329,10 @@ struct Hip {
     // Average milliseconds of one probe kernel over `reps` launches: kind 0
     // copies n doubles, 1 is a triad over n doubles, 2 gathers n doubles at
     // pseudo-random positions of a table of table_len doubles, 3 and 4 are
-    // the copy and the triad over n floats.
+    // the copy and the triad over n floats; 5-8 are sp_probe_overlap modes
+    // 0-3 (n gathers from such a table beside a stream of n 12-byte
+    // elements: gathers only, stream only, alternate workgroups, both in
+    // every workgroup), n rounded up to whole 2048-element blocks.
     double probe(int kind, int64_t n, int64_t table_len, int reps, uint64_t seed);
     // Average milliseconds of one launch of the gather-only replay of a
     // loaded matrix's column indices (A' or, with transpose, A'ᵀ): every
@@ -372,11 +382,12 @@ struct Hip {
     void scale_inv_sqrt(sp_vec* v, const double* ns);
 
     // An OpList on the device, and its three groups as launches: the
-    // differences; the reductions (step_dual's entries with the given 1/τ);
-    // the Halpern updates with the given w and ρ.
+    // differences; the reductions (step_dual's entries with the given 1/τ,
+    // rewritten in the table when it changes); the Halpern updates with the
+    // given w and ρ.
     DevOpList* upload_ops(const OpList& l);
     void run_subs(const DevOpList* d);
-    void run_reductions(const DevOpList* d, double inv_tau);
+    void run_reductions(DevOpList* d, double inv_tau);
     void run_halperns(const DevOpList* d, double w, double rho);
 
     void sync();
@@ -387,6 +398,7 @@ struct Hip {
 
    private:
     const double* read_result(int k, int grid);
+    double probe_overlap(int mode, int64_t n, int64_t table_len, int reps, uint64_t seed);
 };
 
 void free_problem(sp_problem* p);

@@ -101,10 +101,10 @@ class Phase {
         }
         if (desc.empty()) return;
         const size_t o = put(ops), dd = put(desc);
-        ls_.push_back(L{k, kSpmv, (unsigned)desc.size(), dd, o, 0, 0});
+        ls_.push_back(L{k, kSpmv, (unsigned)desc.size(), dd, o, 0});
         if (!longs.empty()) {
             const size_t ll = put(longs);
-            ls_.push_back(L{fin, kFin, (unsigned)((longs.size() + 3) / 4), ll, o, (int)longs.size(), 0});
+            ls_.push_back(L{fin, kFin, (unsigned)((longs.size() + 3) / 4), ll, o, (int)longs.size()});
         }
     }
 
@@ -115,30 +115,32 @@ class Phase {
             for (long long t = 0; t * kCTile < ops[i].n; ++t) blk.push_back(CBlk{(int)i, (int)t});
         if (blk.empty()) return;
         const size_t o = put(ops), bb = put(blk);
-        ls_.push_back(L{k, kElem, (unsigned)blk.size(), bb, o, 0, 0});
+        ls_.push_back(L{k, kElem, (unsigned)blk.size(), bb, o, 0});
     }
 
-    // A K-sum reduction over each entry's length `len(op)` with the workgroup
-    // count of the single-LP launch; results to each entry's dst.
-    template <class Len>
-    void reduce(hipFunction_t k, int K, std::vector<COp> ops, const Hip& h, Len len) {
+    // Reductions of several kinds (each entry's kind set; sp_cred_any_f64),
+    // every entry with the workgroup count of its single-LP launch: one
+    // launch, and one finishing pass for the entries of more than one
+    // workgroup.
+    void reduce_any(std::vector<COp> ops, const Hip& h) {
         std::vector<CBlk> blk;
         std::vector<int> which;
         int pb = 0;
         for (size_t i = 0; i < ops.size(); ++i) {
-            ops[i].G = h.reduce_grid(len(ops[i]));
+            ops[i].K = red_sums(ops[i].kind);
+            ops[i].G = h.reduce_grid(red_len(ops[i]));
             ops[i].pbase = pb;
-            pb += ops[i].G * K;
+            pb += ops[i].G * ops[i].K;
             for (int t = 0; t < ops[i].G; ++t) blk.push_back(CBlk{(int)i, t});
             if (ops[i].G > 1) which.push_back((int)i);
         }
         if (blk.empty()) return;
         partials_ = std::max(partials_, (size_t)pb);
         const size_t o = put(ops), bb = put(blk);
-        ls_.push_back(L{k, kRed, (unsigned)blk.size(), bb, o, 0, K});
+        ls_.push_back(L{b_->k_cred_any, kRed, (unsigned)blk.size(), bb, o, 0});
         if (!which.empty()) {
             const size_t ww = put(which);
-            ls_.push_back(L{nullptr, kRedFin, (unsigned)which.size(), ww, o, 0, K});
+            ls_.push_back(L{nullptr, kRedFin, (unsigned)which.size(), ww, o, 0});
         }
     }
 
@@ -170,7 +172,7 @@ class Phase {
                     go(l.k, l.grid, list, ops, partials);
                     break;
                 case kRedFin:
-                    go(b_->k_cred_finish, l.grid, list, ops, (const double*)partials, l.K);
+                    go(b_->k_cred_finish_any, l.grid, list, ops, (const double*)partials);
                     break;
             }
         }
@@ -183,7 +185,7 @@ class Phase {
         int type;
         unsigned grid;
         size_t list, ops;
-        int count, K;
+        int count;
     };
 
     template <class T>
@@ -210,6 +212,20 @@ class Phase {
 using StepWork = Solver<DriverBackend>::StepWork;
 using PlainEntry = Solver<DriverBackend>::PlainEntry;
 
+// The Halpern steps of every entry (after its reductions, which read x and
+// y before the update).
+inline void enqueue_halpern(Phase& P, sp_backend* b, const std::vector<StepWork>& ws) {
+    std::vector<COp> v;
+    for (const StepWork& s : ws) {
+        COp x, y;
+        x.o0 = s.x->d, x.a0 = s.x_t->d, x.a1 = s.x_anchor->d, x.s0 = s.w, x.s1 = s.rho, x.n = s.x->len;
+        y.o0 = s.y->d, y.a0 = s.y_t->d, y.a1 = s.y_anchor->d, y.s0 = s.w, y.s1 = s.rho, y.n = s.y->len;
+        v.push_back(x);
+        v.push_back(y);
+    }
+    P.elementwise(b->k_chalpern, v);
+}
+
 // The device work of Solver::enqueue_step for every entry of `ws` (all check
 // iterations, or all first iterations of a restart epoch), in its order.
 inline void enqueue_step_work(Phase& P, const Hip& h, const std::vector<StepWork>& ws) {
@@ -223,8 +239,6 @@ inline void enqueue_step_work(Phase& P, const Hip& h, const std::vector<StepWork
         for (const StepWork& s : ws) f(v, s);
         return v;
     };
-    auto len_n = [](const COp& o) { return (int64_t)o.n; };
-    auto len_mn = [](const COp& o) { return (int64_t)(o.m + o.n); };
     auto slot = [](const StepWork& s, int i) { return s.slots + i; };
 
     // Mixed precision: the FP64 iterates, base + displacement.
@@ -251,147 +265,103 @@ inline void enqueue_step_work(Phase& P, const Hip& h, const std::vector<StepWork
                v.push_back(o);
            }),
            probs, true);
-    // The fixed-point residual sums (enqueue_fixed_point).
-    P.elementwise(b->k_csub, each([](std::vector<COp>& v, const StepWork& s) {
-        COp x, y;
-        x.a0 = s.x_t->d, x.a1 = s.x->d, x.o0 = s.tmp_n->d, x.n = s.x->len;
-        y.a0 = s.y_t->d, y.a1 = s.y->d, y.o0 = s.tmp_m2->d, y.n = s.y->len;
-        v.push_back(x);
-        v.push_back(y);
-    }));
-    P.spmv(b->k_cspmv_store, b->k_cfin_store, each([](std::vector<COp>& v, const StepWork& s) {
-               COp o;
-               o.in = s.tmp_n->d, o.o0 = s.tmp_m->d;
-               v.push_back(o);
-           }),
-           probs, true);
-    P.reduce(b->k_cred_dot, 1, each([&](std::vector<COp>& v, const StepWork& s) {
-                 COp d0, d1, d2;
-                 d0.a0 = s.tmp_n->d, d0.a1 = s.tmp_n->d, d0.n = s.tmp_n->len, d0.dst = slot(s, s.fix_base);
-                 d1.a0 = s.tmp_m2->d, d1.a1 = s.tmp_m2->d, d1.n = s.tmp_m2->len, d1.dst = slot(s, s.fix_base + 1);
-                 d2.a0 = s.tmp_m2->d, d2.a1 = s.tmp_m->d, d2.n = s.tmp_m2->len, d2.dst = slot(s, s.fix_base + 2);
-                 v.push_back(d0);
-                 v.push_back(d1);
-                 v.push_back(d2);
-             }),
-             h, len_n);
     using S = Solver<DriverBackend>;
-    if (check) {
-        // kkt: spmv(x_t → tmp_m), primal_stats; spmv_t(y_t → tmp_n), dual_stats; step_dual.
+    if (!check) {
+        // The first iteration of a restart epoch: r(z_anchor)'s sums
+        // (enqueue_fixed_point), then the Halpern steps.
+        P.elementwise(b->k_csub, each([](std::vector<COp>& v, const StepWork& s) {
+            COp x, y;
+            x.a0 = s.x_t->d, x.a1 = s.x->d, x.o0 = s.tmp_n->d, x.n = s.x->len;
+            y.a0 = s.y_t->d, y.a1 = s.y->d, y.o0 = s.tmp_m2->d, y.n = s.y->len;
+            v.push_back(x);
+            v.push_back(y);
+        }));
         P.spmv(b->k_cspmv_store, b->k_cfin_store, each([](std::vector<COp>& v, const StepWork& s) {
                    COp o;
-                   o.in = s.x_t->d, o.o0 = s.tmp_m->d;
+                   o.in = s.tmp_n->d, o.o0 = s.tmp_m->d;
                    v.push_back(o);
                }),
                probs, true);
-        P.reduce(b->k_cred_primal, 2, each([&](std::vector<COp>& v, const StepWork& s) {
-                     COp o;
-                     const sp_problem* p = s.problem;
-                     o.a0 = s.tmp_m->d, o.a1 = p->lc, o.a2 = p->uc, o.a3 = p->pf, o.a4 = p->c, o.a5 = s.x_t->d;
-                     o.m = p->m, o.n = p->n, o.dst = slot(s, S::slot_primal());
-                     v.push_back(o);
-                 }),
-                 h, len_mn);
-        P.spmv(b->k_cspmv_store, b->k_cfin_store, each([](std::vector<COp>& v, const StepWork& s) {
+        P.reduce_any(each([&](std::vector<COp>& v, const StepWork& s) {
+                         COp d0, d1, d2;
+                         d0.a0 = s.tmp_n->d, d0.a1 = s.tmp_n->d, d0.n = s.tmp_n->len, d0.dst = slot(s, s.fix_base);
+                         d1.a0 = s.tmp_m2->d, d1.a1 = s.tmp_m2->d, d1.n = s.tmp_m2->len;
+                         d1.dst = slot(s, s.fix_base + 1);
+                         d2.a0 = s.tmp_m2->d, d2.a1 = s.tmp_m->d, d2.n = s.tmp_m2->len;
+                         d2.dst = slot(s, s.fix_base + 2);
+                         for (COp* d : {&d0, &d1, &d2}) {
+                             d->kind = kRedDot;
+                             v.push_back(*d);
+                         }
+                     }),
+                     h);
+        enqueue_halpern(P, b, ws);
+        return;
+    }
+    // A check iteration (Solver::enqueue_check): the four differences, the
+    // five products, every reduction in one launch, the Halpern steps.
+    P.elementwise(b->k_csub, each([](std::vector<COp>& v, const StepWork& s) {
+        const std::pair<const sp_vec*, const sp_vec*> ab[4] = {
+            {s.x_t, s.x}, {s.y_t, s.y}, {s.y_t, s.y_anchor}, {s.x_t, s.x_anchor}};
+        sp_vec* out[4] = {s.tmp_n, s.tmp_m2, s.dya, s.dxa};
+        for (int q = 0; q < 4; ++q) {
+            COp o;
+            o.a0 = ab[q].first->d, o.a1 = ab[q].second->d, o.o0 = out[q]->d, o.n = out[q]->len;
+            v.push_back(o);
+        }
+    }));
+    auto product = [&](auto in, auto out, bool a) {
+        P.spmv(b->k_cspmv_store, b->k_cfin_store, each([&](std::vector<COp>& v, const StepWork& s) {
                    COp o;
-                   o.in = s.y_t->d, o.o0 = s.tmp_n->d;
+                   o.in = in(s)->d, o.o0 = out(s)->d;
                    v.push_back(o);
                }),
-               probs, false);
-        P.reduce(b->k_cred_dual, 3, each([&](std::vector<COp>& v, const StepWork& s) {
-                     COp o;
+               probs, a);
+    };
+    product([](const StepWork& s) { return s.tmp_n; }, [](const StepWork& s) { return s.tmp_m; }, true);
+    product([](const StepWork& s) { return s.x_t; }, [](const StepWork& s) { return s.axt; }, true);
+    product([](const StepWork& s) { return s.y_t; }, [](const StepWork& s) { return s.atyt; }, false);
+    product([](const StepWork& s) { return s.dya; }, [](const StepWork& s) { return s.atdya; }, false);
+    product([](const StepWork& s) { return s.dxa; }, [](const StepWork& s) { return s.adxa; }, true);
+    P.reduce_any(each([&](std::vector<COp>& v, const StepWork& s) {
                      const sp_problem* p = s.problem;
-                     o.a0 = s.y_t->d, o.a1 = p->lc, o.a2 = p->uc, o.a3 = p->c, o.a4 = s.tmp_n->d, o.a5 = p->lv;
-                     o.a6 = p->uv, o.a7 = p->df, o.m = p->m, o.n = p->n, o.dst = slot(s, S::slot_dual());
-                     v.push_back(o);
-                 }),
-                 h, len_mn);
-        P.reduce(b->k_cred_step, 1, each([&](std::vector<COp>& v, const StepWork& s) {
-                     COp o;
-                     o.a0 = s.x->d, o.a1 = s.x_t->d, o.a2 = s.aty->d, o.a3 = s.tmp_n->d, o.a4 = s.problem->df;
-                     o.s0 = 1.0 / s.tau, o.n = s.problem->n, o.dst = slot(s, S::slot_step());
-                     v.push_back(o);
-                 }),
-                 h, len_n);
-        // Step-implied termination: dual_stats(y_t, c) and four dots.
-        P.reduce(b->k_cred_dual, 3, each([&](std::vector<COp>& v, const StepWork& s) {
-                     if (!s.step_implied) return;
-                     COp o;
-                     const sp_problem* p = s.problem;
-                     o.a0 = s.y_t->d, o.a1 = p->lc, o.a2 = p->uc, o.a3 = p->c, o.a4 = s.c->d, o.a5 = p->lv;
-                     o.a6 = p->uv, o.a7 = p->df, o.m = p->m, o.n = p->n, o.dst = slot(s, S::slot_step_rows());
-                     v.push_back(o);
-                 }),
-                 h, len_mn);
-        P.reduce(b->k_cred_dot, 1, each([&](std::vector<COp>& v, const StepWork& s) {
-                     if (!s.step_implied) return;
-                     const double* pairs[4][2] = {{s.c->d, s.x_t->d}, {s.aty->d, s.x_t->d}, {s.x_t->d, s.x_t->d},
-                                                  {s.x->d, s.x_t->d}};
-                     for (int q = 0; q < 4; ++q) {
+                     auto red = [&](RedKind kind, std::vector<const double*> a, bool rows_and_cols, int64_t len,
+                                    int at) {
                          COp o;
-                         o.a0 = pairs[q][0], o.a1 = pairs[q][1], o.n = s.x_t->len;
-                         o.dst = slot(s, S::slot_step_dots() + q);
+                         const double** f[] = {&o.a0, &o.a1, &o.a2, &o.a3, &o.a4, &o.a5, &o.a6, &o.a7};
+                         for (size_t i = 0; i < a.size(); ++i) *f[i] = a[i];
+                         o.kind = kind;
+                         o.m = rows_and_cols ? p->m : 0;
+                         o.n = rows_and_cols ? p->n : len;
+                         o.dst = slot(s, at);
                          v.push_back(o);
+                     };
+                     const int64_t m = p->m, n = p->n;
+                     red(kRedDot, {s.tmp_n->d, s.tmp_n->d}, false, n, s.fix_base);
+                     red(kRedDot, {s.tmp_m2->d, s.tmp_m2->d}, false, m, s.fix_base + 1);
+                     red(kRedDot, {s.tmp_m2->d, s.tmp_m->d}, false, m, s.fix_base + 2);
+                     red(kRedPrimal, {s.axt->d, p->lc, p->uc, p->pf, p->c, s.x_t->d}, true, 0, S::slot_primal());
+                     red(kRedDual, {s.y_t->d, p->lc, p->uc, p->c, s.atyt->d, p->lv, p->uv, p->df}, true, 0,
+                         S::slot_dual());
+                     red(kRedStep, {s.x->d, s.x_t->d, s.aty->d, s.atyt->d, p->df}, false, n, S::slot_step());
+                     v.back().s0 = 1.0 / s.tau;
+                     if (s.step_implied) {
+                         red(kRedDual, {s.y_t->d, p->lc, p->uc, p->c, s.c->d, p->lv, p->uv, p->df}, true, 0,
+                             S::slot_step_rows());
+                         const double* pairs[4][2] = {{s.c->d, s.x_t->d}, {s.aty->d, s.x_t->d},
+                                                      {s.x_t->d, s.x_t->d}, {s.x->d, s.x_t->d}};
+                         for (int q = 0; q < 4; ++q)
+                             red(kRedDot, {pairs[q][0], pairs[q][1]}, false, n, S::slot_step_dots() + q);
                      }
+                     red(kRedDualRay, {s.dya->d, p->lc, p->uc, s.atdya->d, p->lv, p->uv}, true, 0,
+                         S::slot_dual_ray());
+                     red(kRedPrimalRay, {s.adxa->d, p->lc, p->uc, s.dxa->d, p->c, p->lv, p->uv}, true, 0,
+                         S::slot_primal_ray());
+                     red(kRedDist, {s.x_t->d, s.x_anchor->d}, false, n, S::slot_dist_x());
+                     red(kRedDist, {s.y_t->d, s.y_anchor->d}, false, m, S::slot_dist_y());
                  }),
-                 h, len_n);
-    }
-    // The Halpern steps.
-    P.elementwise(b->k_chalpern, each([](std::vector<COp>& v, const StepWork& s) {
-        COp x, y;
-        x.o0 = s.x->d, x.a0 = s.x_t->d, x.a1 = s.x_anchor->d, x.s0 = s.w, x.s1 = s.rho, x.n = s.x->len;
-        y.o0 = s.y->d, y.a0 = s.y_t->d, y.a1 = s.y_anchor->d, y.s0 = s.w, y.s1 = s.rho, y.n = s.y->len;
-        v.push_back(x);
-        v.push_back(y);
-    }));
-    if (!check) return;
-    // The infeasibility rays and the restart distances.
-    P.elementwise(b->k_csub, each([](std::vector<COp>& v, const StepWork& s) {
-        COp o;
-        o.a0 = s.y_t->d, o.a1 = s.y_anchor->d, o.o0 = s.tmp_m->d, o.n = s.y->len;
-        v.push_back(o);
-    }));
-    P.spmv(b->k_cspmv_store, b->k_cfin_store, each([](std::vector<COp>& v, const StepWork& s) {
-               COp o;
-               o.in = s.tmp_m->d, o.o0 = s.tmp_n->d;
-               v.push_back(o);
-           }),
-           probs, false);
-    P.reduce(b->k_cred_dray, 5, each([&](std::vector<COp>& v, const StepWork& s) {
-                 COp o;
-                 const sp_problem* p = s.problem;
-                 o.a0 = s.tmp_m->d, o.a1 = p->lc, o.a2 = p->uc, o.a3 = s.tmp_n->d, o.a4 = p->lv, o.a5 = p->uv;
-                 o.m = p->m, o.n = p->n, o.dst = slot(s, S::slot_dual_ray());
-                 v.push_back(o);
-             }),
-             h, len_mn);
-    P.elementwise(b->k_csub, each([](std::vector<COp>& v, const StepWork& s) {
-        COp o;
-        o.a0 = s.x_t->d, o.a1 = s.x_anchor->d, o.o0 = s.tmp_n->d, o.n = s.x->len;
-        v.push_back(o);
-    }));
-    P.spmv(b->k_cspmv_store, b->k_cfin_store, each([](std::vector<COp>& v, const StepWork& s) {
-               COp o;
-               o.in = s.tmp_n->d, o.o0 = s.tmp_m->d;
-               v.push_back(o);
-           }),
-           probs, true);
-    P.reduce(b->k_cred_pray, 4, each([&](std::vector<COp>& v, const StepWork& s) {
-                 COp o;
-                 const sp_problem* p = s.problem;
-                 o.a0 = s.tmp_m->d, o.a1 = p->lc, o.a2 = p->uc, o.a3 = s.tmp_n->d, o.a4 = p->c, o.a5 = p->lv;
-                 o.a6 = p->uv, o.m = p->m, o.n = p->n, o.dst = slot(s, S::slot_primal_ray());
-                 v.push_back(o);
-             }),
-             h, len_mn);
-    P.reduce(b->k_cred_dist, 1, each([&](std::vector<COp>& v, const StepWork& s) {
-                 COp x, y;
-                 x.a0 = s.x_t->d, x.a1 = s.x_anchor->d, x.n = s.x_t->len, x.dst = slot(s, S::slot_dist_x());
-                 y.a0 = s.y_t->d, y.a1 = s.y_anchor->d, y.n = s.y_t->len, y.dst = slot(s, S::slot_dist_y());
-                 v.push_back(x);
-                 v.push_back(y);
-             }),
-             h, len_n);
+                 h);
+    enqueue_halpern(P, b, ws);
 }
 
 // Every entry's result slots into one buffer (gathered[i·count + j]), for
@@ -462,6 +432,7 @@ inline void enqueue_power_rounds(Phase& P, const Hip& h, const std::vector<Power
             q.in = w.tmp_m->d, q.o0 = w.v->d;
             t.push_back(q);
             COp dd;
+            dd.kind = kRedDot;
             dd.a0 = w.v->d, dd.a1 = w.v->d, dd.n = w.v->len, dd.dst = w.slots + base + r;
             d.push_back(dd);
             COp sc;
@@ -470,7 +441,7 @@ inline void enqueue_power_rounds(Phase& P, const Hip& h, const std::vector<Power
         }
         P.spmv(b->k_cspmv_store, b->k_cfin_store, a, probs, true);
         P.spmv(b->k_cspmv_store, b->k_cfin_store, t, probs, false);
-        P.reduce(b->k_cred_dot, 1, d, h, [](const COp& o) { return (int64_t)o.n; });
+        P.reduce_any(d, h);
         P.elementwise(b->k_cscale_inv_sqrt, s);
     }
 }

@@ -382,7 +382,8 @@ int sp_backend_profile_read(sp_backend* b, int index, char* name, int name_len, 
 
 int sp_probe(sp_backend* b, int kind, int64_t n, int64_t table_len, int reps, double* ms_per_launch) {
     return guarded(b, [&] {
-        if (!ms_per_launch || kind < 0 || kind > 4) throw sp::Error(SP_ERR_INVALID, "probe: bad argument");
+        if (!ms_per_launch || kind < 0 || kind > 8) throw sp::Error(SP_ERR_INVALID, "probe: bad argument");
+        if (table_len > INT32_MAX) throw sp::Error(SP_ERR_TOO_LARGE, "probe: table beyond int32 indices");
         *ms_per_launch = sp::Hip{b}.probe(kind, n, table_len, reps, 1);
     });
 }

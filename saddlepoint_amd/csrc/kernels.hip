@@ -1294,6 +1294,53 @@ extern "C" __global__ __launch_bounds__(WG) void sp_probe_gather(const int* __re
     }
 }
 
+// Gathers beside a stream: blocks of WG·8 elements. A gather block loads
+// 2048 indices (coalesced) and gathers table[idx] for each; a stream block
+// loads 2048 doubles and 2048 ints (12 bytes an element, like a matrix's
+// values and column indices). mode 0: gather blocks only; 1: stream blocks
+// only; 2: the two kinds in alternate workgroups (both over all nb blocks);
+// 3: every workgroup a stream block, then a gather block (as an SpMV stream
+// block does). Each thread stores one sum (never read), so nothing is elided.
+__device__ __forceinline__ double probe_gather_block(i64 b, const int* __restrict__ idx,
+                                                     const double* __restrict__ table) {
+    const i64 base = b * WG * 8;
+    int c[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) c[u] = idx[base + threadIdx.x + u * WG];
+    double s = 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += table[c[u]];
+    return s;
+}
+__device__ __forceinline__ double probe_stream_block(i64 b, const double* __restrict__ v,
+                                                     const int* __restrict__ ci) {
+    const i64 base = b * WG * 8;
+    double x[8];
+    int y[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        x[u] = v[base + threadIdx.x + u * WG];
+        y[u] = ci[base + threadIdx.x + u * WG];
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += x[u] + (double)y[u];
+    return s;
+}
+extern "C" __global__ __launch_bounds__(WG) void sp_probe_overlap(const int* __restrict__ idx,
+                                                                  const double* __restrict__ table,
+                                                                  const double* __restrict__ v,
+                                                                  const int* __restrict__ ci, int mode,
+                                                                  double* __restrict__ out) {
+    const i64 b = blockIdx.x;
+    double s;
+    if (mode == 0) s = probe_gather_block(b, idx, table);
+    else if (mode == 1) s = probe_stream_block(b, v, ci);
+    else if (mode == 2) s = (b & 1) ? probe_stream_block(b >> 1, v, ci) : probe_gather_block(b >> 1, idx, table);
+    else s = probe_stream_block(b, v, ci) + probe_gather_block(b, idx, table);
+    out[b * WG + threadIdx.x] = s;
+}
+
 extern "C" __global__ __launch_bounds__(WG) void sp_probe_copy_f32(const float* __restrict__ a, float* __restrict__ b,
                                                                    i64 n) {
     GRID_LOOP(i, n) { b[i] = __builtin_nontemporal_load(a + i); }
@@ -1623,93 +1670,31 @@ extern "C" __global__ __launch_bounds__(WG) void sp_ccopy_f64(const CBlk* __rest
     for (i64 i = i0 + threadIdx.x; i < i1; i += WG) o.o0[i] = o.a0[i];
 }
 
-// ---- reductions: workgroup b of the entry's G ----
-
-#define CRED_HEAD                   \
-    const CBlk k = blk[blockIdx.x]; \
-    const COp& o = ops[k.op];
-
-extern "C" __global__ __launch_bounds__(WG) void sp_cred_dot_f64(const CBlk* __restrict__ blk,
-                                                                 const COp* __restrict__ ops,
-                                                                 double* __restrict__ partials) {
-    CRED_HEAD
-    reduce_part<1>(o.n, FDot{o.a0, o.a1}, k.b, o.G, partials + o.pbase, o.dst);
-}
-
-extern "C" __global__ __launch_bounds__(WG) void sp_cred_distance_sq_f64(const CBlk* __restrict__ blk,
-                                                                         const COp* __restrict__ ops,
-                                                                         double* __restrict__ partials) {
-    CRED_HEAD
-    reduce_part<1>(o.n, FDist{o.a0, o.a1}, k.b, o.G, partials + o.pbase, o.dst);
-}
-
-// a0..a5 = ax, lc, uc, pf, c, x
-extern "C" __global__ __launch_bounds__(WG) void sp_cred_primal_stats_f64(const CBlk* __restrict__ blk,
-                                                                          const COp* __restrict__ ops,
-                                                                          double* __restrict__ partials) {
-    CRED_HEAD
-    reduce_part<2>(o.m + o.n, FPrimal{o.a0, o.a1, o.a2, o.a3, o.a4, o.a5, o.m}, k.b, o.G, partials + o.pbase, o.dst);
-}
-
-// a0..a7 = y, lc, uc, c, aty, lv, uv, df
-extern "C" __global__ __launch_bounds__(WG) void sp_cred_dual_stats_f64(const CBlk* __restrict__ blk,
-                                                                        const COp* __restrict__ ops,
-                                                                        double* __restrict__ partials) {
-    CRED_HEAD
-    reduce_part<3>(o.m + o.n, FDual{o.a0, o.a1, o.a2, o.a3, o.a4, o.a5, o.a6, o.a7, o.m}, k.b, o.G,
-                   partials + o.pbase, o.dst);
-}
-
-// a0..a4 = x, x_t, aty, aty_t, df; s0 = 1/τ
-extern "C" __global__ __launch_bounds__(WG) void sp_cred_step_dual_f64(const CBlk* __restrict__ blk,
-                                                                       const COp* __restrict__ ops,
-                                                                       double* __restrict__ partials) {
-    CRED_HEAD
-    reduce_part<1>(o.n, FStep{o.a0, o.a1, o.a2, o.a3, o.a4, o.s0}, k.b, o.G, partials + o.pbase, o.dst);
-}
-
-// a0..a6 = adx, lc, uc, dx, c, lv, uv
-extern "C" __global__ __launch_bounds__(WG) void sp_cred_primal_ray_f64(const CBlk* __restrict__ blk,
-                                                                        const COp* __restrict__ ops,
-                                                                        double* __restrict__ partials) {
-    CRED_HEAD
-    reduce_part<4>(o.m + o.n, FPrimalRay{o.a0, o.a1, o.a2, o.a3, o.a4, o.a5, o.a6, o.m}, k.b, o.G,
-                   partials + o.pbase, o.dst);
-}
-
-// a0..a5 = dy, lc, uc, atdy, lv, uv
-extern "C" __global__ __launch_bounds__(WG) void sp_cred_dual_ray_f64(const CBlk* __restrict__ blk,
-                                                                      const COp* __restrict__ ops,
-                                                                      double* __restrict__ partials) {
-    CRED_HEAD
-    reduce_part<5>(o.m + o.n, FDualRay{o.a0, o.a1, o.a2, o.a3, o.a4, o.a5, o.m}, k.b, o.G, partials + o.pbase,
-                   o.dst);
-}
-
-// One workgroup per listed entry with G > 1: its partials added in order.
-extern "C" __global__ __launch_bounds__(WG) void sp_cred_finish(const int* __restrict__ which,
-                                                                const COp* __restrict__ ops,
-                                                                const double* __restrict__ partials, int K) {
-    const COp& o = ops[which[blockIdx.x]];
-    red_finish_body(partials + o.pbase, o.G, K, o.dst);
-}
-
-// ---- one solver's check iteration in few launches ----
+// ---- reductions of several kinds in one launch ----
 //
-// The differences, reductions and Halpern updates of a check iteration of the
-// fused single-solve driver, each group as one launch over a table built once
-// per solver: entry i's workgroups compute exactly what its single-method
-// launch computes (same workgroup count G, same order, same finishing code),
-// so every result is bitwise that of the method. Kinds of sp_cred_any_f64,
-// with the operands of the sp_cred_* kernels above:
+// The reductions of a check iteration (and of a power-iteration round) read
+// no output of one another, so they run as one launch and one finishing
+// pass over a table of entries: entry i's workgroups b < G compute exactly
+// what its single-method launch computes (same workgroup count G, same
+// order, same finishing code), so every result is bitwise that of the
+// method. Used by the single solve (one table per solver) and the serving
+// loop (every due LP's entries). Kinds, with their operands and lengths:
+//   RK_DOT     a0·a1 over n
+//   RK_DIST    ‖a0 - a1‖² over n
+//   RK_PRIMAL  primal_stats: a0..a5 = ax, lc, uc, pf, c, x over m + n (2 sums)
+//   RK_DUAL    dual_stats: a0..a7 = y, lc, uc, c, aty, lv, uv, df over m + n (3)
+//   RK_STEP    step_dual_residual_sq: a0..a4 = x, x_t, aty, aty_t, df; s0 = 1/τ; over n
+//   RK_PRAY    primal_ray_stats: a0..a6 = adx, lc, uc, dx, c, lv, uv over m + n (4)
+//   RK_DRAY    dual_ray_stats: a0..a5 = dy, lc, uc, atdy, lv, uv over m + n (5)
+// Workgroup b of entry op; the K sums go to dst (G = 1) or to the entry's
+// partials at pbase.
 enum { RK_DOT = 0, RK_DIST = 1, RK_PRIMAL = 2, RK_DUAL = 3, RK_STEP = 4, RK_PRAY = 5, RK_DRAY = 6 };
 
-// Reductions of several kinds; step_dual's 1/τ is the argument (the table is
-// fixed while τ changes at restarts).
 extern "C" __global__ __launch_bounds__(WG) void sp_cred_any_f64(const CBlk* __restrict__ blk,
                                                                  const COp* __restrict__ ops,
-                                                                 double* __restrict__ partials, double inv_tau) {
-    CRED_HEAD
+                                                                 double* __restrict__ partials) {
+    const CBlk k = blk[blockIdx.x];
+    const COp& o = ops[k.op];
     double* part = partials + o.pbase;
     switch (o.kind) {  // uniform across the workgroup
         case RK_DOT:
@@ -1726,7 +1711,7 @@ extern "C" __global__ __launch_bounds__(WG) void sp_cred_any_f64(const CBlk* __r
                            o.dst);
             break;
         case RK_STEP:
-            reduce_part<1>(o.n, FStep{o.a0, o.a1, o.a2, o.a3, o.a4, inv_tau}, k.b, o.G, part, o.dst);
+            reduce_part<1>(o.n, FStep{o.a0, o.a1, o.a2, o.a3, o.a4, o.s0}, k.b, o.G, part, o.dst);
             break;
         case RK_PRAY:
             reduce_part<4>(o.m + o.n, FPrimalRay{o.a0, o.a1, o.a2, o.a3, o.a4, o.a5, o.a6, o.m}, k.b, o.G, part,

@@ -138,10 +138,11 @@ struct Job {
 
 // Device bytes one admitted LP holds (the admission test): both matrices with
 // int32 indices and FP64 values (+ FP32 values in mixed precision), the
-// solver's 13 vectors, the problem's 7 vectors and the mixed-precision
-// displacements, anchor offsets and base products.
+// solver's 13 vectors and the 6 of its check iterations, the problem's 7
+// vectors and the mixed-precision displacements, anchor offsets and base
+// products.
 static int64_t footprint(int64_t m, int64_t n, int64_t nnz, bool mixed) {
-    int64_t bytes = 2 * (12 * nnz + 4 * (m + 1)) + 8 * (7 * n + 6 * m) + 8 * (4 * n + 3 * m);
+    int64_t bytes = 2 * (12 * nnz + 4 * (m + 1)) + 8 * (10 * n + 9 * m) + 8 * (4 * n + 3 * m);
     bytes += 2 * 32 * (nnz / 1024 + 64);  // plans
     if (mixed) bytes += 2 * 4 * nnz + 4 * (3 * n + 2 * m) + 8 * (n + m);
     return bytes;
@@ -582,11 +583,17 @@ class Server {
         DevBuf<BLong> long_t, long_a;
         DevBuf<BInst> inst;
         int nt = 0, na = 0, nlt = 0, nla = 0;
+        // The captured steps and their executable; the template graph is
+        // kept as long as the executable (tools that trace graph launches
+        // may look its nodes up).
+        hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         int graph_steps = 0;
         void drop_graph() {
             if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
             exec = nullptr;
+            graph = nullptr;
         }
         ~Batch() { drop_graph(); }
     };
@@ -725,9 +732,8 @@ class Server {
                 throw;
             }
             hip_check(hipStreamEndCapture(b_->stream, &graph_), "hipStreamEndCapture");
-            hipError_t e = hipGraphInstantiate(&bt.exec, graph_, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph_);
-            hip_check(e, "hipGraphInstantiate");
+            bt.graph = graph_;
+            hip_check(hipGraphInstantiate(&bt.exec, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
             bt.graph_steps = steps;
         }
         hip_check(hipGraphLaunch(bt.exec, b_->stream), "hipGraphLaunch");

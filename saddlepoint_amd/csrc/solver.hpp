@@ -379,6 +379,9 @@ class Solver {
         int fix_base = 0;      // slot of the fixed-point sums
         sp_problem* problem = nullptr;
         sp_vec *x, *y, *x_t, *y_t, *x_bar, *x_anchor, *y_anchor, *ax, *aty, *tmp_n, *tmp_m, *tmp_m2, *c;
+        // Check iterations: A'x_t, A'ᵀy_t, y_t - y_anchor, its A'ᵀ product,
+        // x_t - x_anchor and its A' product (null otherwise).
+        sp_vec *axt = nullptr, *atyt = nullptr, *dya = nullptr, *atdya = nullptr, *dxa = nullptr, *adxa = nullptr;
         sp_vec32 *dx = nullptr, *dy = nullptr;
         double* slots = nullptr;  // device, slot_count() values
         double tau = 0, sigma = 0, w = 0, rho = 0;
@@ -410,6 +413,11 @@ class Solver {
         s.x = x_.get(), s.y = y_.get(), s.x_t = x_t_.get(), s.y_t = y_t_.get(), s.x_bar = x_bar_.get();
         s.x_anchor = x_anchor_.get(), s.y_anchor = y_anchor_.get(), s.ax = ax_.get(), s.aty = aty_.get();
         s.tmp_n = tmp_n_.get(), s.tmp_m = tmp_m_.get(), s.tmp_m2 = tmp_m2_.get(), s.c = c_.get();
+        if (check) {
+            ensure_check_vectors();
+            s.axt = ck_axt_.get(), s.atyt = ck_atyt_.get(), s.dya = ck_dya_.get(), s.atdya = ck_atdya_.get();
+            s.dxa = ck_dxa_.get(), s.adxa = ck_adxa_.get();
+        }
         s.dx = dx32_ ? dx32_.get() : nullptr;
         s.dy = dy32_ ? dy32_.get() : nullptr;
         s.slots = slots_->d;
@@ -679,12 +687,19 @@ class Solver {
         b_.run_halperns(check_ops_.get(), w, p_.reflection);
     }
 
-    // The operand table of enqueue_check, with the vectors only it uses
-    // (made at the first check of the fused path).
-    void build_check_ops() {
+    // The vectors only check iterations of the fused path use (made at the
+    // first one).
+    void ensure_check_vectors() {
+        if (ck_axt_) return;
         const int64_t m = m_, n = n_;
         ck_axt_ = b_.zeros(m), ck_atyt_ = b_.zeros(n), ck_dya_ = b_.zeros(m), ck_atdya_ = b_.zeros(n);
         ck_dxa_ = b_.zeros(n), ck_adxa_ = b_.zeros(m);
+    }
+
+    // The operand table of enqueue_check.
+    void build_check_ops() {
+        const int64_t m = m_, n = n_;
+        ensure_check_vectors();
         const sp_problem* P = dev_.get();
         OpList l;
         auto sub = [&](const Vec& a, const Vec& bv, Vec& out) {
@@ -713,6 +728,7 @@ class Solver {
         red(kRedPrimal, {ck_axt_->d, P->lc, P->uc, P->pf, P->c, x_t_->d}, true, 0, kPrimal);
         red(kRedDual, {y_t_->d, P->lc, P->uc, P->c, ck_atyt_->d, P->lv, P->uv, P->df}, true, 0, kDual);
         red(kRedStep, {x_->d, x_t_->d, aty_->d, ck_atyt_->d, P->df}, false, n, kStep);
+        l.reds.back().s0 = 1.0 / (eta_ / omega_);  // rewritten by run_reductions when τ changes
         if (p_.termination == 1) {
             red(kRedDual, {y_t_->d, P->lc, P->uc, P->c, c_->d, P->lv, P->uv, P->df}, true, 0, kStepRows);
             red(kRedDot, {c_->d, x_t_->d}, false, n, kStepDots);

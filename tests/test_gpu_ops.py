@@ -269,6 +269,25 @@ def test_bandwidth_probes_report_plausible_rates(gpu, kind, per):
     assert 0.0 < gbs <= 8000.0
 
 
+def test_overlap_probe_kinds(gpu):
+    """The gather/stream overlap probe: every kind times a finite launch, the
+    stream alone stays under the HBM peak, and doing both kinds of work takes
+    longer than either alone (each does all of its own blocks); a ragged n is
+    rounded up to whole blocks and a table past int32 indices is refused."""
+    n, table = (1 << 21) + 5, 400_000
+    t = {k: gpu.probe("overlap_" + k, n, table, reps=5) for k in ("gather", "stream", "alternate", "both")}
+    print({k: round(v * 1e3, 2) for k, v in t.items()}, flush=True)
+    assert all(np.isfinite(v) and v > 0.0 for v in t.values())
+    blocks = -(-n // 2048)
+    assert 12 * blocks * 2048 / (t["stream"] * 1e-3) / 1e9 <= 8000.0
+    for k in ("alternate", "both"):
+        assert t[k] > max(t["gather"], t["stream"])
+    with pytest.raises(RuntimeError):
+        gpu.probe("overlap_gather", n, 0, reps=1)
+    with pytest.raises(RuntimeError):
+        gpu.probe("overlap_gather", n, 1 << 31, reps=1)
+
+
 def test_gather_replay_is_no_slower_than_the_spmv(gpu):
     """The gather replay (the matrix's own index stream, gathers only) is the
     ceiling of its SpMV's gathers: it takes no longer than the SpMV itself."""
