@@ -126,12 +126,12 @@ def fifo_latencies(arrivals, service):
 
 
 def run_served(b, lps, params_of, arrivals=None, max_active=256, with_solution=False, use_graphs=None,
-               chunk_work=None):
+               chunk_work=None, max_skipped=None):
     """Serves `lps` (all at t = 0, or at the given arrival times on the
     server's clock); returns (makespan, latencies, results)."""
     from saddlepoint_amd.serving import Server
 
-    srv = Server(b, max_active=max_active, use_graphs=use_graphs, chunk_work=chunk_work)
+    srv = Server(b, max_active=max_active, use_graphs=use_graphs, chunk_work=chunk_work, max_skipped=max_skipped)
     n = len(lps)
     ids = [None] * n
     t_arr = [0.0] * n
@@ -198,7 +198,7 @@ def serving_bench(args) -> dict:
                                            max_iterations=it_cap, spmv_autotune=autotune * pr,
                                            mixed_stall_checks=args.stall_checks))
             mk, lat, res = run_served(b, lps, pm, max_active=args.max_active, chunk_work=args.chunk_work,
-                                      use_graphs=not args.no_graphs)
+                                      use_graphs=not args.no_graphs, max_skipped=args.max_skipped)
             out["served_" + prec] = {"lps_per_s": len(lps) / mk, "makespan_s": mk, "p50_s": pct(lat, 50),
                                      "p99_s": pct(lat, 99)}
             log(f"served {prec}: {len(lps) / mk:.3f} LPs/s, makespan {mk:.1f}s")
@@ -232,7 +232,7 @@ def serving_bench(args) -> dict:
         pm = (lambda i: default_params(tolerance=tols[i], precision=pr, time_limit_s=limit, max_iterations=it_cap,
                                        spmv_autotune=autotune * pr, mixed_stall_checks=args.stall_checks))
         mk, lat, res = run_served(b, lps, pm, with_solution=args.check, max_active=args.max_active,
-                                  chunk_work=args.chunk_work)
+                                  chunk_work=args.chunk_work, max_skipped=args.max_skipped)
         st = {}
         for r in res:
             st[r.status] = st.get(r.status, 0) + 1
@@ -271,7 +271,8 @@ def serving_bench(args) -> dict:
         log(f"served {prec}: {entry['lps_per_s']:.3f} LPs/s (x{entry['speedup_vs_one_at_a_time']:.2f}), "
             f"p50 {entry['p50_s']:.3f}s p99 {entry['p99_s']:.3f}s")
         if args.poisson and pr:
-            _, plat, _ = run_served(b, lps, pm, arrivals, max_active=args.max_active, chunk_work=args.chunk_work)
+            _, plat, _ = run_served(b, lps, pm, arrivals, max_active=args.max_active, chunk_work=args.chunk_work,
+                                    max_skipped=args.max_skipped)
             entry["poisson"] = {"rate_lps_per_s": args.rate_frac * cap, "p50_s": pct(plat, 50),
                                 "p99_s": pct(plat, 99)}
             log(f"  poisson at {args.rate_frac} x capacity: p50 {pct(plat, 50):.3f}s p99 {pct(plat, 99):.3f}s "
@@ -331,12 +332,14 @@ def stream_bench(args) -> dict:
     def arrived_by(t):
         return int(np.searchsorted(arrivals, t, side="right")) if rate > 0 else None
 
+    modes = [m.strip() for m in args.modes.split(",") if m.strip()]
+    out["server"] = {"chunk_work": args.chunk_work, "max_skipped": args.max_skipped, "max_active": args.max_active}
     # One at a time. Closed loop: LP i enters when LP i - K completes (at the
     # start for the first K). Poisson: LP i enters at its arrival time; the
     # GPU idles while nothing has arrived.
     start = time.perf_counter()
     done, lat, i, ends, last_report = 0, [], 0, [], time.time()
-    while i < len(pool):
+    while i < len(pool) and "one" in modes:
         now = time.perf_counter() - start
         if now >= T:
             break
@@ -354,11 +357,13 @@ def stream_bench(args) -> dict:
             entered = arrivals[i] if rate > 0 else (ends[i - K] if i >= K else 0.0)
             lat.append(s1 - entered)
         i += 1
-    arrived = arrived_by(T) if rate > 0 else i + K
-    out["one_at_a_time_fp64"] = summary(done, lat, i, arrived, max(arrived - i, 0) if rate > 0 else K)
-    log(f"  one at a time: {out['one_at_a_time_fp64']['lps_per_s']:.3f} LPs/s ({i} taken)")
-    for prec in ("fp64", "mixed"):
-        srv = Server(b)
+    if "one" in modes:
+        arrived = arrived_by(T) if rate > 0 else i + K
+        out["one_at_a_time_fp64"] = summary(done, lat, i, arrived, max(arrived - i, 0) if rate > 0 else K)
+        log(f"  one at a time: {out['one_at_a_time_fp64']['lps_per_s']:.3f} LPs/s ({i} taken)")
+    for prec in [m for m in ("fp64", "mixed") if m in modes]:
+        srv = Server(b, chunk_work=args.chunk_work or None, max_skipped=args.max_skipped or None,
+                     max_active=args.max_active)
         pr = int(prec == "mixed")
         entered, i = {}, 0
         start = srv.clock()
@@ -396,8 +401,9 @@ def stream_bench(args) -> dict:
         arrived = arrived_by(T) if rate > 0 else i
         key = "served_" + prec
         out[key] = summary(done, lat, i, arrived, len(pending) + (arrived - i if rate > 0 else 0))
-        out[key]["speedup_vs_one_at_a_time"] = out[key]["lps_per_s"] / max(out["one_at_a_time_fp64"]["lps_per_s"],
-                                                                          1e-12)
+        if "one_at_a_time_fp64" in out:
+            out[key]["speedup_vs_one_at_a_time"] = out[key]["lps_per_s"] / max(
+                out["one_at_a_time_fp64"]["lps_per_s"], 1e-12)
         log(f"  served {prec}: {out[key]['lps_per_s']:.3f} LPs/s ({i} taken)")
     b.close()
     return out
@@ -628,11 +634,13 @@ def main() -> int:
     ap.add_argument("--lps", type=int, default=64, help="serving: LPs in the sample")
     ap.add_argument("--small", action="store_true", help="serving: small LPs only (rows log-uniform 1k-5k)")
     ap.add_argument("--modes", default="one,fp64,mixed",
-                    help="serving: modes to run (without 'one' only the served ones, no ratios: for profiling)")
+                    help="serving and --stream: modes to run (without 'one' only the served ones, no ratios)")
     ap.add_argument("--no-graphs", action="store_true", help="serving: eager launches instead of hipGraph replay")
     ap.add_argument("--chunk-work", type=int, default=0,
                     help="serving: sp_server_params.chunk_work (0: the default; negative: every LP every chunk)")
-    ap.add_argument("--max-active", type=int, default=256, help="serving: LPs admitted at once")
+    ap.add_argument("--max-active", type=int, default=256, help="serving and --stream: LPs admitted at once")
+    ap.add_argument("--max-skipped", type=int, default=0,
+                    help="serving and --stream: sp_server_params.max_skipped (0: the library's default)")
     ap.add_argument("--poisson", action="store_true", help="serving: also Poisson arrivals (mixed precision)")
     ap.add_argument("--rate-frac", type=float, default=0.8, help="serving: arrival rate / one-at-a-time capacity")
     ap.add_argument("--lp-time-limit", type=float, default=0.0, help="serving: seconds per LP (0: none)")

@@ -367,10 +367,13 @@ typedef struct {
     int use_graphs;       /* replay the batched steps of a chunk from a hipGraph (default 1) */
     int64_t chunk_work;   /* which admitted LPs a chunk runs: the cheapest ones (cost nnz + m + n)
                              until their costs add up to chunk_work, plus any LP left out of the
-                             last 8 chunks, so that a chunk holds enough work to fill the GPU and
+                             last max_skipped chunks, so that a chunk holds enough work to fill the GPU and
                              small LPs are not paced by large ones. 0: the default (32,000,000);
                              negative: every admitted LP in every chunk. An LP's iterates do not
                              depend on which chunks it runs in. */
+    int max_skipped;      /* an LP left out of this many chunks in a row runs in the next one
+                             (0: the default, 8): the bound on how far chunk_work may pace a large
+                             LP, which sets the latency of the largest LPs against the small ones' */
 } sp_server_params;
 /* Seconds on the server's clock (from its creation). */
 typedef struct { double submitted, admitted, completed; } sp_job_times;

@@ -82,7 +82,7 @@ class Solution(C.Structure):
 
 class ServerParams(C.Structure):
     _fields_ = [("budget_bytes", _i64), ("max_active", C.c_int), ("prep_threads", C.c_int), ("use_graphs", C.c_int),
-                ("chunk_work", _i64)]
+                ("chunk_work", _i64), ("max_skipped", C.c_int)]
 
 
 class JobTimes(C.Structure):

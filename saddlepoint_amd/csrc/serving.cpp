@@ -407,7 +407,7 @@ class Server {
 
     // The LPs this chunk runs: the cheapest first (cost, then id) until their
     // costs reach the chunk's work target, and every LP that was left out of
-    // the last kMaxSkipped chunks; in id order. The rest keep their state.
+    // the last max_skipped chunks; in id order. The rest keep their state.
     static constexpr int64_t kChunkWork = 32'000'000;
     static constexpr int kMaxSkipped = 8;
     std::vector<Job*> chunk_members() {
@@ -416,6 +416,7 @@ class Server {
             if (!j->failed_now && !j->solver->terminated()) live.push_back(j);
         if (sp_.chunk_work < 0) return live;
         const int64_t target = sp_.chunk_work > 0 ? sp_.chunk_work : kChunkWork;
+        const int max_skipped = sp_.max_skipped > 0 ? sp_.max_skipped : kMaxSkipped;
         std::vector<Job*> order = live;
         std::stable_sort(order.begin(), order.end(), [](const Job* a, const Job* b) {
             return a->cost != b->cost ? a->cost < b->cost : a->id < b->id;
@@ -423,7 +424,7 @@ class Server {
         int64_t work = 0;
         std::vector<Job*> run;
         for (Job* j : order) {
-            if (work < target || j->skipped >= kMaxSkipped) {
+            if (work < target || j->skipped >= max_skipped) {
                 run.push_back(j);
                 work += j->cost;
                 j->skipped = 0;
@@ -839,6 +840,7 @@ void sp_default_server_params(sp_server_params* p) {
     p->prep_threads = 8;
     p->use_graphs = 1;
     p->chunk_work = 0;
+    p->max_skipped = 0;
 }
 
 int sp_server_create(sp_backend* b, const sp_server_params* params, sp_server** out) {

@@ -33,7 +33,8 @@ class Served:
 
 class Server:
     def __init__(self, backend: HipBackend, budget_bytes: int | None = None, max_active: int | None = None,
-                 prep_threads: int | None = None, use_graphs: bool | None = None, chunk_work: int | None = None):
+                 prep_threads: int | None = None, use_graphs: bool | None = None, chunk_work: int | None = None,
+                 max_skipped: int | None = None):
         p = ServerParams()
         lib().sp_default_server_params(C.byref(p))
         if budget_bytes is not None:
@@ -46,6 +47,8 @@ class Server:
             p.use_graphs = int(bool(use_graphs))
         if chunk_work is not None:
             p.chunk_work = int(chunk_work)
+        if max_skipped is not None:
+            p.max_skipped = int(max_skipped)
         self.b = backend
         h = C.c_void_p()
         backend._ok(lib().sp_server_create(backend.h, C.byref(p), C.byref(h)))

@@ -238,4 +238,5 @@ def test_default_server_params():
 
     p = ServerParams()
     lib().sp_default_server_params(C.byref(p))
-    assert (p.budget_bytes, p.max_active, p.prep_threads, p.use_graphs, p.chunk_work) == (64 << 30, 256, 8, 1, 0)
+    assert (p.budget_bytes, p.max_active, p.prep_threads, p.use_graphs, p.chunk_work, p.max_skipped) == \
+        (64 << 30, 256, 8, 1, 0, 0)
