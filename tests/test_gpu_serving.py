@@ -66,16 +66,17 @@ def test_served_results_are_bitwise_single_solves(gpu, precision):
     s.close()
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_chunk_schedule_changes_timing_not_results(gpu, precision):
+@pytest.mark.parametrize("precision,skipped", [(0, 0), (1, 0), (0, 2)])
+def test_chunk_schedule_changes_timing_not_results(gpu, precision, skipped):
     """With chunk_work = 1 a chunk runs the cheapest LP and those left out of
-    the last 8 chunks, so the batch membership changes from chunk to chunk
-    (tables rebuilt, graphs captured and dropped); every LP still ends with the
-    bits it gets when every LP runs in every chunk."""
+    the last max_skipped chunks (8 by default, or 2), so the batch membership
+    changes from chunk to chunk (tables rebuilt, graphs captured and dropped);
+    every LP still ends with the bits it gets when every LP runs in every
+    chunk."""
     items = problems()
     res = {}
     for work in (1, -1):
-        s = Server(gpu, chunk_work=work)
+        s = Server(gpu, chunk_work=work, max_skipped=skipped)
         ids = [s.submit(p, default_params(precision=precision)) for _, p in items]
         s.drain(ids)
         res[work] = [s.result(i).result for i in ids]
