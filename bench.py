@@ -432,9 +432,9 @@ def large_bench(args) -> dict:
         if tol not in tols:
             break
         res = {}
-        for prec in ("fp64", "mixed"):
+        for prec in [m for m in ("fp64", "mixed") if m in args.modes.split(",")]:
             prm = default_params(tolerance=tol, precision=int(prec == "mixed"), max_iterations=args.max_iterations,
-                                 time_limit_s=limit)
+                                 time_limit_s=limit, log_every_checks=args.log_every)
             t = time.perf_counter()
             r = solve(b, p, prm)
             wall = time.perf_counter() - t
@@ -445,7 +445,7 @@ def large_bench(args) -> dict:
                  "iters_per_s": r.iterations / max(r.solve_time, 1e-9), "objective": r.objective,
                  "kkt_relative_primal": k.relative_primal, "kkt_relative_dual": k.relative_dual,
                  "kkt_relative_gap": k.relative_gap, "fp64_iterations_in_mixed": r.mixed_fp64_iterations}
-            if prec == "mixed":
+            if prec == "mixed" and "fp64" in res:
                 f = res["fp64"]
                 e["equal_iterations"] = r.iterations == f.iterations
                 e["objective_gap_vs_fp64"] = abs(r.objective - f.objective) / (1 + abs(f.objective))
@@ -614,6 +614,8 @@ def main() -> int:
     ap.add_argument("--large", action="store_true",
                     help="config 2 instead: FP64 vs mixed precision at 1e-4 and 1e-6 (use --scale to shrink)")
     ap.add_argument("--max-iterations", type=int, default=200_000)
+    ap.add_argument("--log-every", type=int, default=0,
+                    help="--large: the solver's progress line every this many checks (0: none)")
     ap.add_argument("--time-limit", type=float, default=0.0, help="--large, --full-solve: seconds per solve (0: none)")
     ap.add_argument("--full-solve", action="store_true",
                     help="config 1 solved to 1e-6 on the GPU (KKT from the device checker)")
