@@ -456,12 +456,13 @@ def large_bench(args) -> dict:
             out["runs"].append(e)
         if any(r.status == "Optimal" for r in res.values()):
             tols.append(1e-6)
-    # Host scaling of the same LP, for the setup comparison.
-    t = time.perf_counter()
-    from saddlepoint_amd.backend import ScaledProblem
-    ScaledProblem.new(p)
-    out["host_scaling_s"] = time.perf_counter() - t
-    log(f"  host ScaledProblem::new: {out['host_scaling_s']:.2f}s")
+    if "fp64" in args.modes.split(",") and "mixed" in args.modes.split(","):
+        # Host scaling of the same LP, for the setup comparison.
+        t = time.perf_counter()
+        from saddlepoint_amd.backend import ScaledProblem
+        ScaledProblem.new(p)
+        out["host_scaling_s"] = time.perf_counter() - t
+        log(f"  host ScaledProblem::new: {out['host_scaling_s']:.2f}s")
     b.close()
     return out
 
