@@ -5,9 +5,10 @@
 // iteration of a restart epoch, and the mixed-precision entry into plain
 // iterations (Solver::enter_plain). Here that sequence runs once for a whole
 // group of LPs: each operation becomes one launch (two for an SpMV with long
-// rows, two for a reduction over more than one workgroup) whose table holds
-// every LP's operands. The kernels (sp_c* in kernels.hip) give each LP the
-// partition and summation order of its single-LP launch, so every LP's
+// rows) whose table holds every LP's operands, and all the reductions of a
+// step are one launch (two when an entry spans more than one workgroup), as
+// in Solver::enqueue_check. The kernels (sp_c* in kernels.hip) give each LP
+// the partition and summation order of its single-LP launch, so every LP's
 // results are bitwise those of Solver::advance.
 //
 // All tables of one phase are packed into one blob, copied to the device in
