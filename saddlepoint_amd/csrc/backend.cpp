@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <atomic>
 #include <cstddef>
 #include <memory>
 #include <climits>
@@ -24,20 +25,26 @@ void hip_check(hipError_t e, const char* what) {
 }
 
 namespace {
-thread_local std::unordered_map<const sp_backend*, std::string> t_errors;
+// Per thread: backend -> (its serial, message). The serial tells a backend
+// apart from an earlier one that another thread's entry still names at the
+// same address.
+thread_local std::unordered_map<const sp_backend*, std::pair<uint64_t, std::string>> t_errors;
+std::atomic<uint64_t> g_backend_serial{0};
 }  // namespace
+
+uint64_t next_backend_serial() { return ++g_backend_serial; }
 
 void set_error(const sp_backend* b, const std::string& msg) {
     if (msg.empty()) {
         t_errors.erase(b);
         return;
     }
-    t_errors[b] = msg;
+    t_errors[b] = {b->serial, msg};
 }
 
 const char* last_error(const sp_backend* b) {
     auto it = t_errors.find(b);
-    return it == t_errors.end() ? "" : it->second.c_str();
+    return it == t_errors.end() || it->second.first != b->serial ? "" : it->second.second.c_str();
 }
 
 void forget_errors(const sp_backend* b) { t_errors.erase(b); }
@@ -1667,6 +1674,7 @@ sp_backend* sp_backend_open(int ordinal) {
     if (ordinal < 0 || ordinal >= count)
         throw sp::Error(SP_ERR_UNAVAILABLE, "HIP unavailable: no device " + std::to_string(ordinal));
     auto* b = new sp_backend;
+    b->serial = sp::next_backend_serial();
     b->device = ordinal;
     try {
         sp::hip_check(hipSetDevice(ordinal), "hipSetDevice");

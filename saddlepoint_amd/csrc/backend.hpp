@@ -152,6 +152,7 @@ struct sp_vec32 {
 };
 
 struct sp_backend {
+    uint64_t serial = 0;  // distinct for every backend opened in this process
     int device = 0;
     hipStream_t stream = nullptr;
     hipModule_t module = nullptr;
@@ -215,6 +216,7 @@ void hip_check(hipError_t e, const char* what);
 // after a call that succeeded). Kept per thread, so a call that fails on one
 // thread (a serving submit) never touches the text another thread reads.
 void set_error(const sp_backend* b, const std::string& msg);
+uint64_t next_backend_serial();
 const char* last_error(const sp_backend* b);
 void forget_errors(const sp_backend* b);
 
