@@ -262,7 +262,8 @@ int sp_backend_profile_read(sp_backend *b, int index, char *name, int name_len, 
  * a stream of n 12-byte elements (a double and an int, like a matrix's
  * values and column indices); 5 = gathers only, 6 = stream only, 7 = gather
  * blocks and stream blocks in alternate workgroups, 8 = every workgroup a
- * stream block and then a gather block. */
+ * stream block and then a gather block. A table_len past 2^31 - 1 (int32
+ * indices) returns SP_ERR_TOO_LARGE. */
 int sp_probe(sp_backend *b, int kind, int64_t n, int64_t table_len, int reps, double *ms_per_launch);
 /* No counterpart in the reference: the ceiling a matrix's own access pattern
  * puts on its SpMV's gathers. Average milliseconds of one launch that loads,
