@@ -605,6 +605,8 @@ def main() -> int:
                     help="the LP the iteration rate is measured on (config1: the headline)")
     ap.add_argument("--panels", type=int, default=-1,
                     help="sp_params.column_panels for the iteration bench (-1: the library's choice; 0: off)")
+    ap.add_argument("--hot", type=int, default=-1, choices=[-1, 0, 1],
+                    help="sp_params.hot_table for the iteration bench (-1: the library's choice; 0: off)")
     ap.add_argument("--serving-sample", type=int, default=16,
                     help="LPs of the config-3 stream served beside the headline (0: none)")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
@@ -774,7 +776,8 @@ def iteration_bench(args) -> dict:
     b = HipBackend(0)
     log(f"backend: {b.name()}")
     t0 = time.time()
-    s = Solver(b, p, default_params(tolerance=1e-6, fused=1, precision=int(mixed), column_panels=args.panels))
+    s = Solver(b, p, default_params(tolerance=1e-6, fused=1, precision=int(mixed), column_panels=args.panels,
+                                    hot_table=args.hot))
     log(f"solver ready (scale + load + power iteration) in {time.time() - t0:.1f}s")
     warmup, steps = whole_windows(args.warmup), whole_windows(args.steps)
     s.advance(warmup)
@@ -815,6 +818,11 @@ def iteration_bench(args) -> dict:
         label = f"A side ({'sp_spmv_panel_f32' if mixed else 'sp_spmv_panel_f64'} + {sides[1]} + finishing)"
         kb[label] = kb[sides[1]]
         sides[1] = label
+    hot = [k for k in per_kernel if k.startswith(sides[0] if mixed else sides[0][:-4]) and k != sides[0]]
+    if hot:
+        # The A'-transpose side with a hot table of y in LDS (sp_params.hot_table).
+        kb[hot[0]] = kb[sides[0]]
+        sides[0] = hot[0]
     # Each side's main kernel timed without per-launch events: two events
     # around 20 back-to-back launches on copies of the vectors it writes.
     for side, name in sides.items():
@@ -867,7 +875,7 @@ def iteration_bench(args) -> dict:
         "dtype": "f32+f64" if mixed else "f64",
         "data": "synthetic (seeded generator; random instance of the named shape)",
         "config": {"workload": WORKLOADS[args.workload] + ("; mixed precision" if mixed else ""), "m": m, "n": n,
-                   "column_panels": args.panels,
+                   "column_panels": args.panels, "hot_table": args.hot,
                    "nnz": nnz, "scale": args.scale, "timed_window": f"iterations {it0 + 1}-{it0 + ran}, "
                    f"{ran // CHECK} whole check windows (checks included)", "parallelism": "single GPU"},
         "iteration_gbs": round(b_iter * ran / elapsed / 1e9, 1),

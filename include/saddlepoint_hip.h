@@ -190,6 +190,12 @@ typedef struct {
                           others by their first column (y comes back in the LP's order). Changes the
                           order of the long rows' sums and of sums over rows only. The serving loop
                           runs without panels */
+    int hot_table; /* the A'ᵀ-side plain-iteration kernel keeps y's first H entries in LDS (H = 2048
+                      in FP64, 4096 in mixed precision) and gathers only the others from memory, on
+                      a persistent grid: -1 (default): when at least half of A'ᵀ's nonzeros gather
+                      one of them (A' rows below H, e.g. after the long-rows-first relabeling that
+                      comes with panels) and A' has at least 2^23 nonzeros; 1: the same without the
+                      size condition; 0: off. Bitwise no change. The serving loop runs without it */
 } sp_params;
 
 /* Status (solver.rs:95-102). */

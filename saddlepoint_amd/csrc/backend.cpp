@@ -55,6 +55,10 @@ constexpr int kWG = 256;
 constexpr int kChunk = 4096;  // SP_CHUNK in kernels.hip
 constexpr int kStreamNnzF64 = 2048;  // SP_NNZ_F64 in kernels.hip
 constexpr int kStreamRows = 4 * 256;  // SP_RPT * WG in kernels.hip
+// Hot tables (Hip::plan_hot): matrices of at least this many nonzeros, with
+// at least this share of them gathering an entry of the table.
+constexpr int64_t kHotMinNnz = int64_t(1) << 23;
+constexpr double kHotMinFrac = 0.5;
 
 template <class T>
 T* dmalloc(size_t count) {
@@ -112,6 +116,29 @@ unsigned ew_grid(int64_t n) {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, 8192));
 }
 
+// The main kernel of an SpMV without column panels: `k` one workgroup per
+// work item, or, for a matrix with a hot table and a kernel that has a
+// variant with one, that variant on a persistent grid (as many workgroups as
+// the CUs hold at once; each takes the items gridDim apart).
+template <class... E>
+void spmv_main(sp_backend* b, hipFunction_t k, const DevCsr& a, const void* va, const void* xin, E... epi) {
+    hipFunction_t hk = nullptr;
+    int occ = 0;
+    if (k == b->k_spmv_t_primal_halpern && a.hot64) {
+        hk = b->k_spmv_t_primal_halpern_hot;
+        occ = b->hot_occ64;
+    } else if (k == b->k_spmv_t_primal_mixed && a.hot32) {
+        hk = b->k_spmv_t_primal_mixed_hot;
+        occ = b->hot_occ32;
+    }
+    if (hk && occ > 0) {
+        const int grid = (int)std::min<int64_t>(a.nblocks, (int64_t)occ * std::max(b->cus, 1));
+        launch(b, hk, (unsigned)grid, a.desc, a.rp, a.ci, va, xin, (void*)a.partials, a.nblocks, epi...);
+        return;
+    }
+    launch(b, k, (unsigned)a.nblocks, a.desc, a.rp, a.ci, va, xin, (void*)a.partials, epi...);
+}
+
 // One SpMV with a fused epilogue: the SpMV kernel, then, if the matrix has
 // rows of several chunks, the finishing kernel with the same epilogue
 // arguments.
@@ -135,7 +162,7 @@ void spmv_launch(sp_backend* b, hipFunction_t k, hipFunction_t kfin, const DevCs
         launch(b, kfin, (unsigned)((a.plong + 3) / 4), (const LongRow*)a.plongs, a.plong, (const void*)a.psum, epi...);
         return;
     }
-    if (a.nblocks) launch(b, k, (unsigned)a.nblocks, a.desc, a.rp, a.ci, va, xin, (void*)a.partials, epi...);
+    if (a.nblocks) spmv_main(b, k, a, va, xin, epi...);
     if (a.nlong)
         launch(b, kfin, (unsigned)((a.nlong + 3) / 4), (const LongRow*)a.longs, a.nlong, (const void*)a.partials,
                epi...);
@@ -585,6 +612,27 @@ void Hip::permute_rows(sp_problem* prob, const std::vector<int64_t>& order) {
     A.rp = drp, A.ci = dci, A.va = dva;
     A.desc = nullptr, A.partials = nullptr, A.longs = nullptr;
     plan_csr(b, A, rp_new.data(), t);
+}
+
+void Hip::plan_hot(sp_problem* prob, int mode) {
+    if (!prob) throw Error(SP_ERR_INVALID, "plan_hot: null problem");
+    if (mode < -1 || mode > 1) throw Error(SP_ERR_INVALID, "hot_table must be -1, 0 or 1");
+    DevCsr& T = prob->at;
+    const DevCsr& A = prob->a;
+    T.hot64 = T.hot32 = false;
+    // Worth a persistent grid and its LDS copies only on a large matrix (each
+    // workgroup's copy reads 16 KB: about 12-16 MB per launch, 1% of a
+    // 100M-nonzero pass) and only when most gathers hit the table.
+    if (mode == 0 || T.pmat || T.nblocks == 0 || (mode < 0 && T.nnz < kHotMinNnz)) return;
+    auto frac = [&](int h) {
+        if (A.nrows < h || T.ncols < h) return 0.0;
+        int cnt = 0;
+        hip_check(hipMemcpyAsync(&cnt, A.rp + h, sizeof(int), hipMemcpyDeviceToHost, b->stream), "hipMemcpyAsync D2H");
+        hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        return (double)cnt / (double)T.nnz;
+    };
+    T.hot64 = frac(kHotF64) >= kHotMinFrac;
+    T.hot32 = frac(kHotF32) >= kHotMinFrac;
 }
 
 void Hip::upload(DevCsr& d, const int64_t* row_ptr, const int64_t* col_idx, const double* values, int64_t nrows,
@@ -1129,9 +1177,9 @@ double Hip::plain_kernel_time(const sp_problem* p, int side, bool mixed, const P
                                 (const double*)p->c, (const double*)p->lv, (const double*)p->uv,
                                 (const double*)a.x_anchor->d, a.tau, a.w, a.rho, xb);
                 else
-                    launch(b, b->k_spmv_t_primal_halpern, (unsigned)M.nblocks, M.desc, M.rp, M.ci, (const void*)M.va,
-                           (const void*)a.y->d, (void*)M.partials, x, (const double*)p->c, (const double*)p->lv,
-                           (const double*)p->uv, (const double*)a.x_anchor->d, a.tau, a.w, a.rho, xb);
+                    spmv_main(b, b->k_spmv_t_primal_halpern, M, (const void*)M.va, (const void*)a.y->d, x,
+                              (const double*)p->c, (const double*)p->lv, (const double*)p->uv,
+                              (const double*)a.x_anchor->d, a.tau, a.w, a.rho, xb);
             };
             once();
             ms = time_reps(b, reps, once) / reps;
@@ -1159,10 +1207,10 @@ double Hip::plain_kernel_time(const sp_problem* p, int side, bool mixed, const P
                                 (const double*)p->uv, (const double*)a.atya->d, (const float*)a.ex->d, a.tau, a.w,
                                 a.rho, dxb);
                 else
-                    launch(b, b->k_spmv_t_primal_mixed, (unsigned)M.nblocks, M.desc, M.rp, M.ci, (const void*)M.va32,
-                           (const void*)a.dy->d, (void*)M.partials, dx, (const double*)a.x->d, (const double*)p->c,
-                           (const double*)p->lv, (const double*)p->uv, (const double*)a.atya->d,
-                           (const float*)a.ex->d, a.tau, a.w, a.rho, dxb);
+                    spmv_main(b, b->k_spmv_t_primal_mixed, M, (const void*)M.va32, (const void*)a.dy->d, dx,
+                              (const double*)a.x->d, (const double*)p->c, (const double*)p->lv,
+                              (const double*)p->uv, (const double*)a.atya->d, (const float*)a.ex->d, a.tau, a.w,
+                              a.rho, dxb);
             };
             once();
             ms = time_reps(b, reps, once) / reps;
@@ -1777,6 +1825,14 @@ sp_backend* sp_backend_open(int ordinal) {
         get(b->k_replay32, "sp_probe_replay_f32");
         get(b->k_spmv_t_primal_mixed, "sp_spmv_t_primal_mixed");
         get(b->k_spmv_dual_mixed, "sp_spmv_dual_mixed");
+        get(b->k_spmv_t_primal_halpern_hot, "sp_spmv_t_primal_halpern_hot_f64");
+        get(b->k_spmv_t_primal_mixed_hot, "sp_spmv_t_primal_mixed_hot");
+        sp::hip_check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&b->hot_occ64, b->k_spmv_t_primal_halpern_hot,
+                                                                         sp::kWG, 0),
+                      "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
+        sp::hip_check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&b->hot_occ32, b->k_spmv_t_primal_mixed_hot,
+                                                                         sp::kWG, 0),
+                      "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
         get(b->k_to_f32, "sp_f64_to_f32");
         get(b->k_delta_from, "sp_delta_from");
         get(b->k_delta_to, "sp_delta_to");

@@ -64,7 +64,14 @@ struct DevCsr {
     LongRow* plongs = nullptr;  // {row, l, P, L}
     Desc* pdesc = nullptr;      // the short rows' stream blocks
     int npdesc = 0;
+    // Hot tables (Hip::plan_hot): the plain-iteration kernel of this matrix
+    // keeps x[0, kHotF64) in LDS (FP64), or x[0, kHotF32) (mixed precision).
+    bool hot64 = false, hot32 = false;
 };
+
+// Hot-table sizes (entries): SP_HOT_F64 and SP_HOT_F32 in kernels.hip.
+constexpr int kHotF64 = 2048;
+constexpr int kHotF32 = 4096;
 
 // Rows longer than this are gathered by column panel when panels are on.
 constexpr int kPanelMinRow = 64;
@@ -202,6 +209,11 @@ struct sp_backend {
     hipFunction_t k_probe_copy, k_probe_triad, k_probe_gather, k_probe_copy32, k_probe_triad32, k_replay64,
         k_replay32, k_probe_overlap;
     hipFunction_t k_spmv32, k_spmv_t_primal_mixed, k_spmv_dual_mixed, k_to_f32, k_delta_from, k_delta_to;
+    // The A'ᵀ-side plain-iteration kernels with a hot table of y in LDS, and
+    // the workgroups of each that one CU holds at once (their persistent grid
+    // is that times the CUs).
+    hipFunction_t k_spmv_t_primal_halpern_hot, k_spmv_t_primal_mixed_hot;
+    int hot_occ64 = 0, hot_occ32 = 0;
     hipFunction_t k_panel_count, k_panel_gather, k_spmv_panel, k_fin_panel, k_spmv_panel32, k_fin_panel32;
     hipFunction_t k_rows_gather, k_gather;
     hipFunction_t k_dot, k_dist, k_primal_stats, k_dual_stats, k_step_dual, k_primal_ray, k_dual_ray;
@@ -259,6 +271,12 @@ struct Hip {
     // exist): row i of A' becomes row order[i], the row-indexed vectors
     // follow, and A'ᵀ's column indices are renamed in place.
     void permute_rows(sp_problem* p, const std::vector<int64_t>& order);
+    // Decides whether A'ᵀ's plain-iteration kernels keep a hot table of y:
+    // when most of its nonzeros (A' rows below H, so A'.rp[H] of them) gather
+    // one of y's first H entries, as after long_rows_first on a matrix of
+    // skewed row lengths (mode -1: and the matrix is large; 1: any size; 0:
+    // never). Call after the last change of row order.
+    void plan_hot(sp_problem* p, int mode);
     // Uploads a CSR matrix (int32 indices) and plans it, as load does.
     void upload(DevCsr& d, const int64_t* row_ptr, const int64_t* col_idx, const double* values, int64_t nrows,
                 int64_t ncols, int64_t nnz, const SpmvTuning& tuning);

@@ -96,12 +96,14 @@ RelabeledLp relabel_vectors(const LpView& p, const ColumnOrder& o);
 void relabel(ScaledProblem& s, const ColumnOrder& o);
 
 // A relabeling of the rows, for LPs run with column panels: the rows with
-// more than `long_row` nonzeros first, in their order, then the others by
-// their smallest column index (relabeled by `cols` when it is not empty; an
-// empty row last), ties by index. The long rows' entries of the row-indexed
-// vectors, which the A'ᵀ side gathers for most of its nonzeros when the long
-// rows hold most of them, then share a few cache lines, and the short rows
-// form contiguous stream blocks whose gathers sweep x̄ in order. Each row
+// more than `long_row` nonzeros first, longest first (ties by index), then
+// the others by their smallest column index (relabeled by `cols` when it is
+// not empty; an empty row last), ties by index. The long rows' entries of the
+// row-indexed vectors, which the A'ᵀ side gathers for most of its nonzeros
+// when the long rows hold most of them, then share a few cache lines (the
+// most gathered first, so that the first H of them can be kept in LDS: see
+// Hip::plan_hot), and the short rows form contiguous stream blocks whose
+// gathers sweep x̄ in order. Each row
 // keeps its nonzeros and each column of A'ᵀ its order, so every row sum and
 // every A'ᵀ product is unchanged; only sums over rows change their order.
 struct RowOrder {

@@ -26,6 +26,9 @@ typedef long long i64;
 #define SP_NNZ_F32 4096
 #define SP_CHUNK 4096   // nonzeros of a long row per workgroup
 #define SP_BATCH 4096   // nonzeros a workgroup loads at once
+// Hot tables (spmv_body_hot): entries of the gathered vector kept in LDS.
+#define SP_HOT_F64 2048
+#define SP_HOT_F32 4096
 
 // One SpMV work item. Stream block: rows [r0, r1) whose nonzeros are [k0, k1).
 // Long-row chunk: row r0, nonzeros [k0, k1) of it, r1 = -1 - partial slot.
@@ -122,10 +125,43 @@ __device__ __forceinline__ void block_sum(A (&s)[K]) {
 #define SP_RPT 4                // rows per thread in the epilogue
 #define SP_ROWS (SP_RPT * WG)   // most rows in one stream block
 
-template <int NNZ, class V, class X, class A, class Epi>
+// Where the gathers of a batch read x[c[u]]: HBM (through the caches), or,
+// for the H entries of a hot table, an LDS copy of x[0, H). The value is the
+// same either way, so the choice never changes a result's bits.
+struct GatherGlobal {
+    template <int N, class X>
+    __device__ __forceinline__ void batch(const X* __restrict__ x, const int (&c)[N], X (&v)[N]) const {
+#pragma unroll
+        for (int u = 0; u < N; ++u) v[u] = x[c[u]];
+    }
+};
+
+template <class X, int H>
+struct GatherHot {
+    static_assert((H & (H - 1)) == 0, "H is a power of two");
+    const X* hot;  // LDS: x[0, H)
+    template <int N>
+    __device__ __forceinline__ void batch(const X* __restrict__ x, const int (&c)[N], X (&v)[N]) const {
+        // Every lane reads LDS (cheap), then only the lanes outside the hot
+        // set issue a memory request: each load runs with them alone in the
+        // wave's mask, and a wave with none skips it. The empty asm makes the
+        // LDS values opaque (one wait for all of them), so the compiler
+        // cannot merge the two reads of an element into one load through a
+        // generic (flat) pointer, which would wait on both memory counters.
+#pragma unroll
+        for (int u = 0; u < N; ++u) v[u] = hot[c[u] & (H - 1)];
+#pragma unroll
+        for (int u = 0; u < N; ++u) asm volatile("" : "+v"(v[u]));
+#pragma unroll
+        for (int u = 0; u < N; ++u)
+            if (c[u] >= H) v[u] = x[c[u]];
+    }
+};
+
+template <int NNZ, class V, class X, class A, class Epi, class G = GatherGlobal>
 __device__ __forceinline__ void spmv_block(const Desc d, const int* __restrict__ rp, const int* __restrict__ ci,
                                            const V* __restrict__ va, const X* __restrict__ x,
-                                           A* __restrict__ partials, const Epi& epi) {
+                                           A* __restrict__ partials, const Epi& epi, const G& gat = G{}) {
     __shared__ A prod[NNZ];
     __shared__ A sums[SP_ROWS];
     __shared__ int rend[SP_ROWS];
@@ -160,8 +196,7 @@ __device__ __forceinline__ void spmv_block(const Desc d, const int* __restrict__
         }
         if (n > 0) {
             X xv[PER];
-#pragma unroll
-            for (int u = 0; u < PER; ++u) xv[u] = x[c[u]];
+            gat.batch(x, c, xv);
 #pragma unroll
             for (int u = 0; u < PER; ++u)
                 if (tid + u * WG < n) prod[tid + u * WG] = (A)v[u] * (A)xv[u];
@@ -235,8 +270,7 @@ __device__ __forceinline__ void spmv_block(const Desc d, const int* __restrict__
 #pragma unroll
             for (int u = 0; u < PERL; ++u) v[u] = va[base + min(tid + u * WG, lastl)];
             X xv[PERL];
-#pragma unroll
-            for (int u = 0; u < PERL; ++u) xv[u] = x[c[u]];
+            gat.batch(x, c, xv);
 #pragma unroll
             for (int u = 0; u < PERL; ++u)
                 if (tid + u * WG < n) s[0] += (A)v[u] * (A)xv[u];
@@ -259,6 +293,27 @@ __device__ __forceinline__ void spmv_body(const Desc* __restrict__ desc, const i
                                           const int* __restrict__ ci, const V* __restrict__ va,
                                           const X* __restrict__ x, A* __restrict__ partials, const Epi& epi) {
     spmv_block<NNZ, V, X, A, Epi>(desc[blockIdx.x], rp, ci, va, x, partials, epi);
+}
+
+// The same work items with a hot table: each workgroup copies x[0, H) into
+// LDS once and then takes work items ndesc apart (a grid of a few workgroups
+// per CU, so the copy is paid a few hundred times, not once per item). For a
+// matrix whose column indices are mostly below H (A'ᵀ once the rows of A' are
+// relabeled long rows first: the first rows of A' are the most referenced
+// entries of y), most gathers then read LDS and only the rest go to memory.
+// Each item is computed exactly as by spmv_body, so the sums are the same.
+template <int NNZ, int H, class V, class X, class A, class Epi>
+__device__ __forceinline__ void spmv_body_hot(const Desc* __restrict__ desc, int ndesc, const int* __restrict__ rp,
+                                              const int* __restrict__ ci, const V* __restrict__ va,
+                                              const X* __restrict__ x, A* __restrict__ partials, const Epi& epi) {
+    __shared__ X hot[H];
+    for (int i = threadIdx.x; i < H; i += WG) hot[i] = x[i];  // the host launches this only when len(x) >= H
+    __syncthreads();
+    const GatherHot<X, H> g{hot};
+    for (int b = blockIdx.x; b < ndesc; b += gridDim.x) {
+        spmv_block<NNZ, V, X, A, Epi>(desc[b], rp, ci, va, x, partials, epi, g);
+        __syncthreads();  // the next item reuses the block's LDS
+    }
 }
 
 // The finishing pass for rows of several chunks: one wave per row adds the
@@ -396,6 +451,19 @@ extern "C" __global__ __launch_bounds__(WG) void sp_spmv_t_primal_halpern_f64(
     double* __restrict__ x_bar) {
     spmv_body<SP_NNZ_F64, double, double, double>(desc, rp, ci, va, xin, partials,
                                                   EpiPrimalHalpern{x, c, lv, uv, anchor, tau, w, rho, x_bar});
+}
+
+// A'ᵀ side of a plain iteration with a hot table of y (spmv_body_hot): 2,048
+// entries (16 KB, 47 KB of LDS in all, three workgroups per CU). Measured on
+// a 4M x 10M matrix of Zipf(1.5) row lengths, rows long rows first: a larger
+// table (two workgroups per CU) gains nothing, a smaller one at four per CU
+// slightly less.
+extern "C" __global__ __launch_bounds__(WG) void sp_spmv_t_primal_halpern_hot_f64(
+    SPMV_ARGS, int ndesc, double* __restrict__ x, const double* __restrict__ c, const double* __restrict__ lv,
+    const double* __restrict__ uv, const double* __restrict__ anchor, double tau, double w, double rho,
+    double* __restrict__ x_bar) {
+    spmv_body_hot<SP_NNZ_F64, SP_HOT_F64, double, double, double>(
+        desc, ndesc, rp, ci, va, xin, partials, EpiPrimalHalpern{x, c, lv, uv, anchor, tau, w, rho, x_bar});
 }
 
 extern "C" __global__ __launch_bounds__(WG) void sp_spmv_dual_f64(SPMV_ARGS, const double* __restrict__ y,
@@ -609,6 +677,16 @@ extern "C" __global__ __launch_bounds__(WG) void sp_spmv_t_primal_mixed(
     const float* __restrict__ ex, double tau, double w, double rho, float* __restrict__ dx_bar) {
     spmv_body<SP_NNZ_F64, float, float, float>(desc, rp, ci, va, xin, partials,
                                                EpiPrimalMixed{dx, xa, c, lv, uv, atya, ex, tau, w, rho, dx_bar});
+}
+
+// The same in mixed precision: 4,096 FP32 entries (16 KB, 34 KB of LDS in
+// all, four workgroups per CU; 8,192 entries at three per CU are slower).
+extern "C" __global__ __launch_bounds__(WG) void sp_spmv_t_primal_mixed_hot(
+    SPMV32_ARGS, int ndesc, float* __restrict__ dx, const double* __restrict__ xa, const double* __restrict__ c,
+    const double* __restrict__ lv, const double* __restrict__ uv, const double* __restrict__ atya,
+    const float* __restrict__ ex, double tau, double w, double rho, float* __restrict__ dx_bar) {
+    spmv_body_hot<SP_NNZ_F64, SP_HOT_F32, float, float, float>(
+        desc, ndesc, rp, ci, va, xin, partials, EpiPrimalMixed{dx, xa, c, lv, uv, atya, ex, tau, w, rho, dx_bar});
 }
 
 extern "C" __global__ __launch_bounds__(WG) void sp_spmv_dual_mixed(

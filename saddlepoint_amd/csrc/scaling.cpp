@@ -247,6 +247,10 @@ RowOrder long_rows_first(const LpView& p, const ColumnOrder& cols, int64_t long_
         key[(size_t)i] = kmin;
         start[(size_t)kmin + 1] += 1;
     }
+    // The long rows by decreasing length, ties by index.
+    std::stable_sort(o.order.begin(), o.order.end(), [&](int64_t a, int64_t b) {
+        return p.row_ptr[a + 1] - p.row_ptr[a] > p.row_ptr[b + 1] - p.row_ptr[b];
+    });
     for (size_t b = 1; b < start.size(); ++b) start[b] += start[b - 1];
     const size_t base = o.order.size();
     o.order.resize((size_t)m);

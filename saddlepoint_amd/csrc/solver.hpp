@@ -116,6 +116,7 @@ class Solver {
             b_.permute_rows(dev_.get(), rows_.order);
             b_.set_panels(dev_.get(), shift);
         }
+        b_.plan_hot(dev_.get(), params.hot_table);
         // The scaled matrices live on the device now; the host keeps the factors.
         scaled_.a = Csr{};
         scaled_.at = Csr{};

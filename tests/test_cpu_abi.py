@@ -240,3 +240,14 @@ def test_default_server_params():
     lib().sp_default_server_params(C.byref(p))
     assert (p.budget_bytes, p.max_active, p.prep_threads, p.use_graphs, p.chunk_work, p.max_skipped) == \
         (64 << 30, 256, 8, 1, 0, 0)
+
+
+def test_default_solver_params_execution_options():
+    """The execution options beyond SolverParams: fused path, FP64, the default
+    plan, device scaling, column relabeling, no stall switch, and automatic
+    column panels and hot tables."""
+    from saddlepoint_amd.backend import default_params
+
+    p = default_params()
+    assert (p.fused, p.precision, p.spmv_autotune, p.gpu_scaling, p.reorder_columns, p.mixed_stall_checks,
+            p.column_panels, p.hot_table) == (1, 0, 0, 1, 1, 0, -1, -1)

@@ -268,3 +268,26 @@ def test_column_panels_solve_like_the_oracle(gpu):
     assert b.status == O.STATUS[c.status] and b.iterations == c.iterations and b.restarts == c.restarts
     assert dx < 1e-8 and dy < 1e-8
     assert mx.iterations == cap and np.all(np.isfinite(mx.x))
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_hot_table_changes_no_bits(gpu, precision):
+    """With rows relabeled long rows first (panels forced on a small wide LP),
+    93% of A'ᵀ's nonzeros gather one of y's first 2,048 entries, so the
+    A'-transpose kernel can keep them in LDS on a persistent grid
+    (hot_table = 1 skips the size condition). The solve is bitwise the one
+    without the table, and the profile shows the table's kernel ran."""
+    p = generate.config2(scale=0.02)
+    kw = dict(max_iterations=640, precision=precision, column_panels=10)
+    a = solve(gpu, p, default_params(hot_table=0, **kw))
+    gpu.profile(True)
+    b = solve(gpu, p, default_params(hot_table=1, **kw))
+    prof = gpu.profile_read()
+    gpu.profile(False)
+    main = "sp_spmv_t_primal_mixed" if precision else "sp_spmv_t_primal_halpern_f64"
+    ran = sorted(k for k, (_, n) in prof.items() if n and k.startswith(main[:-4] if not precision else main))
+    print(f"precision {precision}: {a.status} {a.iterations} it / {b.status} {b.iterations} it; A'T kernels {ran}",
+          flush=True)
+    assert any(k != main for k in ran) and main not in ran, ran
+    assert a.status == b.status and a.iterations == b.iterations == 640 and a.restarts == b.restarts
+    assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
