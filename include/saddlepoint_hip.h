@@ -193,9 +193,12 @@ typedef struct {
     int hot_table; /* the A'ᵀ-side plain-iteration kernel keeps y's first H entries in LDS (H = 2048
                       in FP64, 4096 in mixed precision) and gathers only the others from memory, on
                       a persistent grid: -1 (default): when at least half of A'ᵀ's nonzeros gather
-                      one of them (A' rows below H, e.g. after the long-rows-first relabeling that
-                      comes with panels) and A' has at least 2^23 nonzeros; 1: the same without the
-                      size condition; 0: off. Bitwise no change. The serving loop runs without it */
+                      one of them (A' rows below H) and A' has at least 2^23 nonzeros; 1: the same
+                      without the size condition; 0: off. Without panels, when the 2048 longest rows
+                      hold at least half of the nonzeros, the driver then relabels the rows as
+                      panels do (longest first; y comes back in the LP's order), which changes the
+                      order of sums over rows only; the kernel itself changes no bits. The serving
+                      loop relabels the same way; its batched kernels have no hot table */
 } sp_params;
 
 /* Status (solver.rs:95-102). */

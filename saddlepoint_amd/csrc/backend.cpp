@@ -55,10 +55,6 @@ constexpr int kWG = 256;
 constexpr int kChunk = 4096;  // SP_CHUNK in kernels.hip
 constexpr int kStreamNnzF64 = 2048;  // SP_NNZ_F64 in kernels.hip
 constexpr int kStreamRows = 4 * 256;  // SP_RPT * WG in kernels.hip
-// Hot tables (Hip::plan_hot): matrices of at least this many nonzeros, with
-// at least this share of them gathering an entry of the table.
-constexpr int64_t kHotMinNnz = int64_t(1) << 23;
-constexpr double kHotMinFrac = 0.5;
 
 template <class T>
 T* dmalloc(size_t count) {

@@ -72,6 +72,11 @@ struct DevCsr {
 // Hot-table sizes (entries): SP_HOT_F64 and SP_HOT_F32 in kernels.hip.
 constexpr int kHotF64 = 2048;
 constexpr int kHotF32 = 4096;
+// Hot tables are used (Hip::plan_hot) on matrices of at least kHotMinNnz
+// nonzeros (hot_table = -1) with at least kHotMinFrac of them gathering an
+// entry of the table.
+constexpr int64_t kHotMinNnz = int64_t(1) << 23;
+constexpr double kHotMinFrac = 0.5;
 
 // Rows longer than this are gathered by column panel when panels are on.
 constexpr int kPanelMinRow = 64;

@@ -196,7 +196,6 @@ class Server {
         // The batched kernels run each LP's plain plan, so its single-solve
         // counterpart is the one without column panels.
         job->params.column_panels = 0;
-        job->params.hot_table = 0;  // (bitwise no change; the batched kernels have no hot table)
         // Limits past the largest multiple of 64 (u64::MAX, "no limit") become it.
         const uint64_t last = UINT64_MAX / kChunk * kChunk;
         job->params.max_iterations =

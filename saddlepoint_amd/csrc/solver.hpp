@@ -93,6 +93,7 @@ class Solver {
     Solver(B& b, const LpView& problem, const sp_params& params, ScaledProblem scaled, ColumnOrder relabeled = {})
         : b_(b), p_(params), scaled_(std::move(scaled)), cols_(std::move(relabeled)), m_(problem.m), n_(problem.n),
           nnz_(problem.nnz), c0_(problem.c0), maximize_(problem.maximize) {
+        if (params.hot_table < -1 || params.hot_table > 1) throw Error(SP_ERR_INVALID, "hot_table must be -1, 0 or 1");
         b_norm_ = finite_bound_norm(problem.lc, problem.uc, problem.m);
         c_norm_ = norm2(problem.c, problem.n);
         // Columns relabeled by popularity (the solution is returned in the
@@ -115,6 +116,13 @@ class Solver {
             rows_ = long_rows_first(problem, cols_, kPanelMinRow);
             b_.permute_rows(dev_.get(), rows_.order);
             b_.set_panels(dev_.get(), shift);
+        } else if (params.hot_table != 0 && m_ >= kHotF64 && (params.hot_table > 0 || nnz_ >= kHotMinNnz) &&
+                   (double)longest_rows_nnz(problem, kHotF64) >= kHotMinFrac * (double)nnz_) {
+            // Rows of skewed lengths without panels: the same relabeling, so
+            // that the most gathered entries of y come first for the hot
+            // table (and share cache lines without it).
+            rows_ = long_rows_first(problem, cols_, kPanelMinRow);
+            b_.permute_rows(dev_.get(), rows_.order);
         }
         b_.plan_hot(dev_.get(), params.hot_table);
         // The scaled matrices live on the device now; the host keeps the factors.
@@ -978,7 +986,7 @@ class Solver {
     const sp_params p_;
     ScaledProblem scaled_;
     ColumnOrder cols_;  // empty unless params.reorder_columns
-    RowOrder rows_;     // empty unless column panels are on
+    RowOrder rows_;     // empty unless the rows are relabeled (column panels or a hot table)
     int64_t m_, n_, nnz_;
     double c0_;
     bool maximize_;

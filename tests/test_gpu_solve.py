@@ -291,3 +291,42 @@ def test_hot_table_changes_no_bits(gpu, precision):
     assert any(k != main for k in ran) and main not in ran, ran
     assert a.status == b.status and a.iterations == b.iterations == 640 and a.restarts == b.restarts
     assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
+
+
+def test_skewed_rows_relabeled_for_the_hot_table(gpu):
+    """An LP whose 2,048 longest rows hold 96% of the nonzeros (Zipf(1.5) row
+    lengths, 30k rows, no panels): hot_table = 1 (the size condition skipped)
+    relabels its rows longest first and runs the A'-transpose side with the
+    hot table. Capped, the solve keeps the oracle's status, iteration and
+    restart counts with iterates within 1e-8 (the relabeling reorders sums
+    over rows only); the fused path with the table is bitwise the method path
+    under the same relabeling, and mixed precision runs."""
+    p = generate.serving_lp(generate.seed_of(4, 7001), 20_000, 40_000, zipf_rows=1.5)
+    cap = 1280
+    c, cx, cy = O.solve(p, O.default_params(max_iterations=cap))
+    a = solve(gpu, p, default_params(max_iterations=cap, fused=0, hot_table=1))
+    gpu.profile(True)
+    b = solve(gpu, p, default_params(max_iterations=cap, fused=1, hot_table=1))
+    mx = solve(gpu, p, default_params(max_iterations=cap, precision=1, hot_table=1))
+    prof = gpu.profile_read()
+    gpu.profile(False)
+    dx = np.max(np.abs(b.x - cx)) / (1 + np.max(np.abs(cx)))
+    dy = np.max(np.abs(b.y - cy)) / (1 + np.max(np.abs(cy)))
+    print(f"skewed m={p.nrows} nnz={p.a.nnz}: oracle {O.STATUS[c.status]} {c.iterations} it {c.restarts} restarts; "
+          f"gpu {b.status} {b.iterations} it {b.restarts} restarts; x diff {dx:.2e} y diff {dy:.2e}; "
+          f"mixed {mx.status} {mx.iterations} it", flush=True)
+    for k in ("sp_spmv_t_primal_halpern_hot_f64", "sp_spmv_t_primal_mixed_hot"):
+        assert prof.get(k, (0.0, 0))[1] > 0, (k, sorted(prof))
+    assert a.status == b.status and a.iterations == b.iterations and a.restarts == b.restarts
+    assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
+    assert b.status == O.STATUS[c.status] and b.iterations == c.iterations and b.restarts == c.restarts
+    assert dx < 1e-8 and dy < 1e-8
+    assert mx.iterations == cap and np.all(np.isfinite(mx.x))
+
+
+def test_hot_table_rejects_other_values(gpu):
+    from saddlepoint_amd._lib import SP_ERR_INVALID, BackendError
+
+    with pytest.raises(BackendError) as e:
+        solve(gpu, netlib("afiro"), default_params(hot_table=2))
+    assert e.value.code == SP_ERR_INVALID
