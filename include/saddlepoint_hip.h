@@ -193,12 +193,15 @@ typedef struct {
     int hot_table; /* the A'ᵀ-side plain-iteration kernel keeps y's first H entries in LDS (H = 2048
                       in FP64, 4096 in mixed precision) and gathers only the others from memory, on
                       a persistent grid: -1 (default): when at least half of A'ᵀ's nonzeros gather
-                      one of them (A' rows below H) and A' has at least 2^23 nonzeros; 1: the same
-                      without the size condition; 0: off. Without panels, when the 2048 longest rows
-                      hold at least half of the nonzeros, the driver then relabels the rows as
-                      panels do (longest first; y comes back in the LP's order), which changes the
-                      order of sums over rows only; the kernel itself changes no bits. The serving
-                      loop relabels the same way; its batched kernels have no hot table */
+                      one of them (A' rows below H: skewed row lengths, rows relabeled) and A' has
+                      at least 2^23 nonzeros; 1: the same without the size condition; 0: off.
+                      Bitwise no change. The serving loop's batched kernels have no hot table */
+    int reorder_rows; /* relabel the rows of A' as column panels do (rows of more than 64 nonzeros
+                         first, longest first, then the others by their smallest column; y comes
+                         back in the LP's order), so that each wave's gathers of y and x̄ fall on
+                         fewer cache lines: -1 (default): for LPs of at least 2^20 nonzeros; 1:
+                         always; 0: only with column panels (which need it). Changes the order of
+                         sums over rows only. Served LPs are relabeled as their single solves */
 } sp_params;
 
 /* Status (solver.rs:95-102). */

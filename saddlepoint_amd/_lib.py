@@ -70,7 +70,7 @@ class Params(C.Structure):
                 ("bound_objective_rescaling", C.c_int), ("termination", C.c_int), ("fused", C.c_int),
                 ("log_every_checks", C.c_int), ("precision", C.c_int), ("spmv_autotune", C.c_int),
                 ("gpu_scaling", C.c_int), ("reorder_columns", C.c_int), ("mixed_stall_checks", _i64),
-                ("column_panels", C.c_int), ("hot_table", C.c_int)]
+                ("column_panels", C.c_int), ("hot_table", C.c_int), ("reorder_rows", C.c_int)]
 
 
 class Solution(C.Structure):

@@ -293,6 +293,7 @@ void sp_default_params(sp_params* p) {
     p->reorder_columns = 1;
     p->column_panels = -1;
     p->hot_table = -1;
+    p->reorder_rows = -1;
 }
 
 int sp_solve(sp_backend* b, const sp_lp_view* lp, const sp_params* params, sp_solution* out, double* x_out,

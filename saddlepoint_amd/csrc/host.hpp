@@ -95,8 +95,8 @@ RelabeledLp relabel_vectors(const LpView& p, const ColumnOrder& o);
 // The same relabeling applied to a scaled problem built from the original LP.
 void relabel(ScaledProblem& s, const ColumnOrder& o);
 
-// A relabeling of the rows, for LPs run with column panels or with a hot
-// table of y (when the longest rows hold most of the nonzeros): the rows with
+// A relabeling of the rows, for LPs run with column panels and for large LPs
+// (sp_params.reorder_rows): the rows with
 // more than `long_row` nonzeros first, longest first (ties by index), then
 // the others by their smallest column index (relabeled by `cols` when it is
 // not empty; an empty row last), ties by index. The long rows' entries of the
@@ -112,8 +112,8 @@ struct RowOrder {
     bool empty() const { return order.empty(); }
 };
 RowOrder long_rows_first(const LpView& p, const ColumnOrder& cols, int64_t long_row);
-// The nonzeros of the h longest rows of A' (all of them if m <= h).
-int64_t longest_rows_nnz(const LpView& p, int64_t h);
+// reorder_rows = -1 relabels the rows of LPs with at least this many nonzeros.
+constexpr int64_t kReorderRowsMinNnz = int64_t(1) << 20;
 
 }  // namespace sp
 

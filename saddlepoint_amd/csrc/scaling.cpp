@@ -4,7 +4,6 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
-#include <functional>
 #include <numeric>
 
 #include "host.hpp"
@@ -258,16 +257,6 @@ RowOrder long_rows_first(const LpView& p, const ColumnOrder& cols, int64_t long_
     for (int64_t i = 0; i < m; ++i)
         if (key[(size_t)i] >= 0) o.order[base + (size_t)start[(size_t)key[(size_t)i]]++] = i;
     return o;
-}
-
-int64_t longest_rows_nnz(const LpView& p, int64_t h) {
-    if (p.m <= h) return p.nnz;
-    std::vector<int64_t> len((size_t)p.m);
-    for (int64_t i = 0; i < p.m; ++i) len[(size_t)i] = p.row_ptr[i + 1] - p.row_ptr[i];
-    std::nth_element(len.begin(), len.begin() + h, len.end(), std::greater<int64_t>());
-    int64_t s = 0;
-    for (int64_t i = 0; i < h; ++i) s += len[(size_t)i];
-    return s;
 }
 
 ColumnOrder popularity_order_of_counts(const std::vector<int64_t>& count) {

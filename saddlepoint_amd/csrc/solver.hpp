@@ -94,6 +94,8 @@ class Solver {
         : b_(b), p_(params), scaled_(std::move(scaled)), cols_(std::move(relabeled)), m_(problem.m), n_(problem.n),
           nnz_(problem.nnz), c0_(problem.c0), maximize_(problem.maximize) {
         if (params.hot_table < -1 || params.hot_table > 1) throw Error(SP_ERR_INVALID, "hot_table must be -1, 0 or 1");
+        if (params.reorder_rows < -1 || params.reorder_rows > 1)
+            throw Error(SP_ERR_INVALID, "reorder_rows must be -1, 0 or 1");
         b_norm_ = finite_bound_norm(problem.lc, problem.uc, problem.m);
         c_norm_ = norm2(problem.c, problem.n);
         // Columns relabeled by popularity (the solution is returned in the
@@ -110,20 +112,16 @@ class Solver {
             dev_ = b_.load(scaled_, params.spmv_autotune != 0);
         }
         // Column panels for the long rows of a wide A' (the gathered x̄ far
-        // beyond an XCD's L2), with the rows relabeled long rows first.
+        // beyond an XCD's L2), with the rows relabeled long rows first; large
+        // LPs get the same row relabeling without panels (reorder_rows), so
+        // that each wave's gathers fall on fewer cache lines and the most
+        // gathered entries of y come first, for the A'ᵀ side's hot table.
         const int shift = params.column_panels >= 0 ? params.column_panels : (n_ >= (int64_t(1) << 21) ? 16 : 0);
-        if (shift) {
-            rows_ = long_rows_first(problem, cols_, kPanelMinRow);
-            b_.permute_rows(dev_.get(), rows_.order);
-            b_.set_panels(dev_.get(), shift);
-        } else if (params.hot_table != 0 && m_ >= kHotF64 && (params.hot_table > 0 || nnz_ >= kHotMinNnz) &&
-                   (double)longest_rows_nnz(problem, kHotF64) >= kHotMinFrac * (double)nnz_) {
-            // Rows of skewed lengths without panels: the same relabeling, so
-            // that the most gathered entries of y come first for the hot
-            // table (and share cache lines without it).
+        if (shift || params.reorder_rows > 0 || (params.reorder_rows < 0 && nnz_ >= kReorderRowsMinNnz)) {
             rows_ = long_rows_first(problem, cols_, kPanelMinRow);
             b_.permute_rows(dev_.get(), rows_.order);
         }
+        if (shift) b_.set_panels(dev_.get(), shift);
         b_.plan_hot(dev_.get(), params.hot_table);
         // The scaled matrices live on the device now; the host keeps the factors.
         scaled_.a = Csr{};
@@ -986,7 +984,7 @@ class Solver {
     const sp_params p_;
     ScaledProblem scaled_;
     ColumnOrder cols_;  // empty unless params.reorder_columns
-    RowOrder rows_;     // empty unless the rows are relabeled (column panels or a hot table)
+    RowOrder rows_;     // empty unless the rows are relabeled (column panels or reorder_rows)
     int64_t m_, n_, nnz_;
     double c0_;
     bool maximize_;

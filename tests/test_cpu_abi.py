@@ -245,9 +245,9 @@ def test_default_server_params():
 def test_default_solver_params_execution_options():
     """The execution options beyond SolverParams: fused path, FP64, the default
     plan, device scaling, column relabeling, no stall switch, and automatic
-    column panels and hot tables."""
+    column panels, hot tables and row relabeling."""
     from saddlepoint_amd.backend import default_params
 
     p = default_params()
     assert (p.fused, p.precision, p.spmv_autotune, p.gpu_scaling, p.reorder_columns, p.mixed_stall_checks,
-            p.column_panels, p.hot_table) == (1, 0, 0, 1, 1, 0, -1, -1)
+            p.column_panels, p.hot_table, p.reorder_rows) == (1, 0, 0, 1, 1, 0, -1, -1, -1)

@@ -44,18 +44,19 @@ def test_alone_and_in_the_mix_are_bitwise_equal(gpu, precision):
         assert np.array_equal(a.x, t.x) and np.array_equal(a.y, t.y)
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_served_results_are_bitwise_single_solves(gpu, precision):
+@pytest.mark.parametrize("precision,rows", [(0, -1), (1, -1), (0, 1)])
+def test_served_results_are_bitwise_single_solves(gpu, precision, rows):
     """The batched plain iterations and the batched check phases run each LP's
     own partition and summation order, so a served LP takes exactly the
-    iterations of its single solve: same status, counts and bits."""
+    iterations of its single solve: same status, counts and bits (also with
+    every LP's rows relabeled, reorder_rows = 1)."""
     items = problems()
     s = Server(gpu)
-    ids = [s.submit(p, default_params(precision=precision)) for _, p in items]
+    ids = [s.submit(p, default_params(precision=precision, reorder_rows=rows)) for _, p in items]
     s.drain(ids)
     for (name, p), i in zip(items, ids):
         r = s.result(i).result
-        g = solve(gpu, p, default_params(precision=precision))
+        g = solve(gpu, p, default_params(precision=precision, reorder_rows=rows))
         print(f"{name:>10}: served {r.status} {r.iterations} it; solve {g.status} {g.iterations} it; "
               f"bitwise {np.array_equal(r.x, g.x)}", flush=True)
         assert r.status == g.status and r.iterations == g.iterations and r.restarts == g.restarts, name

@@ -183,6 +183,34 @@ def test_column_relabeling_changes_storage_not_results(gpu, name):
         assert abs(a.objective - b.objective) / (1 + abs(a.objective)) <= 2 * TOL
 
 
+@pytest.mark.parametrize("name", ["israel", "shell", "woodinfe", "config0"])
+def test_row_relabeling_changes_storage_not_results(gpu, name):
+    """reorder_rows = 1 relabels the rows (rows of more than 64 nonzeros first,
+    longest first, then the others by their smallest column; by default only
+    LPs of 2^20 nonzeros or more): y comes back in the LP's order and the
+    solve agrees with the unrelabeled one (sums over rows add in another
+    order, so iteration counts agree within 1%), and the fused path is still
+    bitwise the method path."""
+    p = generate.config0() if name == "config0" else netlib(name)
+    a = solve(gpu, p, default_params(reorder_rows=0))
+    b = solve(gpu, p, default_params(reorder_rows=1))
+    c = solve(gpu, p, default_params(reorder_rows=1, fused=0))
+    print(f"{name}: plain order {a.status} {a.iterations} it; relabeled {b.status} {b.iterations} it", flush=True)
+    assert a.status == b.status and iterations_match(a.iterations, b.iterations)
+    assert b.iterations == c.iterations and np.array_equal(b.x, c.x) and np.array_equal(b.y, c.y)
+    if a.status == "Optimal":
+        O.check(p, b.x, b.y).verify(2 * TOL)
+        assert abs(a.objective - b.objective) / (1 + abs(a.objective)) <= 2 * TOL
+
+
+def test_reorder_rows_rejects_other_values(gpu):
+    from saddlepoint_amd._lib import SP_ERR_INVALID, BackendError
+
+    with pytest.raises(BackendError) as e:
+        solve(gpu, netlib("afiro"), default_params(reorder_rows=3))
+    assert e.value.code == SP_ERR_INVALID
+
+
 def edge_cases():
     import math
 
@@ -295,12 +323,13 @@ def test_hot_table_changes_no_bits(gpu, precision):
 
 def test_skewed_rows_relabeled_for_the_hot_table(gpu):
     """An LP whose 2,048 longest rows hold 96% of the nonzeros (Zipf(1.5) row
-    lengths, 30k rows, no panels): hot_table = 1 (the size condition skipped)
-    relabels its rows longest first and runs the A'-transpose side with the
-    hot table. Capped, the solve keeps the oracle's status, iteration and
-    restart counts with iterates within 1e-8 (the relabeling reorders sums
-    over rows only); the fused path with the table is bitwise the method path
-    under the same relabeling, and mixed precision runs."""
+    lengths, 30k rows, 1.2M nonzeros, no panels): its rows are relabeled
+    longest first (reorder_rows, on from 2^20 nonzeros), and hot_table = 1
+    (the size condition skipped) runs the A'-transpose side with the hot
+    table. Capped, the solve keeps the oracle's status, iteration and restart
+    counts with iterates within 1e-8 (the relabeling reorders sums over rows
+    only); the fused path with the table is bitwise the method path under the
+    same relabeling, and mixed precision runs."""
     p = generate.serving_lp(generate.seed_of(4, 7001), 20_000, 40_000, zipf_rows=1.5)
     cap = 1280
     c, cx, cy = O.solve(p, O.default_params(max_iterations=cap))
