@@ -607,6 +607,8 @@ def main() -> int:
                     help="sp_params.column_panels for the iteration bench (-1: the library's choice; 0: off)")
     ap.add_argument("--hot", type=int, default=-1, choices=[-1, 0, 1],
                     help="sp_params.hot_table for the iteration bench (-1: the library's choice; 0: off)")
+    ap.add_argument("--rows", type=int, default=-1, choices=[-1, 0, 1],
+                    help="sp_params.reorder_rows for the iteration bench (-1: the library's choice; 0: off)")
     ap.add_argument("--serving-sample", type=int, default=16,
                     help="LPs of the config-3 stream served beside the headline (0: none)")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
@@ -777,7 +779,7 @@ def iteration_bench(args) -> dict:
     log(f"backend: {b.name()}")
     t0 = time.time()
     s = Solver(b, p, default_params(tolerance=1e-6, fused=1, precision=int(mixed), column_panels=args.panels,
-                                    hot_table=args.hot))
+                                    hot_table=args.hot, reorder_rows=args.rows))
     log(f"solver ready (scale + load + power iteration) in {time.time() - t0:.1f}s")
     warmup, steps = whole_windows(args.warmup), whole_windows(args.steps)
     s.advance(warmup)
@@ -875,7 +877,7 @@ def iteration_bench(args) -> dict:
         "dtype": "f32+f64" if mixed else "f64",
         "data": "synthetic (seeded generator; random instance of the named shape)",
         "config": {"workload": WORKLOADS[args.workload] + ("; mixed precision" if mixed else ""), "m": m, "n": n,
-                   "column_panels": args.panels, "hot_table": args.hot,
+                   "column_panels": args.panels, "hot_table": args.hot, "reorder_rows": args.rows,
                    "nnz": nnz, "scale": args.scale, "timed_window": f"iterations {it0 + 1}-{it0 + ran}, "
                    f"{ran // CHECK} whole check windows (checks included)", "parallelism": "single GPU"},
         "iteration_gbs": round(b_iter * ran / elapsed / 1e9, 1),
