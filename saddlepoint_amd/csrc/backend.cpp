@@ -1823,12 +1823,14 @@ sp_backend* sp_backend_open(int ordinal) {
         get(b->k_spmv_dual_mixed, "sp_spmv_dual_mixed");
         get(b->k_spmv_t_primal_halpern_hot, "sp_spmv_t_primal_halpern_hot_f64");
         get(b->k_spmv_t_primal_mixed_hot, "sp_spmv_t_primal_mixed_hot");
-        sp::hip_check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&b->hot_occ64, b->k_spmv_t_primal_halpern_hot,
-                                                                         sp::kWG, 0),
-                      "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
-        sp::hip_check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&b->hot_occ32, b->k_spmv_t_primal_mixed_hot,
-                                                                         sp::kWG, 0),
-                      "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
+        // Their persistent grids: workgroups per CU (0, if the runtime cannot
+        // say, keeps the plain kernels).
+        if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&b->hot_occ64, b->k_spmv_t_primal_halpern_hot, sp::kWG,
+                                                               0) != hipSuccess)
+            b->hot_occ64 = 0;
+        if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&b->hot_occ32, b->k_spmv_t_primal_mixed_hot, sp::kWG,
+                                                               0) != hipSuccess)
+            b->hot_occ32 = 0;
         get(b->k_to_f32, "sp_f64_to_f32");
         get(b->k_delta_from, "sp_delta_from");
         get(b->k_delta_to, "sp_delta_to");
