@@ -556,6 +556,7 @@ void Hip::permute_rows(sp_problem* prob, const std::vector<int64_t>& order) {
     const int m = A.nrows;
     if ((int64_t)order.size() != m) throw Error(SP_ERR_SHAPE, "permute_rows: order has the wrong length");
     if (prob->has_fp32 || A.pmat) throw Error(SP_ERR_INVALID, "permute_rows: after FP32 copies or panels");
+    if (m == 0) return;  // nothing to relabel
     std::vector<int> ord32((size_t)m), rank((size_t)m, -1);
     for (int i = 0; i < m; ++i) {
         const int64_t o = order[(size_t)i];

@@ -237,7 +237,7 @@ def edge_cases():
 def test_edge_case_lps_match_the_oracle(gpu, name):
     p = edge_cases()[name]
     c, cx, _ = O.solve(p, O.default_params(tolerance=TOL))
-    for kw in (dict(), dict(precision=1), dict(gpu_scaling=0)):
+    for kw in (dict(), dict(precision=1), dict(gpu_scaling=0), dict(reorder_rows=1)):
         g = solve(gpu, p, default_params(tolerance=TOL, **kw))
         print(f"{name} {kw}: oracle {O.STATUS[c.status]} {c.iterations} it {c.objective:.9f}; "
               f"gpu {g.status} {g.iterations} it {g.objective:.9f}", flush=True)
