@@ -186,8 +186,9 @@ typedef struct {
                           workgroups in flight share an L2-sized slice of it (sp_problem_set_panels):
                           k in [8, 30]: panels of 2^k columns; 0: off; -1 (default): k = 16 when A' has
                           at least 2^21 columns (a gathered vector of 16 MB or more), else off. With
-                          panels the driver also relabels the rows, those long rows first and the
-                          others by their first column (y comes back in the LP's order). Changes the
+                          panels the driver also relabels the rows, those long rows first (longest
+                          first) and the others by their first column (y comes back in the LP's order;
+                          the relabeling of reorder_rows, whatever its setting). Changes the
                           order of the long rows' sums and of sums over rows only. The serving loop
                           runs without panels */
     int hot_table; /* the A'ᵀ-side plain-iteration kernel keeps y's first H entries in LDS (H = 2048
